@@ -1,0 +1,283 @@
+/* ispm.h -- C-ABI of the MI355X (gfx950) sliced-ELLPACK SpMV / pipelined-PCG library.
+ *
+ * Every entry point takes plain pointers and sizes; no HIP or C++ type crosses this
+ * boundary.  The C++ drop-in headers (formats/formats.h, plan/plan.h, fastainv/fastainv.h,
+ * util/cuda/sblas.h, util/cuda/initialize.h under ispm-sparse-lib_amd/) are thin inline
+ * wrappers over these functions, so a caller built against the reference's headers
+ * recompiles unchanged against ours.  Each declaration names the reference interface it
+ * replaces (paths relative to the reference tree).
+ *
+ * Conventions
+ *  - Return value: ISPM_OK (0) or an ISPM_ERR_* code; ispm_last_error() gives a message.
+ *  - Device pointers come from ispm_malloc (or any hipMalloc'd memory on the current device).
+ *  - All device work is enqueued on ONE library stream (ispm_get_stream / ispm_set_stream);
+ *    results are visible after ispm_synchronize() or an event on that stream.
+ *  - Scalar type codes: ISPM_F32, ISPM_F64, ISPM_BF16 (bf16 only as a matrix element type).
+ */
+#ifndef ISPM_H
+#define ISPM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum ispm_status {
+  ISPM_OK = 0,
+  ISPM_ERR_INVALID = 1,     /* bad argument */
+  ISPM_ERR_UNSUPPORTED = 2, /* valid but unsupported combination (reference: silent no-op) */
+  ISPM_ERR_RUNTIME = 3,     /* HIP runtime failure */
+  ISPM_ERR_NOMEM = 4,
+  ISPM_ERR_DEVICE = 5,      /* wrong / missing device (reference: arch-mismatch abort) */
+  ISPM_ERR_NOT_CONVERGED = 6
+};
+
+enum ispm_type { ISPM_F32 = 0, ISPM_F64 = 1, ISPM_BF16 = 2 };
+
+enum ispm_copy_kind { ISPM_H2D = 0, ISPM_D2H = 1, ISPM_D2D = 2, ISPM_DEFAULT = 3 };
+
+const char *ispm_last_error(void);
+
+/* ------------------------------------------------------------------ runtime ------------ */
+
+/* Replaces ispm_initialize() (util/cuda/initialize.h:4, initialize.cu:17-30): checks that the
+ * current device is gfx950 (the only ISA this library is built for) and creates the library
+ * stream.  Returns ISPM_ERR_DEVICE instead of aborting. */
+int ispm_init(void);
+
+/* Replaces get_device_string() (util/cuda/initialize.h:6-10, initialize.cu:32-42):
+ * "dddd:bb:dd (name)" into buf (at most len bytes, NUL-terminated). */
+int ispm_device_string(char *buf, int len);
+
+/* Replaces cuda_device_allocator::allocate/deallocate (util/cuda/allocator.h:14-25):
+ * device allocation, zero-filled like the reference. */
+int ispm_malloc(void **p, size_t bytes);
+int ispm_free(void *p);
+
+/* Replaces copy()/copy_async() (util/cuda/cudamemcpy.h:27-43).  kind: ispm_copy_kind. */
+int ispm_memcpy(void *dst, const void *src, size_t bytes, int kind);
+int ispm_memcpy_async(void *dst, const void *src, size_t bytes, int kind);
+/* Fill device memory with zero bytes, on the library stream (a kernel, not a memset node). */
+int ispm_zero(void *p, size_t bytes);
+
+/* Replaces cuda_pinned_allocator (util/cuda/allocator.h:44-56) and host2devmem
+ * (util/cuda/pinner.h:7-22): mapped pinned host memory and its device alias. */
+int ispm_host_alloc_mapped(void **hp, size_t bytes);
+int ispm_host_free(void *hp);
+int ispm_host_device_pointer(void **dp, void *hp);
+
+int ispm_synchronize(void);
+void *ispm_get_stream(void);            /* hipStream_t of the library stream */
+int ispm_set_stream(void *hip_stream);  /* NULL restores the library's own stream */
+
+/* Replaces cudaEventScoped / cudatimer (util/cuda/timer.h:6-52). Events record on the
+ * library stream. */
+int ispm_event_create(void **ev);
+int ispm_event_record(void *ev);
+int ispm_event_synchronize(void *ev);
+int ispm_event_elapsed_ms(void *start, void *stop, float *ms);
+int ispm_event_destroy(void *ev);
+
+/* ------------------------------------------------------------ SlELL SpMV --------------- */
+
+/* Device view of slell_matrix<E, device_memory_space_tag> (formats/formats.h:44-102). */
+typedef struct ispm_slell_desc {
+  int n_rows, n_cols, n_nz;
+  unsigned n_slices, slice_height, hblock;
+  int var_height, diags;
+  const unsigned *slice_ptr; /* (n_slices+1) records of P = 1+[hblock>1]+[V]+2[D] words,
+                                then the diagonal positions when diags */
+  size_t slice_ptr_len;
+  const unsigned *cols;
+  size_t cols_len;
+  const void *elms;          /* elm_type */
+  size_t elms_len;
+  int elm_type;
+} ispm_slell_desc;
+
+/* spmv_launch_params (cuda/spmv/kernel-iface.h:6-18).  N = threads per workgroup
+ * (64..1024, power of two), S = slice height the launch is specialised for, H = rows per
+ * thread (1 or 2).  Valid iff N >= S/H and S >= H.  use_tex_* are accepted and ignored:
+ * x windows are read through the cache hierarchy, never through texture units. */
+typedef struct ispm_launch {
+  unsigned N, S, H;
+  int use_tex_sliceptr, use_tex_x;
+} ispm_launch;
+
+/* Replaces launch_spmv_kernel<T,E> (cuda/spmv/kernel-iface.h:20-74, dispatch.h:184-205):
+ * y[0:n_rows) = A x.  (vec_type, A.elm_type) in {(F32,F32), (F64,F32), (F64,F64),
+ * (F32,BF16)}.  x holds n_cols entries (entries outside [0,n_cols) read as 0), y holds the
+ * padded n_rows.  Invalid launch parameters return ISPM_ERR_UNSUPPORTED and leave y
+ * unwritten (the reference returns silently, dispatch.h:202-203). */
+int ispm_spmv(const ispm_launch *p, const ispm_slell_desc *A, int vec_type,
+              const void *x, void *y);
+
+/* Multi-RHS: Y[:,j] = A X[:,j] for j < k (1..8), column j of X at X + j*ldx elements.
+ * One read of A serves all k vectors.  No reference counterpart (new in this library). */
+int ispm_spmm(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, int k,
+              const void *X, size_t ldx, void *Y, size_t ldy);
+
+/* --------------------------------------------------------------- sblas ---------------- */
+/* Replace sblas::{copy_indexed, add_indexed, dot, ppcg_update_scalars, ppcg_update_vectors}
+ * (util/cuda/sblas.h:1-25, sblas.cu:18-237), type ISPM_F32 or ISPM_F64.  All pointers are
+ * device pointers (scalars may be mapped host memory).  Dot products accumulate in FP64
+ * for both types. */
+int ispm_copy_indexed(int type, void *dst, const void *src, const int *idx, int n);
+int ispm_add_indexed(int type, void *dst, const void *src, const int *idx, int n);
+int ispm_dot(int type, const void *x, const void *y, void *s, int n);
+int ispm_ppcg_update_scalars(int type, void *alpha, void *beta, const void *gamma,
+                             const void *gammaold, const void *delta);
+int ispm_ppcg_update_vectors(int type, void *resnorm, void *gamma, void *delta,
+                             const void *alpha, const void *beta,
+                             const void *n, const void *m,
+                             void *p, void *s, void *q, void *z, void *x, void *r,
+                             void *u, void *w, int n_elts);
+
+/* ------------------------------------------------------ host formats (CPU) ------------ */
+
+/* csr_matrix<T> (formats/formats.h:18-37) as plain arrays. */
+typedef struct ispm_csr {
+  int n_rows, n_cols, n_nz;
+  const int *row_ptr;   /* n_rows+1 */
+  const int *cols;      /* n_nz */
+  const void *elms;     /* n_nz of elm_type */
+  int elm_type;
+} ispm_csr;
+
+/* slell_matrix<E, host_memory_space_tag> (formats/formats.h:44-102) owned by the library;
+ * release with ispm_slell_host_free. */
+typedef struct ispm_slell_host {
+  int n_rows, n_cols, n_nz;
+  unsigned n_slices, slice_height, hblock;
+  int var_height, diags;
+  unsigned *slice_ptr;
+  size_t slice_ptr_len;
+  unsigned *cols;
+  size_t cols_len;
+  void *elms;
+  size_t elms_len;
+  int elm_type;
+} ispm_slell_host;
+
+/* Replaces csr_to_slell (formats/conversion.h:338-414).  The reference pads the caller's
+ * CSR to a multiple of 512 rows in place (conversion.h:343); here A is read-only and the
+ * result's n_rows is the padded count.  out->elm_type = A->elm_type, or ISPM_BF16/ISPM_F32
+ * narrowing when out_elm_type differs (values rounded to nearest even). */
+int ispm_csr_to_slell(const ispm_csr *A, int slice_height, int hblock, int var_height,
+                      int diags, int min_height, int out_elm_type, ispm_slell_host *out);
+
+/* Replaces csr_to_slell_guess_format (formats/conversion.h:416-492). */
+int ispm_csr_to_slell_guess(const ispm_csr *A, int examine_ratio, int min_height,
+                            int out_elm_type, ispm_slell_host *out);
+void ispm_slell_host_free(ispm_slell_host *m);
+
+/* Copy a host SlELL to the device (slell_matrix<E,device>(host), formats.h:56-66).
+ * Release with ispm_slell_device_free. */
+int ispm_slell_upload(const ispm_slell_host *h, ispm_slell_desc *d);
+void ispm_slell_device_free(ispm_slell_desc *d);
+
+/* ------------------------------------------------------------- SpMV plan --------------- */
+
+enum ispm_plan_kind { ISPM_PLAN_DEFAULT = 0, ISPM_PLAN_ESTIMATE = 1, ISPM_PLAN_EXHAUSTIVE = 2 };
+enum ispm_plan_flags { ISPM_SPMV_NO_TEXTURE_MEMORY = 1, ISPM_SPMV_PLAN_NO_DEVICE = 2 };
+
+typedef struct ispm_plan ispm_plan;
+
+typedef struct ispm_plan_info {
+  unsigned S, hblock, V, D;    /* slell_params mparams (formats/formats.h:39-42) */
+  ispm_launch launch;          /* spmv_launch_params params */
+  double tuned_seconds;        /* best timed launch (exhaustive only) */
+  size_t stored_bytes;         /* 4|slice_ptr| + 4|cols| + sizeof(E)|elms| + sizeof(T)(n_cols+n_rows) */
+  size_t csr_bytes;            /* 4(n+1) + (4+sizeof(E))nnz + sizeof(T)*2n */
+  ispm_slell_desc device;      /* device matrix (valid unless NO_DEVICE) */
+} ispm_plan_info;
+
+/* Replaces spmv_plan<T,E>::spmv_plan(csr, kind, flags) (plan/plan.h:64-146):
+ * vec_type = T, A->elm_type (or elm_type override, e.g. ISPM_BF16) = E. */
+int ispm_plan_create(ispm_plan **out, int vec_type, int elm_type, const ispm_csr *A,
+                     int kind, unsigned flags);
+/* Plan from explicit parameters (no guessing / tuning). */
+int ispm_plan_create_explicit(ispm_plan **out, int vec_type, int elm_type, const ispm_csr *A,
+                              unsigned S, unsigned hblock, int V, int D,
+                              const ispm_launch *launch);
+int ispm_plan_info_get(const ispm_plan *p, ispm_plan_info *info);
+/* spmv_plan::execute_spmv (plan/plan.h:148-161). */
+int ispm_plan_execute(ispm_plan *p, const void *x, void *y);
+int ispm_plan_execute_multi(ispm_plan *p, int k, const void *X, size_t ldx, void *Y, size_t ldy);
+/* change_host_matrix + update_device_matrix_coefs (plan/plan.h:39-56): new values, same
+ * sparsity pattern; conversion reuses the plan's format parameters. */
+int ispm_plan_update_coefs(ispm_plan *p, const ispm_csr *A);
+void ispm_plan_destroy(ispm_plan *p);
+
+/* ------------------------------------------------------------- PCG solver -------------- */
+/* Device-resident FSAI-preconditioned CG.  No counterpart in the reference library: this is
+ * the PCG loop of its caller (foam-extend ispmCudaSolvers, README.md:3-9 of the reference),
+ * built from the reference's primitives (execute_spmv, sblas::ppcg_update_scalars,
+ * sblas::ppcg_update_vectors) plus fused kernels.
+ *   method ISPM_SOLVER_PPCG: Ghysels-Vanroose pipelined PCG with the reference's recurrences
+ *     (util/cuda/sblas.cu:156-217); M = Ainv2*Ainv1 applied as two SpMVs.
+ *   method ISPM_SOLVER_PCG : Hestenes-Stiefel PCG.
+ *   precision ISPM_PREC_FP64 : every vector and scalar in FP64.
+ *   precision ISPM_PREC_MIXED: FP32 vectors, FP32/BF16 matrices, FP64 dot products and
+ *     scalars; x accumulated in FP64 and the residual replaced by the FP64 true residual
+ *     b - A x whenever the FP32 recursive residual has dropped by reliable_delta (PCG only).
+ * Convergence is always decided on the FP64 true residual ||b - A x||_2 / ||b||_2 <= tol.
+ * Iterations run in hipGraph-captured batches of check_every iterations; a device flag
+ * turns the rest of a batch into no-ops once the recursive residual meets the target. */
+enum ispm_solver_method { ISPM_SOLVER_PPCG = 0, ISPM_SOLVER_PCG = 1 };
+enum ispm_solver_precision { ISPM_PREC_FP64 = 0, ISPM_PREC_MIXED = 1 };
+
+typedef struct ispm_solver_opts {
+  int method, precision;
+  double tol;
+  int maxit;
+  int check_every;        /* iterations per graph replay / host check (default 16) */
+  int use_graph;          /* 1: capture batches in a hipGraph */
+  double reliable_delta;  /* mixed: FP32 residual reduction between FP64 replacements (0.1) */
+} ispm_solver_opts;
+
+typedef struct ispm_solve_stats {
+  int iterations, replacements, converged;
+  double relres;          /* FP64 true relative residual at exit */
+  double l1_resnorm;      /* sum |r_i| of the recursive residual (sblas.cu:202 semantics) */
+} ispm_solve_stats;
+
+typedef struct ispm_solver ispm_solver;
+
+/* A: plan of the system matrix (E = F64 for FP64, E = F32 for mixed; the mixed solver uses
+ * it both with FP32 and FP64 vectors, so its values must be those of A).  Ainv1 / Ainv2:
+ * plans of the factors (T = F64 for FP64, T = F32 for mixed), or both NULL for no
+ * preconditioner.  Plans must outlive the solver. */
+int ispm_solver_create(ispm_solver **out, ispm_plan *A, ispm_plan *Ainv1, ispm_plan *Ainv2,
+                       const ispm_solver_opts *opts);
+/* b, x: device FP64 vectors of the padded length; x is the initial guess and the result. */
+int ispm_solver_solve(ispm_solver *s, const double *b, double *x, ispm_solve_stats *stats);
+void ispm_solver_destroy(ispm_solver *s);
+
+/* ------------------------------------------------- approximate inverse (CPU) ---------- */
+
+/* Replaces fastainv_sym<in_real,out_real> (fastainv/fastainv.h:8-30, fastainv.cpp:395-566):
+ * AINV-type factorisation with M = Ainv2 * Ainv1 ~= A^-1.  in type ISPM_F64, out type
+ * ISPM_F32 or ISPM_F64.  progress may be NULL.  Outputs are library-owned CSR arrays;
+ * release with ispm_csr_out_free.  Returns ISPM_ERR_INVALID for n <= 0 (reference: false). */
+typedef struct ispm_csr_out {
+  int n_rows, n_cols, n_nz;
+  int *row_ptr;
+  int *cols;
+  void *elms;
+  int elm_type;
+} ispm_csr_out;
+
+typedef void (*ispm_progress_fn)(void *);
+
+int ispm_fastainv_sym(int out_type, const ispm_csr *A, ispm_progress_fn progress, void *v,
+                      long long max_elts, double droptol,
+                      ispm_csr_out *Ainv1, ispm_csr_out *Ainv2);
+void ispm_csr_out_free(ispm_csr_out *m);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ISPM_H */

@@ -1,0 +1,111 @@
+/* device.h -- device-side helpers shared by the HIP translation units: element loads,
+ * wave64 reductions on ds_swizzle / readlane, and the last-block-finishes reduction. */
+#ifndef ISPM_DEVICE_H
+#define ISPM_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ispm {
+namespace dev {
+
+typedef uint16_t bf16_t;
+
+/* ---- element conversion: stored matrix element E -> accumulation type T ---- */
+template<typename T> __device__ __forceinline__ T cvt(float v) { return (T)v; }
+template<typename T> __device__ __forceinline__ T cvt_d(double v) { return (T)v; }
+
+template<typename T, typename E> struct elt;
+template<typename T> struct elt<T, float> {
+  static __device__ __forceinline__ T get(const float *p, unsigned i) { return (T)p[i]; }
+  static __device__ __forceinline__ void get2(const float *p, unsigned i, T &a, T &b)
+  {
+    float2 v = *reinterpret_cast<const float2 *>(p + i);
+    a = (T)v.x; b = (T)v.y;
+  }
+};
+template<typename T> struct elt<T, double> {
+  static __device__ __forceinline__ T get(const double *p, unsigned i) { return (T)p[i]; }
+  static __device__ __forceinline__ void get2(const double *p, unsigned i, T &a, T &b)
+  {
+    double2 v = *reinterpret_cast<const double2 *>(p + i);
+    a = (T)v.x; b = (T)v.y;
+  }
+};
+__device__ __forceinline__ float bf16_to_f32(unsigned short u) { return __uint_as_float((unsigned)u << 16); }
+template<typename T> struct elt<T, bf16_t> {
+  static __device__ __forceinline__ T get(const bf16_t *p, unsigned i) { return (T)bf16_to_f32(p[i]); }
+  static __device__ __forceinline__ void get2(const bf16_t *p, unsigned i, T &a, T &b)
+  {
+    unsigned v = *reinterpret_cast<const unsigned *>(p + i);
+    a = (T)__uint_as_float(v << 16);
+    b = (T)__uint_as_float(v & 0xffff0000u);
+  }
+};
+
+/* ---- wave64 reductions ---- */
+/* ds_swizzle in bit mode: lane l reads lane (l ^ m) within its 32-lane half. */
+template<int M> __device__ __forceinline__ float swz_xor(float v)
+{
+  return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x1f | (M << 10)));
+}
+template<int M> __device__ __forceinline__ double swz_xor(double v)
+{
+  long long b = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_ds_swizzle((int)(b & 0xffffffff), 0x1f | (M << 10));
+  int hi = __builtin_amdgcn_ds_swizzle((int)(b >> 32), 0x1f | (M << 10));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double readlane(double v, int l)
+{
+  long long b = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), l);
+  int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float readlane(float v, int l)
+{
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+/* Sum over the 64 lanes of a wave; the result is wave-uniform.  Butterfly in each 32-lane
+ * half with ds_swizzle, then the two halves combined through readlane.  The summation
+ * order is fixed, so the result is deterministic. */
+template<typename T> __device__ __forceinline__ T wave_sum(T v)
+{
+  v += swz_xor<1>(v);
+  v += swz_xor<2>(v);
+  v += swz_xor<4>(v);
+  v += swz_xor<8>(v);
+  v += swz_xor<16>(v);
+  return readlane(v, 0) + readlane(v, 32);
+}
+
+/* Block-wide sum of NS values per thread (block of NT threads, NT a multiple of 64).
+ * Result valid in every thread.  lds must hold NS * NT/64 values of T. */
+template<int NT, int NS, typename T> __device__ __forceinline__ void block_sum(T (&v)[NS], T *lds)
+{
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int s = 0; s < NS; s++) v[s] = wave_sum(v[s]);
+  if (NT > 64) {
+    if (lane == 0) {
+#pragma unroll
+      for (int s = 0; s < NS; s++) lds[s * (NT / 64) + w] = v[s];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      T a = 0;
+#pragma unroll
+      for (int i = 0; i < NT / 64; i++) a += lds[s * (NT / 64) + i];
+      v[s] = a;
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace dev
+}  // namespace ispm
+
+#endif

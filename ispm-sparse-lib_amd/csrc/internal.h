@@ -1,0 +1,49 @@
+/* internal.h -- shared declarations of the library's translation units (not installed). */
+#ifndef ISPM_INTERNAL_H
+#define ISPM_INTERNAL_H
+
+#include <hip/hip_runtime_api.h>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include "ispm.h"
+
+namespace ispm {
+
+/* Record a message for ispm_last_error() and return code. */
+int fail(int code, const char *fmt, ...);
+
+/* Map a HIP status to ISPM_ERR_RUNTIME (with message) or ISPM_OK. */
+int hip_status(hipError_t e, const char *what);
+
+#define ISPM_HIP(call)                                              \
+  do {                                                              \
+    hipError_t e_ = (call);                                         \
+    if (e_ != hipSuccess) return ::ispm::hip_status(e_, #call);     \
+  } while (0)
+
+/* Library stream (created by ispm_init / lazily). */
+hipStream_t stream();
+
+/* Make sure the device is initialised (gfx950 check); returns ISPM_OK or an error. */
+int ensure_init();
+
+/* Per-device reduction scratch: partial sums and a completion counter used by the
+ * last-block-finishes reductions.  Allocated once per device at initialisation. */
+struct reduce_scratch {
+  double *partials;         /* kMaxBlocks * kMaxSums doubles */
+  unsigned *counter;        /* one completion counter per reduction slot */
+};
+constexpr int kMaxBlocks = 2048;
+constexpr int kMaxSums = 8;
+constexpr int kCounterSlots = 16;
+reduce_scratch &scratch();
+
+/* Number of compute units (cached). */
+int num_cus();
+
+size_t type_size(int t);
+
+}  // namespace ispm
+
+#endif

@@ -1,0 +1,484 @@
+/* plan.cpp -- C-ABI for host format conversion, device upload and SpMV plans.
+ *
+ * ispm_plan mirrors spmv_plan<T,E> (plan/plan.h:20-161 of the reference):
+ *  - DEFAULT / ESTIMATE: the format comes from the guess heuristic (conversion.h:416-492);
+ *    the launch shape is chosen for CDNA4 instead of the reference's fixed 512 threads:
+ *    one row per thread and one wave (64 lanes) per 64 rows, so that every plane load of
+ *    a wave is a single 512-byte (FP64) run and no cross-wave reduction is needed.
+ *  - EXHAUSTIVE: the reference's search over S in {32..512}, V, D, hblock (plan.h:97-144)
+ *    with the launch sweep retargeted to wave64 workgroups N in {64..1024}, H in {1,2};
+ *    each candidate is timed as the median of several launches on rotating vectors, not one
+ *    launch (plan.h:120-124), and the 65535-slice grid limit of sm_35 does not apply.
+ *  - update_coefs: new values for the same pattern are written through the CSR-entry ->
+ *    slot map recorded at conversion (replaces change_host_matrix + update_device_matrix_coefs,
+ *    plan.h:39-56, without re-converting).
+ */
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include "convert.h"
+#include "ispm.h"
+#include "plan_internal.h"
+
+namespace ispm {
+
+static int fail_plain(int code, const char *msg);
+
+size_t tsize(int t) { return t == ISPM_F64 ? 8 : t == ISPM_F32 ? 4 : t == ISPM_BF16 ? 2 : 0; }
+
+static double csr_value(const ispm_csr *A, long long k)
+{
+  switch (A->elm_type) {
+    case ISPM_F64: return static_cast<const double *>(A->elms)[k];
+    case ISPM_F32: return static_cast<const float *>(A->elms)[k];
+    default: {
+      unsigned short u = static_cast<const unsigned short *>(A->elms)[k];
+      unsigned v = (unsigned)u << 16;
+      float f;
+      std::memcpy(&f, &v, 4);
+      return f;
+    }
+  }
+}
+
+unsigned short f32_to_bf16(float f)
+{
+  unsigned u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (unsigned short)((u >> 16) | 0x40);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+static void store_value(void *dst, int et, size_t slot, double v)
+{
+  switch (et) {
+    case ISPM_F64: static_cast<double *>(dst)[slot] = v; break;
+    case ISPM_F32: static_cast<float *>(dst)[slot] = (float)v; break;
+    default: static_cast<unsigned short *>(dst)[slot] = f32_to_bf16((float)v); break;
+  }
+}
+
+int fill_values(const conv::layout &L, const ispm_csr *A, int et, void *elms)
+{
+  const long long nnz = A->row_ptr[A->n_rows];
+  std::memset(elms, 0, L.n_elms * tsize(et));
+  for (long long k = 0; k < nnz; k++) {
+    long long s = L.map[k];
+    if (s < 0) return -1;
+    store_value(elms, et, (size_t)s, csr_value(A, k));
+  }
+  return 0;
+}
+
+static int check_csr(const ispm_csr *A)
+{
+  if (!A || !A->row_ptr || (!A->cols && A->row_ptr[A->n_rows] > 0) || (!A->elms && A->row_ptr[A->n_rows] > 0))
+    return fail_plain(ISPM_ERR_INVALID, "csr: null array");
+  if (A->n_rows <= 0 || A->n_cols <= 0) return fail_plain(ISPM_ERR_INVALID, "csr: empty matrix");
+  if (A->elm_type != ISPM_F64 && A->elm_type != ISPM_F32 && A->elm_type != ISPM_BF16)
+    return fail_plain(ISPM_ERR_INVALID, "csr: bad element type");
+  return ISPM_OK;
+}
+
+static int to_host(const conv::layout &L, const ispm_csr *A, int et, ispm_slell_host *out)
+{
+  std::memset(out, 0, sizeof *out);
+  out->n_rows = L.n_rows;
+  out->n_cols = L.n_cols;
+  out->n_nz = L.n_nz;
+  out->n_slices = L.n_slices;
+  out->slice_height = L.S;
+  out->hblock = L.W;
+  out->var_height = L.V;
+  out->diags = L.D;
+  out->elm_type = et;
+  out->slice_ptr_len = L.sp.size();
+  out->cols_len = L.cols.size();
+  out->elms_len = L.n_elms;
+  out->slice_ptr = (unsigned *)std::malloc(sizeof(unsigned) * (L.sp.size() ? L.sp.size() : 1));
+  out->cols = (unsigned *)std::malloc(sizeof(unsigned) * (L.cols.size() ? L.cols.size() : 1));
+  out->elms = std::malloc(tsize(et) * (L.n_elms ? L.n_elms : 1));
+  if (!out->slice_ptr || !out->cols || !out->elms) return fail_plain(ISPM_ERR_NOMEM, "host allocation failed");
+  std::copy(L.sp.begin(), L.sp.end(), out->slice_ptr);
+  std::copy(L.cols.begin(), L.cols.end(), out->cols);
+  fill_values(L, A, et, out->elms);
+  return ISPM_OK;
+}
+
+int upload_layout(const conv::layout &L, const void *elms, int et, ispm_slell_desc *d)
+{
+  std::memset(d, 0, sizeof *d);
+  d->n_rows = L.n_rows;
+  d->n_cols = L.n_cols;
+  d->n_nz = L.n_nz;
+  d->n_slices = L.n_slices;
+  d->slice_height = L.S;
+  d->hblock = L.W;
+  d->var_height = L.V;
+  d->diags = L.D;
+  d->elm_type = et;
+  d->slice_ptr_len = L.sp.size();
+  d->cols_len = L.cols.size();
+  d->elms_len = L.n_elms;
+  void *sp = nullptr, *cl = nullptr, *el = nullptr;
+  int st;
+  if ((st = ispm_malloc(&sp, sizeof(unsigned) * L.sp.size()))) return st;
+  if ((st = ispm_malloc(&cl, sizeof(unsigned) * (L.cols.size() ? L.cols.size() : 1)))) return st;
+  if ((st = ispm_malloc(&el, tsize(et) * (L.n_elms ? L.n_elms : 1)))) return st;
+  if ((st = ispm_memcpy(sp, L.sp.data(), sizeof(unsigned) * L.sp.size(), ISPM_H2D))) return st;
+  if (!L.cols.empty() && (st = ispm_memcpy(cl, L.cols.data(), sizeof(unsigned) * L.cols.size(), ISPM_H2D))) return st;
+  if (L.n_elms && (st = ispm_memcpy(el, elms, tsize(et) * L.n_elms, ISPM_H2D))) return st;
+  d->slice_ptr = (const unsigned *)sp;
+  d->cols = (const unsigned *)cl;
+  d->elms = el;
+  return ISPM_OK;
+}
+
+static int conv_status(int r)
+{
+  if (r == 0) return ISPM_OK;
+  if (r == -3) return fail_plain(ISPM_ERR_UNSUPPORTED, "conversion: more than 2^32 stored elements");
+  return fail_plain(ISPM_ERR_INVALID, "conversion failed");
+}
+
+/* ---------------------------------------------------------------- plans ------------- */
+
+size_t stored_bytes(const ispm_slell_desc &d, int vt)
+{
+  return 4 * d.slice_ptr_len + 4 * d.cols_len + tsize(d.elm_type) * d.elms_len +
+         tsize(vt) * ((size_t)d.n_cols + (size_t)d.n_rows);
+}
+
+size_t csr_bytes(const ispm_csr *A, int et, int vt)
+{
+  const size_t nnz = (size_t)A->row_ptr[A->n_rows];
+  return 4 * ((size_t)A->n_rows + 1) + (4 + tsize(et)) * nnz + tsize(vt) * 2 * (size_t)A->n_rows;
+}
+
+/* CDNA4 default launch: one row per thread, wave-multiple workgroups covering a slice. */
+ispm_launch default_launch(const conv::layout &L)
+{
+  ispm_launch p;
+  p.H = 1;
+  p.use_tex_sliceptr = p.use_tex_x = 0;
+  if (L.V) {
+    p.N = 256;
+    p.S = 256;
+  } else {
+    unsigned S = L.S;
+    p.N = std::min(1024u, std::max(64u, S));
+    p.S = S;
+  }
+  return p;
+}
+
+static int make_layout(const ispm_csr *A, int et, unsigned S, unsigned W, int V, int D,
+                       conv::layout &L)
+{
+  int r = conv::convert(A->row_ptr, A->n_rows, A->cols, A->n_cols, A->n_nz, (int)tsize(et),
+                        (int)S, (int)W, V != 0, D != 0, 4, L);
+  return conv_status(r);
+}
+
+static int finish_plan(ispm_plan *p, const ispm_csr *A, unsigned flags)
+{
+  p->elms.assign(p->L.n_elms * tsize(p->E) + 1, 0);
+  if (fill_values(p->L, A, p->E, p->elms.data())) return fail_plain(ISPM_ERR_INVALID, "value map incomplete");
+  p->stored = 0;
+  p->csr = csr_bytes(A, p->E, p->T);
+  if (!(flags & ISPM_SPMV_PLAN_NO_DEVICE)) {
+    int st = upload_layout(p->L, p->elms.data(), p->E, &p->dev);
+    if (st) return st;
+    p->has_dev = true;
+  } else {
+    std::memset(&p->dev, 0, sizeof p->dev);
+    p->dev.n_rows = p->L.n_rows;
+    p->dev.n_cols = p->L.n_cols;
+    p->dev.slice_ptr_len = p->L.sp.size();
+    p->dev.cols_len = p->L.cols.size();
+    p->dev.elms_len = p->L.n_elms;
+    p->dev.elm_type = p->E;
+  }
+  p->stored = stored_bytes(p->dev, p->T);
+  return ISPM_OK;
+}
+
+static int time_candidate(int vt, const ispm_slell_desc &d, const ispm_launch &lp, void *x,
+                          void *y, void *x2, void *y2, double &sec)
+{
+  void *e0 = nullptr, *e1 = nullptr;
+  int st;
+  if ((st = ispm_spmv(&lp, &d, vt, x, y))) return st;   /* warm-up; also validity */
+  if ((st = ispm_event_create(&e0)) || (st = ispm_event_create(&e1))) return st;
+  const int reps = 5;
+  std::vector<float> t;
+  for (int r = 0; r < reps; r++) {
+    ispm_event_record(e0);
+    ispm_spmv(&lp, &d, vt, (r & 1) ? x2 : x, (r & 1) ? y2 : y);
+    ispm_event_record(e1);
+    ispm_event_synchronize(e1);
+    float ms = 0;
+    ispm_event_elapsed_ms(e0, e1, &ms);
+    t.push_back(ms);
+  }
+  ispm_event_destroy(e0);
+  ispm_event_destroy(e1);
+  std::sort(t.begin(), t.end());
+  sec = 1e-3 * t[reps / 2];
+  return ISPM_OK;
+}
+
+int tune_exhaustive(const ispm_csr *A, int vt, int et, unsigned &bS, unsigned &bW, int &bV,
+                    int &bD, ispm_launch &bl, double &best)
+{
+  best = INFINITY;
+  const size_t nvec = ((size_t)A->n_rows + 511) / 512 * 512;
+  const size_t ncols = std::max(nvec, (size_t)A->n_cols);
+  void *x = nullptr, *y = nullptr, *x2 = nullptr, *y2 = nullptr;
+  int st;
+  if ((st = ispm_malloc(&x, tsize(vt) * ncols)) || (st = ispm_malloc(&y, tsize(vt) * nvec)) ||
+      (st = ispm_malloc(&x2, tsize(vt) * ncols)) || (st = ispm_malloc(&y2, tsize(vt) * nvec)))
+    return st;
+  for (unsigned S = 32; S <= 512; S <<= 1)
+    for (int V = 0; V <= 1; V++)
+      for (int D = 0; D <= 1; D++)
+        for (unsigned W = 1; W <= (D ? 1u : 2u); W <<= 1) {
+          conv::layout L;
+          if (make_layout(A, et, S, W, V, D, L)) continue;
+          std::vector<unsigned char> el(L.n_elms * tsize(et) + 1);
+          fill_values(L, A, et, el.data());
+          ispm_slell_desc d;
+          if ((st = upload_layout(L, el.data(), et, &d))) return st;
+          for (unsigned N = 64; N <= 1024; N <<= 1)
+            for (unsigned H = 1; H <= 2; H <<= 1) {
+              ispm_launch lp{N, V ? N : S, H, 0, 0};
+              if (!(lp.N >= lp.S / lp.H && lp.S >= lp.H)) continue;
+              double sec;
+              if (time_candidate(vt, d, lp, x, y, x2, y2, sec)) continue;
+              if (sec < best) {
+                best = sec;
+                bS = S; bW = W; bV = V; bD = D;
+                bl = lp;
+              }
+            }
+          ispm_slell_device_free(&d);
+        }
+  ispm_free(x); ispm_free(y); ispm_free(x2); ispm_free(y2);
+  if (!std::isfinite(best)) return fail_plain(ISPM_ERR_UNSUPPORTED, "exhaustive plan: no candidate ran");
+  return ISPM_OK;
+}
+
+}  // namespace ispm
+
+using namespace ispm;
+
+namespace ispm {
+extern "C" const char *ispm_last_error(void);
+/* The runtime unit keeps the thread-local message; reuse its formatter through a C shim. */
+int set_error_message(int code, const char *msg);
+static int fail_plain(int code, const char *msg) { return set_error_message(code, msg); }
+}
+
+extern "C" {
+
+int ispm_csr_to_slell(const ispm_csr *A, int S, int hblock, int var_height, int diags,
+                      int min_height, int out_elm_type, ispm_slell_host *out)
+{
+  int st = check_csr(A);
+  if (st) return st;
+  if (!out) return fail_plain(ISPM_ERR_INVALID, "csr_to_slell: null output");
+  if (S <= 0 || hblock <= 0) return fail_plain(ISPM_ERR_INVALID, "csr_to_slell: bad slice height / hblock");
+  int et = out_elm_type < 0 ? A->elm_type : out_elm_type;
+  if (!tsize(et)) return fail_plain(ISPM_ERR_INVALID, "csr_to_slell: bad output type");
+  conv::layout L;
+  st = conv_status(conv::convert(A->row_ptr, A->n_rows, A->cols, A->n_cols, A->n_nz, (int)tsize(et),
+                                 S, hblock, var_height != 0, diags != 0, min_height, L));
+  if (st) return st;
+  return to_host(L, A, et, out);
+}
+
+int ispm_csr_to_slell_guess(const ispm_csr *A, int examine_ratio, int min_height,
+                            int out_elm_type, ispm_slell_host *out)
+{
+  int st = check_csr(A);
+  if (st) return st;
+  if (!out) return fail_plain(ISPM_ERR_INVALID, "csr_to_slell_guess: null output");
+  int et = out_elm_type < 0 ? A->elm_type : out_elm_type;
+  if (!tsize(et)) return fail_plain(ISPM_ERR_INVALID, "csr_to_slell_guess: bad output type");
+  conv::layout L;
+  st = conv_status(conv::guess(A->row_ptr, A->n_rows, A->cols, A->n_cols, A->n_nz, (int)tsize(et),
+                               examine_ratio, min_height, L, nullptr));
+  if (st) return st;
+  return to_host(L, A, et, out);
+}
+
+void ispm_slell_host_free(ispm_slell_host *m)
+{
+  if (!m) return;
+  std::free(m->slice_ptr);
+  std::free(m->cols);
+  std::free(m->elms);
+  m->slice_ptr = nullptr;
+  m->cols = nullptr;
+  m->elms = nullptr;
+}
+
+int ispm_slell_upload(const ispm_slell_host *h, ispm_slell_desc *d)
+{
+  if (!h || !d) return fail_plain(ISPM_ERR_INVALID, "slell_upload: null argument");
+  std::memset(d, 0, sizeof *d);
+  void *sp = nullptr, *cl = nullptr, *el = nullptr;
+  const size_t es = tsize(h->elm_type);
+  int st;
+  if ((st = ispm_malloc(&sp, 4 * (h->slice_ptr_len ? h->slice_ptr_len : 1)))) return st;
+  if ((st = ispm_malloc(&cl, 4 * (h->cols_len ? h->cols_len : 1)))) return st;
+  if ((st = ispm_malloc(&el, es * (h->elms_len ? h->elms_len : 1)))) return st;
+  if (h->slice_ptr_len && (st = ispm_memcpy(sp, h->slice_ptr, 4 * h->slice_ptr_len, ISPM_H2D))) return st;
+  if (h->cols_len && (st = ispm_memcpy(cl, h->cols, 4 * h->cols_len, ISPM_H2D))) return st;
+  if (h->elms_len && (st = ispm_memcpy(el, h->elms, es * h->elms_len, ISPM_H2D))) return st;
+  d->n_rows = h->n_rows;
+  d->n_cols = h->n_cols;
+  d->n_nz = h->n_nz;
+  d->n_slices = h->n_slices;
+  d->slice_height = h->slice_height;
+  d->hblock = h->hblock;
+  d->var_height = h->var_height;
+  d->diags = h->diags;
+  d->slice_ptr = (const unsigned *)sp;
+  d->slice_ptr_len = h->slice_ptr_len;
+  d->cols = (const unsigned *)cl;
+  d->cols_len = h->cols_len;
+  d->elms = el;
+  d->elms_len = h->elms_len;
+  d->elm_type = h->elm_type;
+  return ISPM_OK;
+}
+
+void ispm_slell_device_free(ispm_slell_desc *d)
+{
+  if (!d) return;
+  ispm_free((void *)d->slice_ptr);
+  ispm_free((void *)d->cols);
+  ispm_free((void *)d->elms);
+  d->slice_ptr = nullptr;
+  d->cols = nullptr;
+  d->elms = nullptr;
+}
+
+int ispm_plan_create(ispm_plan **out, int vt, int et, const ispm_csr *A, int kind, unsigned flags)
+{
+  if (!out) return fail_plain(ISPM_ERR_INVALID, "plan_create: null output");
+  *out = nullptr;
+  int st = check_csr(A);
+  if (st) return st;
+  if (et < 0) et = A->elm_type;
+  if (!((vt == ISPM_F64 && (et == ISPM_F64 || et == ISPM_F32)) || (vt == ISPM_F32 && (et == ISPM_F32 || et == ISPM_BF16))))
+    return fail_plain(ISPM_ERR_UNSUPPORTED, "plan_create: unsupported (T,E) pair");
+  ispm_plan *p = new ispm_plan;
+  p->T = vt;
+  p->E = et;
+  if (kind == ISPM_PLAN_EXHAUSTIVE) {
+    if (flags & ISPM_SPMV_PLAN_NO_DEVICE) { delete p; return fail_plain(ISPM_ERR_INVALID, "exhaustive plan needs the device"); }
+    unsigned S = 0, W = 1;
+    int V = 0, D = 0;
+    ispm_launch lp;
+    st = tune_exhaustive(A, vt, et, S, W, V, D, lp, p->tuned);
+    if (st) { delete p; return st; }
+    st = make_layout(A, et, S, W, V, D, p->L);
+    if (st) { delete p; return st; }
+    p->launch = lp;
+  } else {
+    conv::guess_info gi;
+    st = conv_status(conv::guess(A->row_ptr, A->n_rows, A->cols, A->n_cols, A->n_nz, (int)tsize(et),
+                                 1, 4, p->L, &gi));
+    if (st) { delete p; return st; }
+    p->launch = default_launch(p->L);
+  }
+  st = finish_plan(p, A, flags);
+  if (st) { ispm_plan_destroy(p); return st; }
+  if (kind == ISPM_PLAN_EXHAUSTIVE && std::getenv("SPMV_TUNE_RESULT")) {
+    /* plan.h:137-142 */
+    size_t eff = 4 * p->L.sp.size() + (4 + tsize(et)) * (size_t)A->n_nz + tsize(vt) * 2 * (size_t)p->L.n_rows;
+    std::fprintf(stderr, "[spmv tuning] %g ms:  %g (%g eff, %g csr-equiv) GB/s\n", 1e3 * p->tuned,
+                 1e-9 * p->stored / p->tuned, 1e-9 * eff / p->tuned, 1e-9 * p->csr / p->tuned);
+  }
+  *out = p;
+  return ISPM_OK;
+}
+
+int ispm_plan_create_explicit(ispm_plan **out, int vt, int et, const ispm_csr *A, unsigned S,
+                              unsigned hblock, int V, int D, const ispm_launch *launch)
+{
+  if (!out) return fail_plain(ISPM_ERR_INVALID, "plan_create_explicit: null output");
+  *out = nullptr;
+  int st = check_csr(A);
+  if (st) return st;
+  if (et < 0) et = A->elm_type;
+  if (!((vt == ISPM_F64 && (et == ISPM_F64 || et == ISPM_F32)) || (vt == ISPM_F32 && (et == ISPM_F32 || et == ISPM_BF16))))
+    return fail_plain(ISPM_ERR_UNSUPPORTED, "plan_create_explicit: unsupported (T,E) pair");
+  ispm_plan *p = new ispm_plan;
+  p->T = vt;
+  p->E = et;
+  st = make_layout(A, et, S, hblock, V, D, p->L);
+  if (st) { delete p; return st; }
+  p->launch = launch ? *launch : default_launch(p->L);
+  st = finish_plan(p, A, 0);
+  if (st) { ispm_plan_destroy(p); return st; }
+  *out = p;
+  return ISPM_OK;
+}
+
+int ispm_plan_info_get(const ispm_plan *p, ispm_plan_info *info)
+{
+  if (!p || !info) return fail_plain(ISPM_ERR_INVALID, "plan_info: null argument");
+  info->S = p->L.S;
+  info->hblock = p->L.W;
+  info->V = p->L.V;
+  info->D = p->L.D;
+  info->launch = p->launch;
+  info->tuned_seconds = p->tuned;
+  info->stored_bytes = p->stored;
+  info->csr_bytes = p->csr;
+  info->device = p->dev;
+  return ISPM_OK;
+}
+
+int ispm_plan_execute(ispm_plan *p, const void *x, void *y)
+{
+  if (!p) return fail_plain(ISPM_ERR_INVALID, "plan_execute: null plan");
+  if (!p->has_dev) return fail_plain(ISPM_ERR_INVALID, "plan_execute: plan has no device matrix");
+  return ispm_spmv(&p->launch, &p->dev, p->T, x, y);
+}
+
+int ispm_plan_execute_multi(ispm_plan *p, int k, const void *X, size_t ldx, void *Y, size_t ldy)
+{
+  if (!p) return fail_plain(ISPM_ERR_INVALID, "plan_execute_multi: null plan");
+  if (!p->has_dev) return fail_plain(ISPM_ERR_INVALID, "plan_execute_multi: plan has no device matrix");
+  return ispm_spmm(&p->launch, &p->dev, p->T, k, X, ldx, Y, ldy);
+}
+
+int ispm_plan_update_coefs(ispm_plan *p, const ispm_csr *A)
+{
+  if (!p) return fail_plain(ISPM_ERR_INVALID, "plan_update_coefs: null plan");
+  int st = check_csr(A);
+  if (st) return st;
+  if ((long long)A->row_ptr[A->n_rows] != (long long)p->L.map.size())
+    return fail_plain(ISPM_ERR_INVALID, "plan_update_coefs: pattern size differs from the plan's");
+  if (fill_values(p->L, A, p->E, p->elms.data())) return fail_plain(ISPM_ERR_INVALID, "value map incomplete");
+  if (p->has_dev && p->L.n_elms)
+    return ispm_memcpy((void *)p->dev.elms, p->elms.data(), tsize(p->E) * p->L.n_elms, ISPM_H2D);
+  return ISPM_OK;
+}
+
+void ispm_plan_destroy(ispm_plan *p)
+{
+  if (!p) return;
+  if (p->has_dev) ispm_slell_device_free(&p->dev);
+  delete p;
+}
+
+}  // extern "C"

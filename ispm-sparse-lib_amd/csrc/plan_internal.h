@@ -1,0 +1,28 @@
+/* plan_internal.h -- the SpMV plan object behind the opaque ispm_plan handle. */
+#ifndef ISPM_PLAN_INTERNAL_H
+#define ISPM_PLAN_INTERNAL_H
+
+#include <vector>
+#include "convert.h"
+#include "ispm.h"
+
+struct ispm_plan {
+  int T = ISPM_F64, E = ISPM_F64;         /* vector and element types */
+  ispm::conv::layout L;                   /* layout + CSR-entry -> slot map */
+  std::vector<unsigned char> elms;        /* host copy of the element values */
+  ispm_slell_desc dev{};
+  bool has_dev = false;
+  ispm_launch launch{};
+  double tuned = 0;
+  size_t stored = 0, csr = 0;
+};
+
+namespace ispm {
+size_t tsize(int t);
+unsigned short f32_to_bf16(float f);
+int fill_values(const conv::layout &L, const ispm_csr *A, int et, void *elms);
+int upload_layout(const conv::layout &L, const void *elms, int et, ispm_slell_desc *d);
+ispm_launch default_launch(const conv::layout &L);
+}
+
+#endif

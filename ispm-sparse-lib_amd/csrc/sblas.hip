@@ -1,0 +1,220 @@
+/* sblas.hip -- pipelined-PCG vector kernels (replaces util/cuda/sblas.cu:18-237).
+ *
+ * Differences in mechanism from the reference, not in results:
+ *  - reductions are single-pass: each block reduces with wave64 ds_swizzle butterflies
+ *    (device.h) and the last block to finish (acq_rel counter) sums the block partials in
+ *    block order, so there is no second launch and the result is deterministic;
+ *  - dot products and the three recurrence sums accumulate in FP64 for both FP32 and
+ *    FP64 vectors;
+ *  - the fused update reads 10 and writes 8 vectors in one pass with 16-byte accesses.
+ */
+#include <hip/hip_runtime.h>
+#include "device.h"
+#include "internal.h"
+#include "sblas_kernels.h"
+
+namespace ispm {
+namespace {
+
+template<typename T>
+__global__ void __launch_bounds__(256) fill_zero_kernel(unsigned char *p, size_t bytes)
+{
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * 256;
+  size_t nv = bytes / 16;
+  uint4 *q = reinterpret_cast<uint4 *>(p);
+  for (size_t k = i; k < nv; k += stride) q[k] = make_uint4(0, 0, 0, 0);
+  for (size_t k = nv * 16 + i; k < bytes; k += stride) p[k] = 0;
+}
+
+template<typename T>
+__global__ void __launch_bounds__(256) copy_indexed_kernel(T *dst, const T *src, const int *idx, int n)
+{
+  int k = blockIdx.x * 256 + threadIdx.x;
+  if (k < n) dst[k] = src[idx[k]];
+}
+
+template<typename T>
+__global__ void __launch_bounds__(256) add_indexed_kernel(T *dst, const T *src, const int *idx, int n)
+{
+  int k = blockIdx.x * 256 + threadIdx.x;
+  if (k < n) dst[idx[k]] += src[k];   /* non-atomic, like sblas.cu:42: indices must be distinct */
+}
+
+template<typename T>
+__global__ void ppcg_scalars_kernel(T *alpha, T *beta, const T *gamma, const T *gammaold,
+                                    const T *delta)
+{
+  double g = (double)*gamma;
+  double b = g / (double)*gammaold;
+  *beta = (T)b;
+  *alpha = (T)(g / ((double)*delta - b * g / (double)*alpha));
+}
+
+template<typename T, bool VEC>
+__global__ void __launch_bounds__(256) dot_kernel(const T *x, const T *y, T *s, int n,
+                                                  double *parts, unsigned *counter)
+{
+  __shared__ double lds[kMaxSums * 4];
+  double v[1] = {0.0};
+  const int gt = blockIdx.x * 256 + threadIdx.x, gs = gridDim.x * 256;
+  if (VEC) {
+    typedef typename vec16<T>::type V16;
+    constexpr int L = vec16<T>::n;
+    const int nv = n / L;
+    const V16 *xv = reinterpret_cast<const V16 *>(x), *yv = reinterpret_cast<const V16 *>(y);
+    for (int i = gt; i < nv; i += gs) {
+      V16 a = xv[i], b = yv[i];
+      v[0] += vec16<T>::dot(a, b);
+    }
+    for (int i = nv * L + gt; i < n; i += gs) v[0] += (double)x[i] * (double)y[i];
+  } else {
+    for (int i = gt; i < n; i += gs) v[0] += (double)x[i] * (double)y[i];
+  }
+  double out[1];
+  if (finish_reduction<1>(v, parts, counter, lds, out)) {
+    if (threadIdx.x == 0) *s = (T)out[0];
+  }
+}
+
+}  // namespace
+
+/* Launch of the fused recurrence update on the library stream (also used by the solver). */
+int launch_ppcg_update(int type, bool vec, int grid, void *resnorm, void *gamma, void *delta,
+                       double *r2, const void *alpha, const void *beta, const void *n,
+                       const void *m, void *p, void *s, void *q, void *z, void *x, void *r,
+                       void *u, void *w, int n_elts, unsigned *counter)
+{
+  double *parts = scratch().partials;
+  if (type == ISPM_F64) {
+    typedef double T;
+    if (vec) hipLaunchKernelGGL((ppcg_update_kernel<T, true>), dim3(grid), dim3(256), 0, stream(), (T *)resnorm, (T *)gamma, (T *)delta, r2, (const T *)alpha, (const T *)beta, (const T *)n, (const T *)m, (T *)p, (T *)s, (T *)q, (T *)z, (T *)x, (T *)r, (T *)u, (T *)w, n_elts, parts, counter);
+    else     hipLaunchKernelGGL((ppcg_update_kernel<T, false>), dim3(grid), dim3(256), 0, stream(), (T *)resnorm, (T *)gamma, (T *)delta, r2, (const T *)alpha, (const T *)beta, (const T *)n, (const T *)m, (T *)p, (T *)s, (T *)q, (T *)z, (T *)x, (T *)r, (T *)u, (T *)w, n_elts, parts, counter);
+  } else {
+    typedef float T;
+    if (vec) hipLaunchKernelGGL((ppcg_update_kernel<T, true>), dim3(grid), dim3(256), 0, stream(), (T *)resnorm, (T *)gamma, (T *)delta, r2, (const T *)alpha, (const T *)beta, (const T *)n, (const T *)m, (T *)p, (T *)s, (T *)q, (T *)z, (T *)x, (T *)r, (T *)u, (T *)w, n_elts, parts, counter);
+    else     hipLaunchKernelGGL((ppcg_update_kernel<T, false>), dim3(grid), dim3(256), 0, stream(), (T *)resnorm, (T *)gamma, (T *)delta, r2, (const T *)alpha, (const T *)beta, (const T *)n, (const T *)m, (T *)p, (T *)s, (T *)q, (T *)z, (T *)x, (T *)r, (T *)u, (T *)w, n_elts, parts, counter);
+  }
+  ISPM_HIP(hipGetLastError());
+  return ISPM_OK;
+}
+
+}  // namespace ispm
+
+using namespace ispm;
+
+static int grid_for(int n)
+{
+  int g = num_cus() * 4;
+  if (g > kMaxBlocks) g = kMaxBlocks;
+  int need = (n + 255) / 256;
+  if (need < 1) need = 1;
+  return need < g ? need : g;
+}
+
+static bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
+extern "C" int ispm_zero(void *p, size_t bytes)
+{
+  int st = ensure_init();
+  if (st) return st;
+  if (!bytes) return ISPM_OK;
+  if (!p) return fail(ISPM_ERR_INVALID, "ispm_zero: null pointer");
+  if (!aligned16(p)) {
+    ISPM_HIP(hipMemsetAsync(p, 0, bytes, stream()));
+    return ISPM_OK;
+  }
+  size_t blocks = (bytes / 16 + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(fill_zero_kernel<char>, dim3((unsigned)blocks), dim3(256), 0, stream(),
+                     (unsigned char *)p, bytes);
+  ISPM_HIP(hipGetLastError());
+  return ISPM_OK;
+}
+
+extern "C" int ispm_copy_indexed(int type, void *dst, const void *src, const int *idx, int n)
+{
+  int st = ensure_init();
+  if (st) return st;
+  if (n <= 0) return ISPM_OK;
+  if (!dst || !src || !idx) return fail(ISPM_ERR_INVALID, "copy_indexed: null pointer");
+  dim3 g((n + 255) / 256), b(256);
+  if (type == ISPM_F64)
+    hipLaunchKernelGGL(copy_indexed_kernel<double>, g, b, 0, stream(), (double *)dst, (const double *)src, idx, n);
+  else if (type == ISPM_F32)
+    hipLaunchKernelGGL(copy_indexed_kernel<float>, g, b, 0, stream(), (float *)dst, (const float *)src, idx, n);
+  else return fail(ISPM_ERR_UNSUPPORTED, "copy_indexed: type %d", type);
+  ISPM_HIP(hipGetLastError());
+  return ISPM_OK;
+}
+
+extern "C" int ispm_add_indexed(int type, void *dst, const void *src, const int *idx, int n)
+{
+  int st = ensure_init();
+  if (st) return st;
+  if (n <= 0) return ISPM_OK;
+  if (!dst || !src || !idx) return fail(ISPM_ERR_INVALID, "add_indexed: null pointer");
+  dim3 g((n + 255) / 256), b(256);
+  if (type == ISPM_F64)
+    hipLaunchKernelGGL(add_indexed_kernel<double>, g, b, 0, stream(), (double *)dst, (const double *)src, idx, n);
+  else if (type == ISPM_F32)
+    hipLaunchKernelGGL(add_indexed_kernel<float>, g, b, 0, stream(), (float *)dst, (const float *)src, idx, n);
+  else return fail(ISPM_ERR_UNSUPPORTED, "add_indexed: type %d", type);
+  ISPM_HIP(hipGetLastError());
+  return ISPM_OK;
+}
+
+extern "C" int ispm_dot(int type, const void *x, const void *y, void *s, int n)
+{
+  int st = ensure_init();
+  if (st) return st;
+  if (!x || !y || !s) return fail(ISPM_ERR_INVALID, "dot: null pointer");
+  if (n < 0) return fail(ISPM_ERR_INVALID, "dot: n < 0");
+  const int grid = grid_for(n);
+  reduce_scratch &rs = scratch();
+  const bool vec = aligned16(x) && aligned16(y);
+  unsigned *ctr = rs.counter + 0;
+  if (type == ISPM_F64) {
+    if (vec) hipLaunchKernelGGL((dot_kernel<double, true>), dim3(grid), dim3(256), 0, stream(), (const double *)x, (const double *)y, (double *)s, n, rs.partials, ctr);
+    else     hipLaunchKernelGGL((dot_kernel<double, false>), dim3(grid), dim3(256), 0, stream(), (const double *)x, (const double *)y, (double *)s, n, rs.partials, ctr);
+  } else if (type == ISPM_F32) {
+    if (vec) hipLaunchKernelGGL((dot_kernel<float, true>), dim3(grid), dim3(256), 0, stream(), (const float *)x, (const float *)y, (float *)s, n, rs.partials, ctr);
+    else     hipLaunchKernelGGL((dot_kernel<float, false>), dim3(grid), dim3(256), 0, stream(), (const float *)x, (const float *)y, (float *)s, n, rs.partials, ctr);
+  } else return fail(ISPM_ERR_UNSUPPORTED, "dot: type %d", type);
+  ISPM_HIP(hipGetLastError());
+  return ISPM_OK;
+}
+
+extern "C" int ispm_ppcg_update_scalars(int type, void *alpha, void *beta, const void *gamma,
+                                        const void *gammaold, const void *delta)
+{
+  int st = ensure_init();
+  if (st) return st;
+  if (!alpha || !beta || !gamma || !gammaold || !delta) return fail(ISPM_ERR_INVALID, "ppcg_update_scalars: null pointer");
+  if (type == ISPM_F64)
+    hipLaunchKernelGGL(ppcg_scalars_kernel<double>, dim3(1), dim3(1), 0, stream(), (double *)alpha, (double *)beta, (const double *)gamma, (const double *)gammaold, (const double *)delta);
+  else if (type == ISPM_F32)
+    hipLaunchKernelGGL(ppcg_scalars_kernel<float>, dim3(1), dim3(1), 0, stream(), (float *)alpha, (float *)beta, (const float *)gamma, (const float *)gammaold, (const float *)delta);
+  else return fail(ISPM_ERR_UNSUPPORTED, "ppcg_update_scalars: type %d", type);
+  ISPM_HIP(hipGetLastError());
+  return ISPM_OK;
+}
+
+extern "C" int ispm_ppcg_update_vectors(int type, void *resnorm, void *gamma, void *delta,
+                                        const void *alpha, const void *beta, const void *n,
+                                        const void *m, void *p, void *s, void *q, void *z,
+                                        void *x, void *r, void *u, void *w, int n_elts)
+{
+  int st = ensure_init();
+  if (st) return st;
+  if (!resnorm || !gamma || !delta || !alpha || !beta || !n || !m || !p || !s || !q || !z || !x || !r || !u || !w)
+    return fail(ISPM_ERR_INVALID, "ppcg_update_vectors: null pointer");
+  if (n_elts < 0) return fail(ISPM_ERR_INVALID, "ppcg_update_vectors: n < 0");
+  if (type != ISPM_F64 && type != ISPM_F32) return fail(ISPM_ERR_UNSUPPORTED, "ppcg_update_vectors: type %d", type);
+  const bool vec = aligned16(n) && aligned16(m) && aligned16(p) && aligned16(s) && aligned16(q) &&
+                   aligned16(z) && aligned16(x) && aligned16(r) && aligned16(u) && aligned16(w);
+  reduce_scratch &rs = scratch();
+  return launch_ppcg_update(type, vec, grid_for(n_elts), resnorm, gamma, delta, nullptr, alpha,
+                            beta, n, m, p, s, q, z, x, r, u, w, n_elts, rs.counter + 1);
+}

@@ -1,0 +1,607 @@
+/* solver.hip -- device-resident FSAI-preconditioned CG on the library stream.
+ *
+ * Three variants share one driver:
+ *  - FP64 pipelined PCG (Ghysels-Vanroose): per iteration t = Ainv1 w, m = Ainv2 t, n = A m
+ *    and ONE fused kernel that evaluates the reference's scalar update
+ *    (beta = g/g_old, alpha = g/(delta - beta g/alpha_old), util/cuda/sblas.cu:156-165) in every
+ *    block and then the eight recurrences with their three sums (sblas.cu:177-217), plus
+ *    (r,r) for the convergence test.
+ *  - FP64 PCG (Hestenes-Stiefel): z = M r, rho = (r,z), p = z + beta p, q = A p, alpha,
+ *    x += alpha p, r -= alpha q.
+ *  - mixed PCG: the same loop with FP32 vectors, FP32 A and FP32/BF16 factors; x is
+ *    accumulated in FP64 and the residual is replaced by the FP64 true residual
+ *    (b - A x with the FP32-stored values read into FP64 accumulators) whenever the FP32
+ *    recursive residual has dropped by reliable_delta ("reliable updates").
+ * All scalars live in device memory in FP64; every reduction is single pass (last block
+ * finishes) and deterministic.  Iterations are launched in hipGraph-captured batches; once
+ * the recursive residual reaches its target a device flag turns the rest of the batch into
+ * no-ops, and the host then decides on the FP64 true residual.
+ */
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstring>
+#include <vector>
+#include "device.h"
+#include "internal.h"
+#include "plan_internal.h"
+#include "sblas_kernels.h"
+
+namespace ispm {
+
+int spmv_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, const void *x,
+                  void *y, const unsigned *skip);
+
+namespace {
+
+struct scalars {
+  double rho, pq, alpha, beta;
+  double gamma, gamma_old, delta, alpha_old;
+  double rr, l1, tol2, r64;
+  double rscale;
+  unsigned done, iters;
+};
+
+enum { OP_PQ, OP_RZ, OP_GAMMA, OP_DELTA, OP_R64 };
+
+template<typename T> struct vtraits {
+  static constexpr int L = 16 / sizeof(T);
+};
+
+template<typename T, int OP>
+__global__ void __launch_bounds__(256)
+k_dot(const T *__restrict__ x, const T *__restrict__ y, int n, scalars *s, double *parts,
+      unsigned *ctr, int skip_if_done)
+{
+  __shared__ double lds[4];
+  if (skip_if_done && s->done) return;
+  constexpr int L = vtraits<T>::L;
+  double v[1] = {0.0};
+  const int gt = blockIdx.x * 256 + threadIdx.x, gs = gridDim.x * 256, nv = n / L;
+  for (int i = gt; i < nv; i += gs) {
+    pack16<T> a = ld16(x, i), b = ld16(y, i);
+#pragma unroll
+    for (int l = 0; l < L; l++) v[0] += (double)a.v[l] * (double)b.v[l];
+  }
+  for (int i = nv * L + gt; i < n; i += gs) v[0] += (double)x[i] * (double)y[i];
+  double out[1];
+  if (finish_reduction<1>(v, parts, ctr, lds, out) && threadIdx.x == 0) {
+    const double r = out[0];
+    if (OP == OP_PQ) { s->pq = r; s->alpha = s->rho / r; }
+    if (OP == OP_RZ) { s->beta = r / s->rho; s->rho = r; }
+    if (OP == OP_GAMMA) s->gamma = r;
+    if (OP == OP_DELTA) s->delta = r;
+    if (OP == OP_R64) s->r64 = r;
+  }
+}
+
+/* x += alpha p; r -= alpha q; (r,r); done when (r,r) <= tol2 */
+template<typename T>
+__global__ void __launch_bounds__(256)
+k_xr(T *__restrict__ x, T *__restrict__ r, const T *__restrict__ p, const T *__restrict__ q, int n,
+     scalars *s, double *parts, unsigned *ctr)
+{
+  __shared__ double lds[4];
+  if (s->done) return;
+  constexpr int L = vtraits<T>::L;
+  const T a = (T)s->alpha;
+  double v[1] = {0.0};
+  const int gt = blockIdx.x * 256 + threadIdx.x, gs = gridDim.x * 256, nv = n / L;
+  for (int i = gt; i < nv; i += gs) {
+    pack16<T> xx = ld16(x, i), rr = ld16(r, i), pp = ld16(p, i), qq = ld16(q, i);
+#pragma unroll
+    for (int l = 0; l < L; l++) {
+      xx.v[l] = xx.v[l] + a * pp.v[l];
+      rr.v[l] = rr.v[l] - a * qq.v[l];
+      v[0] += (double)rr.v[l] * (double)rr.v[l];
+    }
+    st16(x, i, xx);
+    st16(r, i, rr);
+  }
+  for (int i = nv * L + gt; i < n; i += gs) {
+    x[i] = x[i] + a * p[i];
+    T ri = r[i] - a * q[i];
+    r[i] = ri;
+    v[0] += (double)ri * (double)ri;
+  }
+  double out[1];
+  if (finish_reduction<1>(v, parts, ctr, lds, out) && threadIdx.x == 0) {
+    s->rr = out[0];
+    s->iters++;
+    if (out[0] <= s->tol2) s->done = 1;
+  }
+}
+
+/* p = z + beta p */
+template<typename T>
+__global__ void __launch_bounds__(256) k_p(T *__restrict__ p, const T *__restrict__ z, int n, const scalars *s)
+{
+  if (s->done) return;
+  constexpr int L = vtraits<T>::L;
+  const T b = (T)s->beta;
+  const int gt = blockIdx.x * 256 + threadIdx.x, gs = gridDim.x * 256, nv = n / L;
+  for (int i = gt; i < nv; i += gs) {
+    pack16<T> pp = ld16(p, i), zz = ld16(z, i);
+#pragma unroll
+    for (int l = 0; l < L; l++) pp.v[l] = zz.v[l] + b * pp.v[l];
+    st16(p, i, pp);
+  }
+  for (int i = nv * L + gt; i < n; i += gs) p[i] = z[i] + b * p[i];
+}
+
+template<typename T>
+__global__ void __launch_bounds__(256) k_copy(T *__restrict__ d, const T *__restrict__ a, int n, const scalars *s)
+{
+  if (s && s->done) return;
+  const int gt = blockIdx.x * 256 + threadIdx.x, gs = gridDim.x * 256;
+  for (int i = gt; i < n; i += gs) d[i] = a[i];
+}
+
+template<typename T>
+__global__ void __launch_bounds__(256) k_scale(T *__restrict__ d, int n, double c)
+{
+  const int gt = blockIdx.x * 256 + threadIdx.x, gs = gridDim.x * 256;
+  for (int i = gt; i < n; i += gs) d[i] = (T)((double)d[i] * c);
+}
+
+/* Fused pipelined-PCG step: scalar update then the recurrences of sblas.cu:192-200. */
+template<typename T>
+__global__ void __launch_bounds__(256)
+k_ppcg(const T *__restrict__ nn, const T *__restrict__ m, T *__restrict__ p, T *__restrict__ s_,
+       T *__restrict__ q, T *__restrict__ z, T *__restrict__ x, T *__restrict__ r,
+       T *__restrict__ u, T *__restrict__ w, int n, scalars *s, double *parts, unsigned *ctr)
+{
+  __shared__ double lds[16];
+  if (s->done) return;
+  const double g = s->gamma, go = s->gamma_old, dl = s->delta, ao = s->alpha_old;
+  const double bd = g / go;
+  const double ad = g / (dl - bd * g / ao);
+  const T a = (T)ad, b = (T)bd;
+  double acc[4] = {0, 0, 0, 0};
+  constexpr int L = vtraits<T>::L;
+  auto one = [&](T &zi, T &qi, T &si, T &pi, T &ui, T &wi, T &ri, T &xi, T ni, T mi) {
+    zi = ni + b * zi;
+    qi = mi + b * qi;
+    si = wi + b * si;
+    pi = ui + b * pi;
+    ui = ui - a * qi;
+    wi = wi - a * zi;
+    ri = ri - a * si;
+    xi = xi + a * pi;
+    acc[0] += fabs((double)ri);
+    acc[1] += (double)ri * (double)ui;
+    acc[2] += (double)wi * (double)ui;
+    acc[3] += (double)ri * (double)ri;
+  };
+  const int gt = blockIdx.x * 256 + threadIdx.x, gs = gridDim.x * 256, nv = n / L;
+  for (int i = gt; i < nv; i += gs) {
+    pack16<T> zn = ld16(nn, i), mm = ld16(m, i), zz = ld16(z, i), qq = ld16(q, i), ss = ld16(s_, i),
+              pp = ld16(p, i), uu = ld16(u, i), ww = ld16(w, i), rr = ld16(r, i), xx = ld16(x, i);
+#pragma unroll
+    for (int l = 0; l < L; l++)
+      one(zz.v[l], qq.v[l], ss.v[l], pp.v[l], uu.v[l], ww.v[l], rr.v[l], xx.v[l], zn.v[l], mm.v[l]);
+    st16(z, i, zz); st16(q, i, qq); st16(s_, i, ss); st16(p, i, pp);
+    st16(u, i, uu); st16(w, i, ww); st16(r, i, rr); st16(x, i, xx);
+  }
+  for (int i = nv * L + gt; i < n; i += gs) {
+    T zi = z[i], qi = q[i], si = s_[i], pi = p[i], ui = u[i], wi = w[i], ri = r[i], xi = x[i];
+    one(zi, qi, si, pi, ui, wi, ri, xi, nn[i], m[i]);
+    z[i] = zi; q[i] = qi; s_[i] = si; p[i] = pi; u[i] = ui; w[i] = wi; r[i] = ri; x[i] = xi;
+  }
+  double out[4];
+  if (finish_reduction<4>(acc, parts, ctr, lds, out) && threadIdx.x == 0) {
+    s->gamma_old = g;
+    s->alpha_old = ad;
+    s->alpha = ad;
+    s->beta = bd;
+    s->l1 = out[0];
+    s->gamma = out[1];
+    s->delta = out[2];
+    s->rr = out[3];
+    s->iters++;
+    if (out[3] <= s->tol2) s->done = 1;
+  }
+}
+
+/* r = b - Ax (FP64), (r,r) -> s->r64 */
+__global__ void __launch_bounds__(256)
+k_resid(const double *__restrict__ b, const double *__restrict__ Ax, double *__restrict__ r, int n,
+        scalars *s, double *parts, unsigned *ctr)
+{
+  __shared__ double lds[4];
+  double v[1] = {0.0};
+  const int gt = blockIdx.x * 256 + threadIdx.x, gs = gridDim.x * 256;
+  for (int i = gt; i < n; i += gs) {
+    const double ri = b[i] - Ax[i];
+    r[i] = ri;
+    v[0] += ri * ri;
+  }
+  double out[1];
+  if (finish_reduction<1>(v, parts, ctr, lds, out) && threadIdx.x == 0) s->r64 = out[0];
+}
+
+/* dst = src / ||src|| with ||src||^2 = s->r64; s->rscale = ||src|| */
+template<typename T>
+__global__ void __launch_bounds__(256) k_cast_scaled(T *__restrict__ d, const double *__restrict__ a, int n, scalars *s)
+{
+  const double nrm = sqrt(s->r64);
+  const double c = nrm > 0 ? 1.0 / nrm : 0.0;
+  const int gt = blockIdx.x * 256 + threadIdx.x, gs = gridDim.x * 256;
+  for (int i = gt; i < n; i += gs) d[i] = (T)(a[i] * c);
+  if (gt == 0) s->rscale = nrm;
+}
+
+/* x64 += rscale * xs; xs = 0 */
+__global__ void __launch_bounds__(256) k_x64(double *__restrict__ x, float *__restrict__ xs, int n, const scalars *s)
+{
+  const double c = s->rscale;
+  const int gt = blockIdx.x * 256 + threadIdx.x, gs = gridDim.x * 256;
+  for (int i = gt; i < n; i += gs) {
+    x[i] += c * (double)xs[i];
+    xs[i] = 0.0f;
+  }
+}
+
+}  // namespace
+}  // namespace ispm
+
+using namespace ispm;
+
+struct ispm_solver {
+  ispm_plan *A = nullptr, *M1 = nullptr, *M2 = nullptr;
+  ispm_solver_opts o{};
+  int n = 0;
+  int T = ISPM_F64;
+  size_t ts = 8;
+  bool mixed = false, pipelined = false;
+  void *r = nullptr, *u = nullptr, *w = nullptr, *m = nullptr, *nv = nullptr, *z = nullptr,
+       *q = nullptr, *sv = nullptr, *p = nullptr, *t = nullptr, *xs = nullptr;
+  double *r64 = nullptr, *tmp64 = nullptr;
+  scalars *s = nullptr;
+  scalars *hs = nullptr;   /* pinned host mirror */
+  double *parts = nullptr; /* reduction partials, private to this solver */
+  unsigned *ctr = nullptr; /* completion counters, private */
+  int grid = 1;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  int graph_iters = 0;
+  const void *graph_x = nullptr;   /* x the captured graph was recorded with (FP64 variants) */
+};
+
+namespace {
+
+int alloc(void **p, size_t bytes) { return ispm_malloc(p, bytes); }
+
+int spmv(ispm_plan *P, int vt, const void *x, void *y, const unsigned *skip)
+{
+  return spmv_internal(&P->launch, &P->dev, vt, x, y, skip);
+}
+
+template<typename T> int launch_dot(ispm_solver *S, int op, const void *x, const void *y, int skip, int slot)
+{
+  unsigned *c = S->ctr + slot;
+  const T *a = (const T *)x, *b = (const T *)y;
+  hipStream_t st = stream();
+  dim3 g(S->grid), bl(256);
+  switch (op) {
+    case OP_PQ: hipLaunchKernelGGL((k_dot<T, OP_PQ>), g, bl, 0, st, a, b, S->n, S->s, S->parts, c, skip); break;
+    case OP_RZ: hipLaunchKernelGGL((k_dot<T, OP_RZ>), g, bl, 0, st, a, b, S->n, S->s, S->parts, c, skip); break;
+    case OP_GAMMA: hipLaunchKernelGGL((k_dot<T, OP_GAMMA>), g, bl, 0, st, a, b, S->n, S->s, S->parts, c, skip); break;
+    case OP_DELTA: hipLaunchKernelGGL((k_dot<T, OP_DELTA>), g, bl, 0, st, a, b, S->n, S->s, S->parts, c, skip); break;
+    default: hipLaunchKernelGGL((k_dot<T, OP_R64>), g, bl, 0, st, a, b, S->n, S->s, S->parts, c, skip); break;
+  }
+  ISPM_HIP(hipGetLastError());
+  return ISPM_OK;
+}
+
+int dot(ispm_solver *S, int op, const void *x, const void *y, int skip, int vt)
+{
+  return vt == ISPM_F64 ? launch_dot<double>(S, op, x, y, skip, 0) : launch_dot<float>(S, op, x, y, skip, 0);
+}
+
+/* z = M r (two SpMVs, or a copy without preconditioner) */
+int apply_M(ispm_solver *S, const void *rin, void *zout, const unsigned *skip)
+{
+  int st;
+  if (!S->M1) {
+    if (S->T == ISPM_F64) hipLaunchKernelGGL(k_copy<double>, dim3(S->grid), dim3(256), 0, stream(), (double *)zout, (const double *)rin, S->n, skip ? S->s : nullptr);
+    else hipLaunchKernelGGL(k_copy<float>, dim3(S->grid), dim3(256), 0, stream(), (float *)zout, (const float *)rin, S->n, skip ? S->s : nullptr);
+    ISPM_HIP(hipGetLastError());
+    return ISPM_OK;
+  }
+  if ((st = spmv(S->M1, S->T, rin, S->t, skip))) return st;
+  return spmv(S->M2, S->T, S->t, zout, skip);
+}
+
+const unsigned *done_flag(ispm_solver *S) { return &S->s->done; }
+
+/* one iteration (enqueued; may be captured) */
+int iterate(ispm_solver *S)
+{
+  int st;
+  const unsigned *sk = done_flag(S);
+  hipStream_t sm = stream();
+  dim3 g(S->grid), b(256);
+  if (S->pipelined) {
+    if ((st = apply_M(S, S->w, S->m, sk))) return st;
+    if ((st = spmv(S->A, S->T, S->m, S->nv, sk))) return st;
+    hipLaunchKernelGGL(k_ppcg<double>, g, b, 0, sm, (const double *)S->nv, (const double *)S->m,
+                       (double *)S->p, (double *)S->sv, (double *)S->q, (double *)S->z,
+                       (double *)S->xs, (double *)S->r, (double *)S->u, (double *)S->w, S->n, S->s,
+                       S->parts, S->ctr + 1);
+    ISPM_HIP(hipGetLastError());
+    return ISPM_OK;
+  }
+  if ((st = apply_M(S, S->r, S->z, sk))) return st;
+  if ((st = dot(S, OP_RZ, S->r, S->z, 1, S->T))) return st;
+  if (S->T == ISPM_F64) hipLaunchKernelGGL(k_p<double>, g, b, 0, sm, (double *)S->p, (const double *)S->z, S->n, S->s);
+  else hipLaunchKernelGGL(k_p<float>, g, b, 0, sm, (float *)S->p, (const float *)S->z, S->n, S->s);
+  ISPM_HIP(hipGetLastError());
+  if ((st = spmv(S->A, S->T, S->p, S->q, sk))) return st;
+  if ((st = dot(S, OP_PQ, S->p, S->q, 1, S->T))) return st;
+  if (S->T == ISPM_F64)
+    hipLaunchKernelGGL(k_xr<double>, g, b, 0, sm, (double *)S->xs, (double *)S->r, (const double *)S->p,
+                       (const double *)S->q, S->n, S->s, S->parts, S->ctr + 1);
+  else
+    hipLaunchKernelGGL(k_xr<float>, g, b, 0, sm, (float *)S->xs, (float *)S->r, (const float *)S->p,
+                       (const float *)S->q, S->n, S->s, S->parts, S->ctr + 1);
+  ISPM_HIP(hipGetLastError());
+  return ISPM_OK;
+}
+
+int run_batch(ispm_solver *S, int k)
+{
+  if (!S->o.use_graph) {
+    for (int i = 0; i < k; i++) {
+      int st = iterate(S);
+      if (st) return st;
+    }
+    return ISPM_OK;
+  }
+  if (!S->exec || S->graph_iters != k) {
+    if (S->exec) { (void)hipGraphExecDestroy(S->exec); S->exec = nullptr; }
+    if (S->graph) { (void)hipGraphDestroy(S->graph); S->graph = nullptr; }
+    hipStream_t sm = stream();
+    ISPM_HIP(hipStreamBeginCapture(sm, hipStreamCaptureModeThreadLocal));
+    int st = ISPM_OK;
+    for (int i = 0; i < k && !st; i++) st = iterate(S);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(sm, &g);
+    if (st) { if (g) (void)hipGraphDestroy(g); return st; }
+    if (e != hipSuccess) return hip_status(e, "hipStreamEndCapture");
+    S->graph = g;
+    ISPM_HIP(hipGraphInstantiate(&S->exec, g, nullptr, nullptr, 0));
+    S->graph_iters = k;
+  }
+  ISPM_HIP(hipGraphLaunch(S->exec, stream()));
+  return ISPM_OK;
+}
+
+int pull(ispm_solver *S)
+{
+  ISPM_HIP(hipMemcpyAsync(S->hs, S->s, sizeof(scalars), hipMemcpyDeviceToHost, stream()));
+  ISPM_HIP(hipStreamSynchronize(stream()));
+  return ISPM_OK;
+}
+int push(ispm_solver *S)
+{
+  ISPM_HIP(hipMemcpyAsync(S->s, S->hs, sizeof(scalars), hipMemcpyHostToDevice, stream()));
+  ISPM_HIP(hipStreamSynchronize(stream()));
+  return ISPM_OK;
+}
+
+int zero(void *p, size_t bytes) { return ispm_zero(p, bytes); }
+
+/* r64 = b - A x (FP64 true residual), s->r64 = ||r64||^2 */
+int true_residual(ispm_solver *S, const double *b, const double *x)
+{
+  int st = spmv(S->A, ISPM_F64, x, S->tmp64, nullptr);
+  if (st) return st;
+  hipLaunchKernelGGL(k_resid, dim3(S->grid), dim3(256), 0, stream(), b, (const double *)S->tmp64, S->r64,
+                     S->n, S->s, S->parts, S->ctr + 2);
+  ISPM_HIP(hipGetLastError());
+  return ISPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ispm_solver_create(ispm_solver **out, ispm_plan *A, ispm_plan *M1, ispm_plan *M2,
+                       const ispm_solver_opts *o)
+{
+  if (!out || !A || !o) return fail(ISPM_ERR_INVALID, "solver_create: null argument");
+  *out = nullptr;
+  int st = ensure_init();
+  if (st) return st;
+  if ((M1 == nullptr) != (M2 == nullptr)) return fail(ISPM_ERR_INVALID, "solver_create: give both factors or none");
+  if (!A->has_dev) return fail(ISPM_ERR_INVALID, "solver_create: A plan has no device matrix");
+  const bool mixed = o->precision == ISPM_PREC_MIXED;
+  if (mixed && o->method != ISPM_SOLVER_PCG) return fail(ISPM_ERR_UNSUPPORTED, "solver_create: mixed precision uses method PCG");
+  if (!mixed && A->E != ISPM_F64) return fail(ISPM_ERR_INVALID, "solver_create: FP64 solve needs an FP64 A plan");
+  if (mixed && A->E != ISPM_F32) return fail(ISPM_ERR_INVALID, "solver_create: mixed solve needs an FP32 A plan");
+  const int T = mixed ? ISPM_F32 : ISPM_F64;
+  if (M1 && (!M1->has_dev || !M2->has_dev || M1->T != T || M2->T != T))
+    return fail(ISPM_ERR_INVALID, "solver_create: factor plans must be on the device with vector type %s", mixed ? "F32" : "F64");
+  if (M1 && (M1->L.n_rows != A->L.n_rows || M2->L.n_rows != A->L.n_rows))
+    return fail(ISPM_ERR_INVALID, "solver_create: factor sizes differ from A");
+  ispm_solver *S = new ispm_solver;
+  S->A = A; S->M1 = M1; S->M2 = M2;
+  S->o = *o;
+  if (S->o.check_every <= 0) S->o.check_every = 16;
+  if (S->o.maxit <= 0) S->o.maxit = 100000;
+  if (S->o.reliable_delta <= 0 || S->o.reliable_delta >= 1) S->o.reliable_delta = 0.1;
+  S->mixed = mixed;
+  S->pipelined = o->method == ISPM_SOLVER_PPCG;
+  S->T = T;
+  S->ts = mixed ? 4 : 8;
+  S->n = A->L.n_rows;
+  const size_t vb = S->ts * (size_t)S->n + 64;
+  S->grid = std::min(num_cus() * 4, kMaxBlocks);
+  void **vecs[] = {&S->r, &S->z, &S->q, &S->p, &S->t, &S->xs};
+  for (void **v : vecs) if ((st = alloc(v, vb))) { ispm_solver_destroy(S); return st; }
+  if (S->pipelined) {
+    void **pv[] = {&S->u, &S->w, &S->m, &S->nv, &S->sv};
+    for (void **v : pv) if ((st = alloc(v, vb))) { ispm_solver_destroy(S); return st; }
+  }
+  if ((st = alloc((void **)&S->r64, 8 * (size_t)S->n + 64)) || (st = alloc((void **)&S->tmp64, 8 * (size_t)S->n + 64)) ||
+      (st = alloc((void **)&S->s, sizeof(scalars))) || (st = alloc((void **)&S->parts, sizeof(double) * kMaxBlocks * 4)) ||
+      (st = alloc((void **)&S->ctr, sizeof(unsigned) * 4))) {
+    ispm_solver_destroy(S);
+    return st;
+  }
+  hipError_t e = hipHostMalloc((void **)&S->hs, sizeof(scalars), hipHostMallocDefault);
+  if (e != hipSuccess) { ispm_solver_destroy(S); return hip_status(e, "hipHostMalloc"); }
+  std::memset(S->hs, 0, sizeof(scalars));
+  *out = S;
+  return ISPM_OK;
+}
+
+int ispm_solver_solve(ispm_solver *S, const double *b, double *x, ispm_solve_stats *stats)
+{
+  if (!S || !b || !x) return fail(ISPM_ERR_INVALID, "solver_solve: null argument");
+  if ((reinterpret_cast<uintptr_t>(b) | reinterpret_cast<uintptr_t>(x)) & 15)
+    return fail(ISPM_ERR_INVALID, "solver_solve: b and x must be 16-byte aligned");
+  int st;
+  const int n = S->n;
+  const size_t vb = S->ts * (size_t)n;
+  hipStream_t sm = stream();
+  dim3 g(S->grid), bl(256);
+  ispm_solve_stats res{};
+  /* ||b|| */
+  std::memset(S->hs, 0, sizeof(scalars));
+  if ((st = push(S))) return st;
+  if ((st = launch_dot<double>(S, OP_R64, b, b, 0, 3))) return st;
+  if ((st = pull(S))) return st;
+  const double bnorm = std::sqrt(S->hs->r64) > 0 ? std::sqrt(S->hs->r64) : 1.0;
+  const double tol = S->o.tol > 0 ? S->o.tol : 1e-10;
+  /* r = b - A x */
+  if ((st = true_residual(S, b, x))) return st;
+  if ((st = pull(S))) return st;
+  double rel = std::sqrt(S->hs->r64) / bnorm;
+  const double margin = 0.9;
+  for (void *v : {S->z, S->q, S->p, S->sv}) if (v && (st = zero(v, vb))) return st;
+
+  auto setup_recurrence = [&](double tol2) -> int {
+    int e;
+    scalars &h = *S->hs;
+    if (S->mixed) {
+      /* r = r64 / ||r64||; rho restarts the Krylov sequence (p = 0) */
+      hipLaunchKernelGGL(k_cast_scaled<float>, g, bl, 0, sm, (float *)S->r, (const double *)S->r64, n, S->s);
+      ISPM_HIP(hipGetLastError());
+      if ((e = pull(S))) return e;
+      h.rho = INFINITY;
+      h.done = 0;
+      h.tol2 = tol2;
+      if ((e = zero(S->xs, vb)) || (e = zero(S->p, vb))) return e;
+      return push(S);
+    }
+    /* FP64: r = r64 */
+    ISPM_HIP(hipMemcpyAsync(S->r, S->r64, 8 * (size_t)n, hipMemcpyDeviceToDevice, sm));
+    h.done = 0;
+    h.tol2 = tol2;
+    if (S->pipelined) {
+      /* u = M r; w = A u; gamma = (r,u); delta = (w,u); recurrences restart */
+      if ((e = apply_M(S, S->r, S->u, nullptr))) return e;
+      if ((e = spmv(S->A, ISPM_F64, S->u, S->w, nullptr))) return e;
+      if ((e = launch_dot<double>(S, OP_GAMMA, S->r, S->u, 0, 3))) return e;
+      if ((e = launch_dot<double>(S, OP_DELTA, S->w, S->u, 0, 3))) return e;
+      for (void *v : {S->z, S->q, S->p, S->sv}) if ((e = zero(v, vb))) return e;
+      if ((e = pull(S))) return e;
+      h.gamma_old = INFINITY;
+      h.alpha_old = 1.0;
+      h.done = 0;
+      h.tol2 = tol2;
+      return push(S);
+    }
+    h.rho = INFINITY;
+    if ((e = zero(S->p, vb))) return e;
+    return push(S);
+  };
+
+  if (rel <= tol) {
+    res.converged = 1;
+    res.relres = rel;
+    if (stats) *stats = res;
+    return ISPM_OK;
+  }
+  /* FP64 variants iterate on x directly (xs aliases x); mixed accumulates in xs */
+  void *xs_saved = S->xs;
+  if (!S->mixed) S->xs = x;
+  if (S->exec && !S->mixed && S->graph_x != x) {
+    /* graph was captured with another x: rebuild */
+    (void)hipGraphExecDestroy(S->exec); S->exec = nullptr;
+    (void)hipGraphDestroy(S->graph); S->graph = nullptr;
+  }
+  S->graph_x = x;
+  double rec_target;   /* target of the recursive residual, relative to its starting norm */
+  if (S->mixed) {
+    double rmax = std::sqrt(S->hs->r64);
+    rec_target = std::max(S->o.reliable_delta, margin * tol * bnorm / rmax);
+    st = setup_recurrence(rec_target * rec_target);
+  } else {
+    double t = margin * tol * bnorm;
+    rec_target = t;
+    st = setup_recurrence(t * t);
+  }
+  if (st) { S->xs = xs_saved; return st; }
+  int guard = 0;
+  while (true) {
+    if ((st = run_batch(S, S->o.check_every))) break;
+    if ((st = pull(S))) break;
+    scalars &h = *S->hs;
+    res.iterations = (int)h.iters;
+    res.l1_resnorm = h.l1;
+    const bool out_of_iters = (int)h.iters >= S->o.maxit;
+    if (!h.done && !out_of_iters) continue;
+    if (S->mixed) {
+      hipLaunchKernelGGL(k_x64, g, bl, 0, sm, x, (float *)S->xs, n, S->s);
+      ISPM_HIP(hipGetLastError());
+    }
+    if ((st = true_residual(S, b, x)) || (st = pull(S))) break;
+    rel = std::sqrt(S->hs->r64) / bnorm;
+    res.relres = rel;
+    if (rel <= tol) { res.converged = 1; break; }
+    if (out_of_iters || ++guard > 10000) break;
+    res.replacements++;
+    if (S->mixed) {
+      const double rmax_old = S->hs->rscale, rmax = std::sqrt(S->hs->r64);
+      /* keep the search direction: rescale p and rho to the new residual scale */
+      const double c = rmax_old / rmax;
+      hipLaunchKernelGGL(k_cast_scaled<float>, g, bl, 0, sm, (float *)S->r, (const double *)S->r64, n, S->s);
+      hipLaunchKernelGGL(k_scale<float>, g, bl, 0, sm, (float *)S->p, n, c);
+      ISPM_HIP(hipGetLastError());
+      if ((st = pull(S))) break;
+      rec_target = std::max(S->o.reliable_delta, margin * tol * bnorm / rmax);
+      h.rho = h.rho * c * c;
+      h.done = 0;
+      h.tol2 = rec_target * rec_target;
+      if ((st = push(S))) break;
+    } else {
+      /* the recursive residual drifted from the true one: replace it and tighten */
+      rec_target *= 0.5;
+      st = setup_recurrence(rec_target * rec_target);
+      if (st) break;
+    }
+  }
+  S->xs = xs_saved;
+  if (stats) *stats = res;
+  if (st) return st;
+  return res.converged ? ISPM_OK : fail(ISPM_ERR_NOT_CONVERGED, "solver: relres %.3e after %d iterations", res.relres, res.iterations);
+}
+
+void ispm_solver_destroy(ispm_solver *S)
+{
+  if (!S) return;
+  if (S->exec) (void)hipGraphExecDestroy(S->exec);
+  if (S->graph) (void)hipGraphDestroy(S->graph);
+  for (void *v : {S->r, S->u, S->w, S->m, S->nv, S->z, S->q, S->sv, S->p, S->t, S->xs}) ispm_free(v);
+  ispm_free(S->r64);
+  ispm_free(S->tmp64);
+  ispm_free(S->s);
+  ispm_free(S->parts);
+  ispm_free(S->ctr);
+  if (S->hs) (void)hipHostFree(S->hs);
+  delete S;
+}
+
+}  // extern "C"

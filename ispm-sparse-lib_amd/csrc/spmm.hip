@@ -1,0 +1,23 @@
+/* spmm.hip -- multi-vector SlELL product: one pass over the matrix serves k = 2..8
+ * right-hand sides (kernel in spmv_kernel.h, one row per thread). */
+#include "spmv_kernel.h"
+
+namespace ispm {
+using namespace spmvk;
+
+int spmm_dispatch(const ispm_launch *p, const ispm_slell_desc *A, int vt, int k,
+                  const void *X, size_t ldx, void *Y, size_t ldy)
+{
+  switch (k) {
+    case 2: return launch_types_h<2, 1>(p, A, vt, X, ldx, Y, ldy);
+    case 3: return launch_types_h<3, 1>(p, A, vt, X, ldx, Y, ldy);
+    case 4: return launch_types_h<4, 1>(p, A, vt, X, ldx, Y, ldy);
+    case 5: return launch_types_h<5, 1>(p, A, vt, X, ldx, Y, ldy);
+    case 6: return launch_types_h<6, 1>(p, A, vt, X, ldx, Y, ldy);
+    case 7: return launch_types_h<7, 1>(p, A, vt, X, ldx, Y, ldy);
+    case 8: return launch_types_h<8, 1>(p, A, vt, X, ldx, Y, ldy);
+    default: return fail(ISPM_ERR_UNSUPPORTED, "spmm: k=%d", k);
+  }
+}
+
+}  // namespace ispm

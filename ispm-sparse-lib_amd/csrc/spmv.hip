@@ -1,0 +1,50 @@
+/* spmv.hip -- C-ABI entry point of the single-vector SlELL SpMV (kernel in spmv_kernel.h). */
+#include "spmv_kernel.h"
+
+namespace ispm {
+/* defined in spmm.hip */
+int spmm_dispatch(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, int k,
+                  const void *X, size_t ldx, void *Y, size_t ldy);
+}
+
+namespace ispm {
+/* SpMV for the solver: same checks as ispm_spmv, optional device skip flag. */
+int spmv_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, const void *x,
+                  void *y, const unsigned *skip)
+{
+  unsigned H;
+  int st = spmvk::check_args(p, A, vec_type, x, y, H);
+  if (st) return st;
+  if (A->n_slices == 0) return ISPM_OK;
+  return spmvk::launch_types<1>(p, A, vec_type, x, 0, y, 0, H, skip);
+}
+}  // namespace ispm
+
+using namespace ispm;
+using namespace ispm::spmvk;
+
+extern "C" int ispm_spmv(const ispm_launch *p, const ispm_slell_desc *A, int vec_type,
+                         const void *x, void *y)
+{
+  int st = ensure_init();
+  if (st) return st;
+  unsigned H;
+  st = check_args(p, A, vec_type, x, y, H);
+  if (st) return st;
+  if (A->n_slices == 0) return ISPM_OK;
+  return launch_types<1>(p, A, vec_type, x, 0, y, 0, H);
+}
+
+extern "C" int ispm_spmm(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, int k,
+                         const void *X, size_t ldx, void *Y, size_t ldy)
+{
+  int st = ensure_init();
+  if (st) return st;
+  unsigned H;
+  st = check_args(p, A, vec_type, X, Y, H);
+  if (st) return st;
+  if (k < 1 || k > 8) return fail(ISPM_ERR_UNSUPPORTED, "spmm: k=%d outside 1..8", k);
+  if (A->n_slices == 0) return ISPM_OK;
+  if (k == 1) return launch_types<1>(p, A, vec_type, X, 0, Y, 0, H);
+  return spmm_dispatch(p, A, vec_type, k, X, ldx, Y, ldy);
+}

@@ -1,0 +1,354 @@
+#ifndef ISPM_SPMV_KERNEL_H
+#define ISPM_SPMV_KERNEL_H
+/* spmv_kernel.h -- sliced-ELLPACK ("SlELL") SpMV / SpMM for gfx950.
+ *
+ * Layout (formats/formats.h:39-102, conversion.h:338-414 of the reference): slice s covers
+ * rows [y0, y0+sz) (sz a power of two), its elements stored column-major, element (r,k) at
+ * e0 + r + k*sz.  Slice records in slice_ptr have P = 1+[W>1]+[V]+2[D] words:
+ * [elm_off, (row_start if V), (col_off if W>1 or D), (diag_off if D)].
+ *   plain ELL : nell planes of (col, val).
+ *   diagonal  : nell ELL planes, then ndiag dense planes whose column is
+ *               diag_pos[d] + r - sz; columns outside [0, n_cols) read x as 0.
+ *   hblock W=2: nell planes of singles, then block planes holding a pair index c/2 and two
+ *               value planes (even, odd column).
+ *
+ * Mapping (one workgroup per slice, like cuda/spmv/kernel.h:1-135, re-designed for wave64):
+ *   thread t owns rows row..row+H-1 of the slice with row = (t*H) mod R and walks planes
+ *   k = t*H/R, +kstep, ..., where R = min(N*H, sz) and kstep = N*H/R.  Every plane load of a
+ *   wave is one contiguous run of 64*H elements (fully coalesced), diagonal x runs are
+ *   contiguous too.  When several threads share a row (kstep > 1) their partial sums are
+ *   combined by xor-shuffles inside the wave and through LDS across waves (replacing the
+ *   shared-memory halving of cuda/reduce.h:18-127).  When kstep == 1 the plane index is
+ *   wave-uniform, so diagonal offsets come through scalar loads.
+ *   Workgroups are remapped so each XCD (8 per MI355X, round-robin workgroup dispatch)
+ *   processes one contiguous range of slices: neighbouring slices share x windows, which
+ *   then stay in that XCD's L2.
+ *
+ * The summation order inside a row differs from CSR order; FP64 results agree with an
+ * FP64 CSR product to rounding (|dy| <= 1e-13 sum_k |a_rk x_k| in the tests).
+ */
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include "device.h"
+#include "internal.h"
+
+namespace ispm {
+namespace spmvk {
+
+using dev::bf16_t;
+using dev::elt;
+
+struct spmv_args {
+  const unsigned *sp;
+  const unsigned *cols;
+  const void *elms;
+  const void *x;
+  void *y;
+  size_t ldx, ldy;       /* SpMM: column strides in elements */
+  unsigned n_slices;
+  unsigned log2S;        /* fixed-height slices */
+  unsigned n_cols;
+  unsigned per_xcd;      /* slices per XCD range */
+  const unsigned *skip;  /* if non-null and *skip != 0 the launch does nothing (solver done flag) */
+};
+
+template<typename T> __device__ __forceinline__ T xload(const T *x, unsigned i, unsigned n)
+{
+  T v = x[i < n ? i : n - 1];
+  return i < n ? v : (T)0;
+}
+
+template<typename T, int H> struct vstore;
+template<> struct vstore<double, 2> {
+  static __device__ __forceinline__ void st(double *y, const double (&v)[2])
+  { *reinterpret_cast<double2 *>(y) = make_double2(v[0], v[1]); }
+};
+template<> struct vstore<float, 2> {
+  static __device__ __forceinline__ void st(float *y, const float (&v)[2])
+  { *reinterpret_cast<float2 *>(y) = make_float2(v[0], v[1]); }
+};
+template<typename T> struct vstore<T, 1> {
+  static __device__ __forceinline__ void st(T *y, const T (&v)[1]) { *y = v[0]; }
+};
+
+template<typename T, typename E, int H, int W, bool V, bool D, int K>
+__global__ void __launch_bounds__(1024) slell_kernel(spmv_args a)
+{
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const unsigned N = blockDim.x;
+  const unsigned t = threadIdx.x;
+  const unsigned b = blockIdx.x;
+  const unsigned slice = (b & 7u) * a.per_xcd + (b >> 3);
+  if (slice >= a.n_slices) return;
+  if (a.skip && *a.skip) return;
+
+  constexpr unsigned P = 1 + (W > 1) + (V ? 1 : 0) + (D ? 2 : 0);
+  const unsigned *rec = a.sp + P * slice;
+  const unsigned e0 = rec[0], e1 = rec[P];
+  unsigned y0, log2sz;
+  if (V) {
+    y0 = rec[1];
+    log2sz = 31 - __clz(rec[1 + P] - y0);
+  } else {
+    y0 = slice << a.log2S;
+    log2sz = a.log2S;
+  }
+  const unsigned sz = 1u << log2sz;
+  unsigned c0, nell, nsec = 0, d0 = 0;
+  if (D) {
+    c0 = rec[1 + V];
+    d0 = rec[2 + V];
+    const unsigned c1 = rec[1 + V + P];
+    nell = (c1 - c0) >> log2sz;
+    nsec = ((e1 - e0) >> log2sz) - nell;
+  } else if (W > 1) {
+    c0 = rec[1 + V];
+    const unsigned c1 = rec[1 + V + P];
+    const unsigned nse = (W * (c1 - c0) - (e1 - e0)) / (W - 1);
+    nell = nse >> log2sz;
+    nsec = ((c1 - c0) >> log2sz) - nell;
+  } else {
+    c0 = e0;
+    nell = (e1 - e0) >> log2sz;
+  }
+
+  const E *__restrict__ el = static_cast<const E *>(a.elms);
+  const unsigned *__restrict__ cl = a.cols;
+  const T *__restrict__ xv = static_cast<const T *>(a.x);
+  T *__restrict__ yv = static_cast<T *>(a.y);
+  const unsigned ncols = a.n_cols;
+
+  const unsigned NH = N * H;
+  const unsigned log2NH = 31 - __clz(NH);
+  const unsigned log2R = log2NH < log2sz ? log2NH : log2sz;
+  const unsigned R = 1u << log2R;
+  const unsigned kstep = NH >> log2R;
+  const unsigned tH = t * H;
+  const unsigned rloc = tH & (R - 1);
+  const unsigned kstart = tH >> log2R;
+  const unsigned g = R / H;               /* threads per plane column */
+  const unsigned passes = sz >> log2R;
+  const unsigned ktot = nell + nsec;
+
+  for (unsigned pass = 0; pass < passes; pass++) {
+    const unsigned row = pass * R + rloc;
+    T acc[K][H];
+#pragma unroll
+    for (int j = 0; j < K; j++)
+#pragma unroll
+      for (int h = 0; h < H; h++) acc[j][h] = 0;
+
+    auto ell_plane = [&](unsigned k) {
+      const unsigned off = (k << log2sz) + row;
+      T e[H];
+      unsigned c[H];
+      if (H == 2) {
+        elt<T, E>::get2(el, e0 + off, e[0], e[H - 1]);
+        uint2 cc = *reinterpret_cast<const uint2 *>(cl + c0 + off);
+        c[0] = cc.x; c[H - 1] = cc.y;
+      } else {
+        e[0] = elt<T, E>::get(el, e0 + off);
+        c[0] = cl[c0 + off];
+      }
+#pragma unroll
+      for (int j = 0; j < K; j++)
+#pragma unroll
+        for (int h = 0; h < H; h++) acc[j][h] += e[h] * xload(xv + j * a.ldx, c[h], ncols);
+    };
+    auto sec_plane = [&](unsigned d) {
+      if (D) {
+        const unsigned off = ((nell + d) << log2sz) + row;
+        T e[H];
+        if (H == 2) elt<T, E>::get2(el, e0 + off, e[0], e[H - 1]);
+        else e[0] = elt<T, E>::get(el, e0 + off);
+        const unsigned xi = a.sp[d0 + d] + row - sz;
+#pragma unroll
+        for (int j = 0; j < K; j++)
+#pragma unroll
+          for (int h = 0; h < H; h++) acc[j][h] += e[h] * xload(xv + j * a.ldx, xi + h, ncols);
+      } else if (W > 1) {
+        const unsigned boff = (nell << log2sz);
+        const unsigned coff = ((nell + d) << log2sz) + row;
+        unsigned c[H];
+        if (H == 2) {
+          uint2 cc = *reinterpret_cast<const uint2 *>(cl + c0 + coff);
+          c[0] = cc.x; c[H - 1] = cc.y;
+        } else {
+          c[0] = cl[c0 + coff];
+        }
+        T ev[H], od[H];
+        const unsigned eo = e0 + boff + ((2 * d) << log2sz) + row;
+        if (H == 2) {
+          elt<T, E>::get2(el, eo, ev[0], ev[H - 1]);
+          elt<T, E>::get2(el, eo + sz, od[0], od[H - 1]);
+        } else {
+          ev[0] = elt<T, E>::get(el, eo);
+          od[0] = elt<T, E>::get(el, eo + sz);
+        }
+#pragma unroll
+        for (int j = 0; j < K; j++)
+#pragma unroll
+          for (int h = 0; h < H; h++) {
+            const unsigned xc = 2 * c[h];
+            acc[j][h] += ev[h] * xload(xv + j * a.ldx, xc, ncols)
+                       + od[h] * xload(xv + j * a.ldx, xc + 1, ncols);
+          }
+      }
+    };
+
+    if (kstep == 1) {
+      /* every thread walks all planes: the plane index is wave-uniform */
+#pragma unroll 4
+      for (unsigned k = 0; k < nell; k++) ell_plane(k);
+#pragma unroll 8
+      for (unsigned d = 0; d < nsec; d++) sec_plane(d);
+    } else {
+      unsigned k = kstart;
+#pragma unroll 2
+      for (; k < nell; k += kstep) ell_plane(k);
+#pragma unroll 4
+      for (; k < ktot; k += kstep) sec_plane(k - nell);
+
+      /* combine the kstep partial sums of each row */
+      for (unsigned off = g; off < 64 && off < N; off <<= 1)
+#pragma unroll
+        for (int j = 0; j < K; j++)
+#pragma unroll
+          for (int h = 0; h < H; h++) acc[j][h] += __shfl_xor(acc[j][h], (int)off, 64);
+      const unsigned G = g > 64 ? g : 64;
+      if (N > G) {
+        T *lds = reinterpret_cast<T *>(lds_raw);
+#pragma unroll
+        for (int j = 0; j < K; j++)
+#pragma unroll
+          for (int h = 0; h < H; h++) lds[(j * H + h) * N + t] = acc[j][h];
+        __syncthreads();
+        if (t < g) {
+          const unsigned parts = N / G;
+#pragma unroll
+          for (int j = 0; j < K; j++)
+#pragma unroll
+            for (int h = 0; h < H; h++) {
+              T s = 0;
+              for (unsigned m = 0; m < parts; m++) s += lds[(j * H + h) * N + t + m * G];
+              acc[j][h] = s;
+            }
+        }
+      }
+    }
+    if (t < g) {
+#pragma unroll
+      for (int j = 0; j < K; j++) vstore<T, H>::st(yv + j * a.ldy + y0 + pass * R + t * H, acc[j]);
+    }
+  }
+}
+
+template<typename T, typename E, int H, int K>
+int launch_tv(const ispm_launch *p, const ispm_slell_desc *A, const void *x, size_t ldx,
+              void *y, size_t ldy, const unsigned *skip = nullptr)
+{
+  spmv_args a;
+  a.skip = skip;
+  a.sp = A->slice_ptr;
+  a.cols = A->cols;
+  a.elms = A->elms;
+  a.x = x;
+  a.y = y;
+  a.ldx = ldx;
+  a.ldy = ldy;
+  a.n_slices = A->n_slices;
+  a.log2S = A->var_height ? 0u : (unsigned)(31 - __builtin_clz(A->slice_height));
+  a.n_cols = (unsigned)A->n_cols;
+  a.per_xcd = (A->n_slices + 7) / 8;
+  const unsigned N = p->N;
+  const unsigned grid = a.per_xcd * 8;
+  /* LDS is needed only when partial sums of one row live in different waves. */
+  size_t lds = 0;
+  bool need = N > 64;
+  if (!A->var_height) {
+    const unsigned S = A->slice_height, R = N * H < S ? N * H : S, g = R / H;
+    need = N * H > S && N > (g > 64 ? g : 64);
+  }
+  if (need) lds = sizeof(T) * N * H * K;
+  if (lds > 65536) return fail(ISPM_ERR_UNSUPPORTED, "spmv: %zu bytes of LDS exceed 64 KiB", lds);
+  hipStream_t s = stream();
+  const bool V = A->var_height != 0, Dg = A->diags != 0;
+  const unsigned W = A->hblock;
+  if (W == 2) {
+    if (V) hipLaunchKernelGGL((slell_kernel<T, E, H, 2, true, false, K>), dim3(grid), dim3(N), lds, s, a);
+    else   hipLaunchKernelGGL((slell_kernel<T, E, H, 2, false, false, K>), dim3(grid), dim3(N), lds, s, a);
+  } else if (Dg) {
+    if (V) hipLaunchKernelGGL((slell_kernel<T, E, H, 1, true, true, K>), dim3(grid), dim3(N), lds, s, a);
+    else   hipLaunchKernelGGL((slell_kernel<T, E, H, 1, false, true, K>), dim3(grid), dim3(N), lds, s, a);
+  } else {
+    if (V) hipLaunchKernelGGL((slell_kernel<T, E, H, 1, true, false, K>), dim3(grid), dim3(N), lds, s, a);
+    else   hipLaunchKernelGGL((slell_kernel<T, E, H, 1, false, false, K>), dim3(grid), dim3(N), lds, s, a);
+  }
+  ISPM_HIP(hipGetLastError());
+  return ISPM_OK;
+}
+
+template<typename T, typename E, int K>
+int launch_h(const ispm_launch *p, const ispm_slell_desc *A, const void *x, size_t ldx,
+             void *y, size_t ldy, unsigned H, const unsigned *skip = nullptr)
+{
+  if (H == 2) return launch_tv<T, E, 2, K>(p, A, x, ldx, y, ldy, skip);
+  return launch_tv<T, E, 1, K>(p, A, x, ldx, y, ldy, skip);
+}
+
+template<int K, int HFIX>
+int launch_types_h(const ispm_launch *p, const ispm_slell_desc *A, int vt, const void *x,
+                   size_t ldx, void *y, size_t ldy)
+{
+  const int et = A->elm_type;
+  if (vt == ISPM_F64 && et == ISPM_F64) return launch_tv<double, double, HFIX, K>(p, A, x, ldx, y, ldy);
+  if (vt == ISPM_F64 && et == ISPM_F32) return launch_tv<double, float, HFIX, K>(p, A, x, ldx, y, ldy);
+  if (vt == ISPM_F32 && et == ISPM_F32) return launch_tv<float, float, HFIX, K>(p, A, x, ldx, y, ldy);
+  if (vt == ISPM_F32 && et == ISPM_BF16) return launch_tv<float, bf16_t, HFIX, K>(p, A, x, ldx, y, ldy);
+  return fail(ISPM_ERR_UNSUPPORTED, "spmm: unsupported (vector, element) types (%d, %d)", vt, et);
+}
+
+template<int K>
+int launch_types(const ispm_launch *p, const ispm_slell_desc *A, int vt, const void *x,
+                 size_t ldx, void *y, size_t ldy, unsigned H, const unsigned *skip = nullptr)
+{
+  const int et = A->elm_type;
+  if (vt == ISPM_F64 && et == ISPM_F64) return launch_h<double, double, K>(p, A, x, ldx, y, ldy, H, skip);
+  if (vt == ISPM_F64 && et == ISPM_F32) return launch_h<double, float, K>(p, A, x, ldx, y, ldy, H, skip);
+  if (vt == ISPM_F32 && et == ISPM_F32) return launch_h<float, float, K>(p, A, x, ldx, y, ldy, H, skip);
+  if (vt == ISPM_F32 && et == ISPM_BF16) return launch_h<float, bf16_t, K>(p, A, x, ldx, y, ldy, H, skip);
+  return fail(ISPM_ERR_UNSUPPORTED, "spmv: unsupported (vector, element) types (%d, %d)", vt, et);
+}
+
+inline bool pow2(unsigned v) { return v && !(v & (v - 1)); }
+
+inline int check_args(const ispm_launch *p, const ispm_slell_desc *A, int vt, const void *x, void *y,
+               unsigned &H)
+{
+  if (!p || !A || !x || !y) return fail(ISPM_ERR_INVALID, "spmv: null argument");
+  /* reference validity rule (cuda/spmv/kernel-iface.h:4) and supported sets
+   * (cuda/spmv/dispatch.h:127-181): otherwise nothing is launched */
+  if (!(p->N >= p->S / (p->H ? p->H : 1) && p->S >= p->H))
+    return fail(ISPM_ERR_UNSUPPORTED, "spmv: invalid launch parameters N=%u S=%u H=%u", p->N, p->S, p->H);
+  if (!pow2(p->N) || p->N < 64 || p->N > 1024 || (p->H != 1 && p->H != 2))
+    return fail(ISPM_ERR_UNSUPPORTED, "spmv: unsupported launch N=%u H=%u", p->N, p->H);
+  if (A->hblock != 1 && A->hblock != 2)
+    return fail(ISPM_ERR_UNSUPPORTED, "spmv: hblock %u unsupported", A->hblock);
+  if (A->hblock == 2 && A->diags)
+    return fail(ISPM_ERR_INVALID, "spmv: hblock and diagonal slices are exclusive");
+  if (!A->var_height && !pow2(A->slice_height))
+    return fail(ISPM_ERR_UNSUPPORTED, "spmv: slice height %u is not a power of two", A->slice_height);
+  if (A->n_cols <= 0 && A->n_slices) return fail(ISPM_ERR_INVALID, "spmv: n_cols must be positive");
+  (void)vt;
+  H = p->H;
+  /* two-row vector stores need 2-element alignment of y (and x is never vector-loaded) */
+  const size_t ts = type_size(vt);
+  if (H == 2 && (reinterpret_cast<uintptr_t>(y) % (2 * ts))) H = 1;
+  if (H == 2 && !A->var_height && A->slice_height < 2) H = 1;
+  return ISPM_OK;
+}
+
+}  // namespace spmvk
+}  // namespace ispm
+
+#endif

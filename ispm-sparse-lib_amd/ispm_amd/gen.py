@@ -1,0 +1,217 @@
+"""Seeded synthetic workloads (host, numpy).
+
+Random numbers come from a SplitMix64 stream: value i (1-based) of stream `seed` is
+mix(seed + i * 0x9E3779B97F4A7C15); uniform doubles take the top 53 bits.  Every generator
+is deterministic in its seed.
+
+Workloads (row counts and nnz are exact):
+  poisson7(32)            7-point Dirichlet Poisson, diagonal 6 + 1e-3, off-diagonals -1,
+                          off-grid neighbours dropped: n = 32,768, nnz = 223,232.
+  fv7_lognormal(128)      7-point finite-volume matrix on a 128^3 hex mesh, face
+                          conductances lognormal(0, 1.5), Dirichlet boundary faces add their
+                          conductance to the diagonal; rows randomly permuted and then
+                          RCM-reordered: n = 2,097,152, nnz = 14,581,760.
+  stencil27_contrast(192) 27-point stencil, coefficient kappa in {1, 1e4} per random 8^3
+                          block, a_ij = -(kappa_i + kappa_j)/2, Dirichlet neighbours add
+                          kappa_i to the diagonal: n = 7,077,888, nnz = 189,119,224.  Every
+                          value is a multiple of 1/2 below 2^19, so FP32 storage is exact.
+  rgg_laplacian(n, k)     graph Laplacian of a 3-D random geometric k-nearest-neighbour
+                          graph plus 1e-2 I.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import CSR
+
+GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(seed: int, n: int, offset: int = 0) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        i = np.arange(offset + 1, offset + n + 1, dtype=np.uint64)
+        z = np.uint64(seed & 0xFFFFFFFFFFFFFFFF) + i * GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform01(seed: int, n: int, offset: int = 0) -> np.ndarray:
+    return (splitmix64(seed, n, offset) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+
+
+def uniform(seed: int, n: int, lo=-1.0, hi=1.0) -> np.ndarray:
+    return lo + (hi - lo) * uniform01(seed, n)
+
+
+def normal(seed: int, n: int) -> np.ndarray:
+    """Box-Muller on consecutive pairs of the stream."""
+    m = (n + 1) // 2
+    u = uniform01(seed, 2 * m)
+    u1 = 1.0 - u[0::2]
+    u2 = u[1::2]
+    r = np.sqrt(-2.0 * np.log(u1))
+    z = np.empty(2 * m)
+    z[0::2] = r * np.cos(2 * np.pi * u2)
+    z[1::2] = r * np.sin(2 * np.pi * u2)
+    return z[:n]
+
+
+def _stencil_csr(nx, ny, nz, offsets, val_fn, dtype=np.float64):
+    """CSR of a structured-grid stencil.  offsets: list of (dx,dy,dz); val_fn(mask, rows,
+    cols, off_index) -> values for the valid entries of one offset.  Rows are sorted by
+    column because offsets are visited in increasing linear offset."""
+    n = nx * ny * nz
+    lin = [dz * nx * ny + dy * nx + dx for dx, dy, dz in offsets]
+    order = np.argsort(lin, kind="stable")
+    idx = np.arange(n, dtype=np.int64)
+    x = idx % nx
+    y = (idx // nx) % ny
+    z = idx // (nx * ny)
+    K = len(offsets)
+    cols = np.empty((n, K), np.int32)
+    vals = np.zeros((n, K), dtype)
+    mask = np.empty((n, K), bool)
+    for j, o in enumerate(order):
+        dx, dy, dz = offsets[o]
+        m = (x + dx >= 0) & (x + dx < nx) & (y + dy >= 0) & (y + dy < ny) & (z + dz >= 0) & (z + dz < nz)
+        mask[:, j] = m
+        cols[:, j] = (idx + lin[o]).astype(np.int32)
+        vals[:, j] = val_fn(m, idx, (dx, dy, dz))
+    counts = mask.sum(1)
+    row_ptr = np.zeros(n + 1, np.int64)
+    np.cumsum(counts, out=row_ptr[1:])
+    c = cols[mask]
+    v = vals[mask]
+    del cols, vals, mask
+    return CSR(n, n, row_ptr.astype(np.int32), c, v)
+
+
+def poisson7(m: int = 32, shift: float = 1e-3) -> CSR:
+    offs = [(0, 0, -1), (0, -1, 0), (-1, 0, 0), (0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)]
+
+    def val(mask, idx, o):
+        return np.where(mask, (6.0 + shift) if o == (0, 0, 0) else -1.0, 0.0)
+
+    return _stencil_csr(m, m, m, offs, val)
+
+
+def stencil27_contrast(m: int = 192, block: int = 8, contrast: float = 1e4, seed: int = 3001) -> CSR:
+    nb = m // block
+    kb = np.where(uniform01(seed, nb ** 3) < 0.5, 1.0, contrast)
+    idx = np.arange(m ** 3, dtype=np.int64)
+    x = idx % m
+    y = (idx // m) % m
+    z = idx // (m * m)
+    kappa = kb[(x // block) + nb * ((y // block) + nb * (z // block))]
+    del x, y, z
+    offs = [(dx, dy, dz) for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1)]
+    diag = np.zeros(m ** 3)
+
+    def val(mask, idx_, o):
+        if o == (0, 0, 0):
+            return np.zeros(idx_.size)
+        lin = o[2] * m * m + o[1] * m + o[0]
+        nbr = np.clip(idx_ + lin, 0, m ** 3 - 1)
+        kn = np.where(mask, kappa[nbr], kappa)        # off-grid neighbour: Dirichlet, kappa_i
+        c = 0.5 * (kappa + kn)
+        diag[:] += c
+        return np.where(mask, -c, 0.0)
+
+    A = _stencil_csr(m, m, m, offs, val)
+    # write the diagonal (one entry per row, the 14th offset in sorted order when interior)
+    rp = A.row_ptr.astype(np.int64)
+    rows = np.repeat(np.arange(A.n_rows), np.diff(rp))
+    is_diag = A.cols == rows
+    A.vals[is_diag] = diag
+    return A
+
+
+def fv7_lognormal(m: int = 128, sigma: float = 1.5, seed: int = 2001, permute_seed: int = 2002,
+                  rcm: bool = True) -> CSR:
+    import scipy.sparse as sp
+    n = m ** 3
+    # conductance per face: x-faces (m+1)*m*m incl. the two boundary layers, same for y, z
+    cx = np.exp(sigma * normal(seed, (m + 1) * m * m)).reshape(m, m, m + 1)      # [z, y, xface]
+    cy = np.exp(sigma * normal(seed + 1, (m + 1) * m * m)).reshape(m, m + 1, m)  # [z, yface, x]
+    cz = np.exp(sigma * normal(seed + 2, (m + 1) * m * m)).reshape(m + 1, m, m)  # [zface, y, x]
+    diag = (cx[:, :, :-1] + cx[:, :, 1:] + cy[:, :-1, :] + cy[:, 1:, :] + cz[:-1] + cz[1:]).ravel()
+    idx = np.arange(n).reshape(m, m, m)
+    rows, cols, vals = [np.arange(n)], [np.arange(n)], [diag]
+    for a, b, c in ((idx[:, :, :-1], idx[:, :, 1:], cx[:, :, 1:-1]),
+                    (idx[:, :-1, :], idx[:, 1:, :], cy[:, 1:-1, :]),
+                    (idx[:-1], idx[1:], cz[1:-1])):
+        a, b, c = a.ravel(), b.ravel(), c.ravel()
+        rows += [a, b]
+        cols += [b, a]
+        vals += [-c, -c]
+    M = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+    perm = np.argsort(uniform01(permute_seed, n), kind="stable")
+    M = M[perm][:, perm].tocsr()
+    if rcm:
+        from scipy.sparse.csgraph import reverse_cuthill_mckee
+        p = reverse_cuthill_mckee(M, symmetric_mode=True)
+        M = M[p][:, p].tocsr()
+    M.sort_indices()
+    return CSR.from_scipy(M)
+
+
+def rgg_laplacian(n: int, k: int, seed: int, shift: float = 1e-2, dtype=np.float64) -> CSR:
+    """Graph Laplacian of a 3-D random geometric kNN graph (symmetrised) plus shift*I."""
+    import scipy.sparse as sp
+    from scipy.spatial import cKDTree
+    pts = uniform01(seed, 3 * n).reshape(n, 3)
+    _, nb = cKDTree(pts).query(pts, k=k + 1)
+    r = np.repeat(np.arange(n), k)
+    c = nb[:, 1:].ravel()
+    W = sp.csr_matrix((np.ones(r.size), (r, c)), shape=(n, n))
+    W = ((W + W.T) > 0).astype(np.float64)
+    L = sp.diags(np.asarray(W.sum(1)).ravel() + shift) - W
+    L = L.tocsr()
+    L.sort_indices()
+    out = CSR.from_scipy(L)
+    if dtype != np.float64:
+        out.vals = out.vals.astype(dtype)
+    return out
+
+
+def random_sparse(n: int, seed: int, min_len=1, max_len=40, n_cols=None, integer=False,
+                  pair_bias=0.0, spd=False) -> CSR:
+    """Irregular test matrix: row lengths uniform in [min_len, max_len]; with pair_bias > 0
+    that fraction of entries is emitted as aligned column pairs (2c, 2c+1)."""
+    n_cols = n if n_cols is None else n_cols
+    u = uniform01(seed, 4 * n * max_len + 4 * n)
+    pos = 0
+    rows, cols = [], []
+    for r in range(n):
+        L = min_len + int(u[pos] * (max_len - min_len + 1)); pos += 1
+        cs = set()
+        while len(cs) < min(L, n_cols):
+            if pair_bias and u[pos] < pair_bias:
+                c0 = 2 * int(u[pos + 1] * (n_cols // 2))
+                cs.add(c0)
+                if c0 + 1 < n_cols:
+                    cs.add(c0 + 1)
+                pos += 2
+            else:
+                cs.add(int(u[pos] * n_cols)); pos += 1
+            if pos + 4 >= u.size:
+                u = np.concatenate([u, uniform01(seed + 7919, u.size, offset=u.size)])
+        cs = sorted(cs)[:max(L, 1)]
+        rows += [r] * len(cs)
+        cols += cs
+    rows = np.array(rows, np.int64)
+    cols = np.array(cols, np.int64)
+    vals = uniform(seed + 1, rows.size)
+    if integer:
+        vals = np.round(vals * 8)
+    import scipy.sparse as sp
+    M = sp.csr_matrix((vals, (rows, cols)), shape=(n, n_cols))
+    if spd:
+        M = M + M.T
+        M = M + sp.diags(np.asarray(abs(M).sum(1)).ravel() + 1.0)
+    M = M.tocsr()
+    M.sort_indices()
+    return CSR.from_scipy(M)

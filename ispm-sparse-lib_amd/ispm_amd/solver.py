@@ -1,0 +1,103 @@
+"""Device-resident FSAI-preconditioned CG (ispm_solver_* in include/ispm.h)."""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import (BF16, CSR, F32, F64, DeviceArray, Plan, check, lib)
+
+PPCG, PCG = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+
+
+class SolverOpts(C.Structure):
+    _fields_ = [("method", C.c_int), ("precision", C.c_int), ("tol", C.c_double),
+                ("maxit", C.c_int), ("check_every", C.c_int), ("use_graph", C.c_int),
+                ("reliable_delta", C.c_double)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("replacements", C.c_int), ("converged", C.c_int),
+                ("relres", C.c_double), ("l1_resnorm", C.c_double)]
+
+
+def _bind():
+    L = lib()
+    L.ispm_solver_create.restype = C.c_int
+    L.ispm_solver_create.argtypes = [C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.POINTER(SolverOpts)]
+    L.ispm_solver_solve.restype = C.c_int
+    L.ispm_solver_solve.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(SolveStats)]
+    L.ispm_solver_destroy.restype = None
+    L.ispm_solver_destroy.argtypes = [C.c_void_p]
+    return L
+
+
+class Solver:
+    """Owns the plans of A and of the factors and a device solver object.
+
+    precision="fp64": A in FP64, factors in FP64 (or the element type of the given factor
+    CSRs), FP64 vectors.  precision="mixed": A in FP32, factors in BF16 (factor_type), FP32
+    vectors, FP64 scalars, reliable FP64 residual replacement."""
+
+    def __init__(self, A: CSR, M=None, precision="fp64", method=None, tol=1e-10, maxit=100000,
+                 check_every=16, use_graph=True, reliable_delta=0.1, factor_type=None,
+                 A_plan=None, M_plans=None):
+        L = _bind()
+        mixed = precision == "mixed"
+        vt = F32 if mixed else F64
+        if method is None:
+            method = PCG if mixed else PPCG
+        self.A_plan = A_plan or Plan(A, F64, elm_type=F32 if mixed else F64)
+        self.M_plans = None
+        if M_plans is not None:
+            self.M_plans = M_plans
+        elif M is not None:
+            ft = factor_type if factor_type is not None else (BF16 if mixed else F64)
+            self.M_plans = (Plan(M[0], vt, elm_type=ft), Plan(M[1], vt, elm_type=ft))
+        self.n = self.A_plan.n_rows_padded
+        o = SolverOpts(method, PREC_MIXED if mixed else PREC_FP64, tol, maxit, check_every,
+                       int(use_graph), reliable_delta)
+        p = C.c_void_p()
+        m1 = self.M_plans[0].p if self.M_plans else None
+        m2 = self.M_plans[1].p if self.M_plans else None
+        check(L.ispm_solver_create(C.byref(p), self.A_plan.p, m1, m2, C.byref(o)))
+        self.p = p
+
+    def solve_device(self, b: DeviceArray, x: DeviceArray, allow_fail=False):
+        st = SolveStats()
+        code = lib().ispm_solver_solve(self.p, b.vp, x.vp, C.byref(st))
+        if code != 0 and not (allow_fail and code == 6):
+            check(code)
+        return st
+
+    def __del__(self):
+        try:
+            lib().ispm_solver_destroy(self.p)
+        except Exception:
+            pass
+
+
+@dataclass
+class Result:
+    x: np.ndarray
+    iterations: int
+    replacements: int
+    relres: float
+    converged: bool
+    l1_resnorm: float
+
+
+def solve(A: CSR, b, M=None, tol=1e-10, precision="fp64", method=None, x0=None, **kw) -> Result:
+    S = Solver(A, M=M, precision=precision, method=method, tol=tol, **kw)
+    n = S.n
+    bd = DeviceArray.from_numpy(np.concatenate([np.asarray(b, np.float64), np.zeros(n - A.n_rows)]))
+    x = np.zeros(n)
+    if x0 is not None:
+        x[:A.n_rows] = x0
+    xd = DeviceArray.from_numpy(x)
+    st = S.solve_device(bd, xd, allow_fail=True)
+    return Result(xd.numpy()[:A.n_rows], st.iterations, st.replacements, st.relres,
+                  bool(st.converged), st.l1_resnorm)

@@ -1,0 +1,38 @@
+"""The C-ABI library loads and exports every entry point declared in include/ispm.h (no GPU
+work is done here)."""
+import ctypes
+import os
+import re
+
+import ispm_amd as I
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    src = open(os.path.join(ROOT, "include", "ispm.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(ispm_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_symbols_exported():
+    names = declared()
+    assert len(names) > 40
+    lib = ctypes.CDLL(I.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_header():
+    assert set(declared()) <= set(I.declared_symbols())
+
+
+def test_static_archive_built():
+    assert os.path.exists(os.path.join(I.PKG_DIR, "libispm0-pic.a"))
+
+
+def test_cpu_only_calls():
+    # host-side conversion works without a device
+    A = I.CSR(2, 2, [0, 1, 2], [0, 1], [1.0, 2.0])
+    hs = I.csr_to_slell(A, 32)
+    assert hs.n_rows == 512 and hs.n_slices == 16

@@ -1,0 +1,194 @@
+"""GPU parity of the SlELL SpMV / SpMM against the CPU oracle (CSR product in FP64).
+
+Bars: integer-valued A and x -> bit-exact; FP64 -> |dy| <= 1e-13 sum_k |a_rk x_k| per entry;
+FP32 vectors -> |dy| <= 1e-5 sum_k |a_rk x_k| (bf16/fp32 element values are compared against
+the oracle run on the same rounded values, so only the accumulation differs).
+"""
+import numpy as np
+import pytest
+
+import ispm_amd as I
+import ispm_amd.gen as G
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FORMATS = [dict(S=32), dict(S=64), dict(S=256), dict(S=64, D=True), dict(S=256, D=True),
+           dict(S=128, hblock=2), dict(S=32, hblock=2), dict(S=256, V=True),
+           dict(S=64, V=True, D=True), dict(S=128, V=True, hblock=2), dict(S=512)]
+LAUNCH = [(64, 1), (128, 1), (256, 2), (512, 1), (1024, 2), (64, 2)]
+TYPES = [(I.F64, I.F64), (I.F64, I.F32), (I.F32, I.F32), (I.F32, I.BF16)]
+
+
+def rounded_vals(A, et):
+    v = A.vals.astype(np.float64)
+    if et == I.F32:
+        return v.astype(np.float32).astype(np.float64)
+    if et == I.BF16:
+        f = v.astype(np.float32).view(np.uint32).astype(np.uint64)
+        f = ((f + 0x7FFF + ((f >> 16) & 1)) >> 16) << 16
+        return f.astype(np.uint32).view(np.float32).astype(np.float64)
+    return v
+
+
+def run_spmv(hs, x, vt, N, H):
+    dm = I.DeviceSlell(hs)
+    dt = np.float64 if vt == I.F64 else np.float32
+    xv = np.zeros(max(hs.n_rows, hs.n_cols), dt)
+    xv[:x.size] = x
+    xd = I.DeviceArray.from_numpy(xv)
+    yd = I.DeviceArray(hs.n_rows, dt)
+    yd.upload(np.full(hs.n_rows, 7.0, dt))
+    st = dm.spmv(xd, yd, vt, N=N, S=hs.slice_height if not hs.var_height else N * H, H=H)
+    I.synchronize()
+    return st, yd.numpy()
+
+
+def check(A, hs, x, y, vt, et, integer=False):
+    Ar = I.CSR(A.n_rows, A.n_cols, A.row_ptr, A.cols, rounded_vals(A, et))
+    xr = x.astype(np.float64) if vt == I.F64 else x.astype(np.float32).astype(np.float64)
+    yref = O.csr_spmv(Ar, xr)
+    scale = O.csr_spmv_abs(Ar, xr)
+    got = y[:A.n_rows].astype(np.float64)
+    if integer:
+        np.testing.assert_array_equal(got, yref)
+    else:
+        tol = 1e-13 if vt == I.F64 else 1e-5
+        bad = np.abs(got - yref) > tol * scale + 1e-300
+        assert not bad.any(), f"{bad.sum()} rows off; worst {np.abs(got - yref)[bad].max()}"
+    # padded rows are written as zero
+    assert np.all(y[A.n_rows:] == 0)
+
+
+@pytest.fixture(scope="module")
+def mats(gpu):
+    return {
+        "poisson7": G.poisson7(16),
+        "irregular": G.random_sparse(3000, 11, min_len=1, max_len=60),
+        "pairs": G.random_sparse(2500, 12, min_len=2, max_len=30, pair_bias=0.7),
+        "skewed": G.random_sparse(1800, 13, min_len=0, max_len=200),
+    }
+
+
+@pytest.mark.parametrize("fmt", FORMATS, ids=lambda f: "-".join(f"{k}{int(v)}" for k, v in f.items()))
+@pytest.mark.parametrize("name", ["poisson7", "irregular", "pairs", "skewed"])
+def test_spmv_formats_fp64(mats, name, fmt):
+    A = mats[name]
+    hs = I.csr_to_slell(A, fmt["S"], fmt.get("hblock", 1), fmt.get("V", False), fmt.get("D", False))
+    x = G.uniform(100, A.n_cols)
+    ran = 0
+    for N, H in LAUNCH:
+        S = hs.slice_height if not hs.var_height else N * H
+        if not (N >= S // H and S >= H):
+            continue
+        st, y = run_spmv(hs, x, I.F64, N, H)
+        assert st == 0, I.lib().ispm_last_error()
+        check(A, hs, x, y, I.F64, I.F64)
+        ran += 1
+    assert ran > 0
+
+
+@pytest.mark.parametrize("vt,et", TYPES)
+@pytest.mark.parametrize("fmt", [dict(S=64, D=True), dict(S=128, hblock=2), dict(S=256, V=True), dict(S=32)],
+                         ids=lambda f: "-".join(f"{k}{int(v)}" for k, v in f.items()))
+def test_spmv_types(mats, vt, et, fmt):
+    A = mats["pairs"] if fmt.get("hblock") else mats["irregular"]
+    hs = I.csr_to_slell(A, fmt["S"], fmt.get("hblock", 1), fmt.get("V", False), fmt.get("D", False), out_type=et)
+    x = G.uniform(101, A.n_cols)
+    N, H = (256, 1)
+    st, y = run_spmv(hs, x, vt, N, H)
+    assert st == 0, I.lib().ispm_last_error()
+    check(A, hs, x, y, vt, et)
+
+
+@pytest.mark.parametrize("fmt", FORMATS[:8], ids=lambda f: "-".join(f"{k}{int(v)}" for k, v in f.items()))
+def test_spmv_integer_exact(mats, fmt):
+    A0 = G.random_sparse(2000, 21, min_len=0, max_len=50, integer=True, pair_bias=0.5)
+    hs = I.csr_to_slell(A0, fmt["S"], fmt.get("hblock", 1), fmt.get("V", False), fmt.get("D", False))
+    x = np.round(G.uniform(102, A0.n_cols) * 16)
+    for N, H in ((64, 1), (256, 2), (1024, 1)):
+        S = hs.slice_height if not hs.var_height else N * H
+        if not (N >= S // H and S >= H):
+            continue
+        st, y = run_spmv(hs, x, I.F64, N, H)
+        assert st == 0
+        check(A0, hs, x, y, I.F64, I.F64, integer=True)
+
+
+def test_invalid_launch_is_noop(mats):
+    A = mats["poisson7"]
+    hs = I.csr_to_slell(A, 512)
+    st, y = run_spmv(hs, G.uniform(1, A.n_cols), I.F64, 64, 1)   # N < S/H: reference no-op
+    assert st == 2
+    assert np.all(y == 7.0)
+
+
+def test_spmm_matches_spmv(mats):
+    A = mats["irregular"]
+    for fmt in (dict(S=64, D=True), dict(S=128), dict(S=256, V=True)):
+        p = I.Plan(A, I.F64, explicit=fmt)
+        n = p.n_rows_padded
+        for k in (3, 8):
+            X = np.zeros((k, n))
+            X[:, :A.n_cols] = G.uniform(200 + k, k * A.n_cols).reshape(k, A.n_cols)
+            Xd = I.DeviceArray.from_numpy(X.ravel())
+            Yd = I.DeviceArray(k * n)
+            p.execute_multi(k, Xd, n, Yd, n)
+            Y = Yd.numpy().reshape(k, n)
+            for j in range(k):
+                xd = I.DeviceArray.from_numpy(X[j])
+                yd = I.DeviceArray(n)
+                p.execute(xd, yd)
+                np.testing.assert_array_equal(Y[j], yd.numpy())
+
+
+def test_plan_default_and_update_coefs(mats):
+    A = mats["poisson7"]
+    p = I.Plan(A, I.F64)
+    inf = p.info()
+    assert inf.D == 1 and inf.S == 256
+    n = p.n_rows_padded
+    x = G.uniform(5, A.n_rows)
+    xd = I.DeviceArray.from_numpy(np.concatenate([x, np.zeros(n - A.n_rows)]))
+    yd = I.DeviceArray(n)
+    p.execute(xd, yd)
+    check(A, None, x, yd.numpy(), I.F64, I.F64)
+    A2 = I.CSR(A.n_rows, A.n_cols, A.row_ptr, A.cols, A.vals * 3.0 + 1.0)
+    p.update_coefs(A2)
+    p.execute(xd, yd)
+    check(A2, None, x, yd.numpy(), I.F64, I.F64)
+
+
+@pytest.mark.parametrize("vt,et", [(I.F64, I.F64), (I.F64, I.F32), (I.F32, I.F32)])
+def test_full_size_27pt_192(gpu, vt, et):
+    """27-point 192^3 contrast matrix at full size (7,077,888 rows): exact A*1 (all values are
+    multiples of 1/2, so FP64 sums are exact in any order) and the per-entry FP64 bound on a
+    random x against the oracle CSR product."""
+    A = _big()
+    p = I.Plan(A, vt, elm_type=et)
+    inf = p.info()
+    assert inf.D == 1 and inf.S == 64
+    n = p.n_rows_padded
+    dt = np.float64 if vt == I.F64 else np.float32
+    ones = I.DeviceArray.from_numpy(np.ones(n, dt))
+    yd = I.DeviceArray(n, dt)
+    p.execute(ones, yd)
+    y = yd.numpy()[:A.n_rows].astype(np.float64)
+    rs = O.csr_spmv(A, np.ones(A.n_rows))
+    if vt == I.F64:
+        np.testing.assert_array_equal(y, rs)
+    else:
+        assert np.all(np.abs(y - rs) <= 1e-5 * O.csr_spmv_abs(A, np.ones(A.n_rows)))
+    x = G.uniform(7, A.n_rows)
+    xd = I.DeviceArray.from_numpy(np.concatenate([x, np.zeros(n - A.n_rows)]).astype(dt))
+    p.execute(xd, yd)
+    check(A, None, x, yd.numpy(), vt, et)
+
+
+_BIG = {}
+
+
+def _big():
+    if "A" not in _BIG:
+        _BIG["A"] = G.stencil27_contrast(192)
+    return _BIG["A"]
