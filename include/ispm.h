@@ -202,6 +202,13 @@ int ispm_plan_create_explicit(ispm_plan **out, int vec_type, int elm_type, const
                               unsigned S, unsigned hblock, int V, int D,
                               const ispm_launch *launch);
 int ispm_plan_info_get(const ispm_plan *p, ispm_plan_info *info);
+/* Launch shape the default plan uses for a format (CDNA4 heuristic: one row per thread,
+ * workgroups of max(64, min(1024, S)) threads; 256 threads for variable-height slices). */
+int ispm_plan_default_launch(unsigned slice_height, int var_height, ispm_launch *out);
+/* The PLAN_EXHAUSTIVE search (plan/plan.h:97-144): times every (S, V, D, hblock) format and
+ * launch shape on the device and returns the fastest. */
+int ispm_plan_tune(int vec_type, int elm_type, const ispm_csr *A, unsigned *S, unsigned *hblock,
+                   int *V, int *D, ispm_launch *launch, double *seconds);
 /* spmv_plan::execute_spmv (plan/plan.h:148-161). */
 int ispm_plan_execute(ispm_plan *p, const void *x, void *y);
 int ispm_plan_execute_multi(ispm_plan *p, int k, const void *X, size_t ldx, void *Y, size_t ldy);

@@ -432,6 +432,33 @@ int ispm_plan_create_explicit(ispm_plan **out, int vt, int et, const ispm_csr *A
   return ISPM_OK;
 }
 
+int ispm_plan_default_launch(unsigned slice_height, int var_height, ispm_launch *out)
+{
+  if (!out) return fail_plain(ISPM_ERR_INVALID, "plan_default_launch: null output");
+  conv::layout L;
+  L.S = slice_height;
+  L.V = var_height != 0;
+  *out = default_launch(L);
+  return ISPM_OK;
+}
+
+int ispm_plan_tune(int vt, int et, const ispm_csr *A, unsigned *S, unsigned *hblock, int *V,
+                   int *D, ispm_launch *launch, double *seconds)
+{
+  int st = check_csr(A);
+  if (st) return st;
+  if (!S || !hblock || !V || !D || !launch) return fail_plain(ISPM_ERR_INVALID, "plan_tune: null output");
+  double best = 0;
+  ispm_launch lp{};
+  unsigned s = 0, w = 1;
+  int v = 0, d = 0;
+  st = tune_exhaustive(A, vt, et, s, w, v, d, lp, best);
+  if (st) return st;
+  *S = s; *hblock = w; *V = v; *D = d; *launch = lp;
+  if (seconds) *seconds = best;
+  return ISPM_OK;
+}
+
 int ispm_plan_info_get(const ispm_plan *p, ispm_plan_info *info)
 {
   if (!p || !info) return fail_plain(ISPM_ERR_INVALID, "plan_info: null argument");

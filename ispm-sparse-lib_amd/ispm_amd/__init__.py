@@ -120,6 +120,10 @@ _SIGS = {
     "ispm_plan_create_explicit": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.POINTER(Csr),
                                             C.c_uint, C.c_uint, C.c_int, C.c_int, C.POINTER(Launch)]),
     "ispm_plan_info_get": (C.c_int, [C.c_void_p, C.POINTER(PlanInfo)]),
+    "ispm_plan_default_launch": (C.c_int, [C.c_uint, C.c_int, C.POINTER(Launch)]),
+    "ispm_plan_tune": (C.c_int, [C.c_int, C.c_int, C.POINTER(Csr), C.POINTER(C.c_uint), C.POINTER(C.c_uint),
+                                 C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(Launch),
+                                 C.POINTER(C.c_double)]),
     "ispm_plan_execute": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "ispm_plan_execute_multi": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t,
                                           C.c_void_p, C.c_size_t]),

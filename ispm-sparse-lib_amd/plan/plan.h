@@ -1,0 +1,93 @@
+/* plan.h -- SpMV plan (plan/plan.h:1-161 of the reference), same members and methods.
+ * PLAN_DEFAULT / PLAN_ESTIMATE take the guessed format with a CDNA4 launch shape;
+ * PLAN_EXHAUSTIVE times every format and launch shape on the device (ispm_plan_tune). */
+#ifndef ISPM_PLAN_H
+#define ISPM_PLAN_H
+
+#include <cstdio>
+#include <cstdlib>
+#include "formats/formats.h"
+#include "formats/conversion.h"
+#include "cuda/spmv/kernel-iface.h"
+#include "util/cuda/timer.h"
+
+enum spmv_plan_kind {
+  PLAN_DEFAULT,
+  PLAN_ESTIMATE,
+  PLAN_EXHAUSTIVE,
+};
+
+enum spmv_plan_flags {
+  SPMV_NO_TEXTURE_MEMORY = 001,
+  SPMV_PLAN_NO_DEVICE = 002
+};
+
+template<class T, class E = T>
+struct spmv_plan
+{
+  slell_matrix<E, host_memory_space_tag> *host_matrix;
+  slell_matrix<E, device_memory_space_tag> *device_matrix;
+  slell_params mparams;
+  spmv_launch_params params;
+
+  spmv_plan() : host_matrix(nullptr), device_matrix(nullptr) {}
+  spmv_plan(const csr_matrix<E> &m, spmv_plan_kind kind = PLAN_DEFAULT, unsigned flags = 0);
+  /* a plan owns its matrices: copying would free them twice */
+  spmv_plan(const spmv_plan &) = delete;
+  spmv_plan &operator=(const spmv_plan &) = delete;
+  ~spmv_plan()
+  {
+    delete host_matrix;
+    delete device_matrix;
+  }
+
+  void update_device_matrix_coefs() { device_matrix->elms = host_matrix->elms; }
+
+  void upload()
+  {
+    delete device_matrix;
+    device_matrix = new slell_matrix<E, device_memory_space_tag>(*host_matrix);
+  }
+
+  void change_host_matrix(const csr_matrix<E> &m)
+  {
+    delete host_matrix;
+    host_matrix = new slell_matrix<E, host_memory_space_tag>(
+        csr_to_slell(m, mparams.S, mparams.H, mparams.V, mparams.D));
+  }
+
+  void execute_spmv(vector<T, device_memory_space_tag> &x, vector<T, device_memory_space_tag> &y)
+  {
+    launch_spmv_kernel<T, E>(params, *device_matrix, x, y);
+  }
+
+  void execute_spmv(T *xptr, T *yptr) { launch_spmv_kernel<T, E>(params, *device_matrix, xptr, yptr); }
+};
+
+template<class T, class E>
+spmv_plan<T, E>::spmv_plan(const csr_matrix<E> &m, spmv_plan_kind kind, unsigned flags)
+  : host_matrix(nullptr), device_matrix(nullptr)
+{
+  const bool usetex = !(flags & SPMV_NO_TEXTURE_MEMORY);
+  const bool up = !(flags & SPMV_PLAN_NO_DEVICE);
+  ispm_launch lp;
+  if (kind == PLAN_EXHAUSTIVE) {
+    ispm_csr c = ispm_detail::csr_view(m);
+    unsigned S, W;
+    int V, D;
+    double secs;
+    ISPM_CHECK(ispm_plan_tune(ispm_type_code<T>::value, ispm_type_code<E>::value, &c, &S, &W, &V, &D, &lp, &secs));
+    mparams = slell_params{S, W, (unsigned)V, (unsigned)D};
+    host_matrix = new slell_matrix<E, host_memory_space_tag>(csr_to_slell(m, S, W, V != 0, D != 0));
+    upload();
+  } else {
+    host_matrix = new slell_matrix<E, host_memory_space_tag>(csr_to_slell_guess_format(m));
+    if (up) upload();
+    mparams = slell_params{host_matrix->slice_height, host_matrix->hblock,
+                           (unsigned)host_matrix->var_height, (unsigned)host_matrix->diags};
+    ISPM_CHECK(ispm_plan_default_launch(host_matrix->slice_height, host_matrix->var_height, &lp));
+  }
+  params = spmv_launch_params{lp.N, lp.S, lp.H, usetex, usetex};
+}
+
+#endif

@@ -1,0 +1,134 @@
+/* api_smoke.cpp -- a caller written against the reference's C++ API (plan/plan.h,
+ * formats/formats.h, fastainv/fastainv.h, util/cuda/sblas.h, util/cuda/initialize.h),
+ * compiled with g++ against the drop-in headers and linked to libispm0.so.  Prints
+ * "ok" lines; any failure aborts with a message.  Run by tests/test_gpu_cxx_api.py. */
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+#include "fastainv/droptol-tuning.h"
+#include "fastainv/fastainv.h"
+#include "plan/plan.h"
+#include "util/cuda/initialize.h"
+#include "util/cuda/pinner.h"
+#include "util/cuda/sblas.h"
+
+#define REQUIRE(c)                                                         \
+  do {                                                                     \
+    if (!(c)) {                                                            \
+      std::fprintf(stderr, "FAILED %s at line %d\n", #c, __LINE__);        \
+      std::exit(1);                                                        \
+    }                                                                      \
+  } while (0)
+
+static csr_matrix<double> poisson(int m)
+{
+  csr_matrix<double> A;
+  const int n = m * m * m;
+  A.n_rows = A.n_cols = n;
+  A.row_ptr.push_back(0);
+  for (int z = 0; z < m; z++)
+    for (int y = 0; y < m; y++)
+      for (int x = 0; x < m; x++) {
+        const int i = x + m * (y + m * z);
+        const int nb[7] = {z > 0 ? i - m * m : -1, y > 0 ? i - m : -1, x > 0 ? i - 1 : -1, i,
+                           x < m - 1 ? i + 1 : -1, y < m - 1 ? i + m : -1, z < m - 1 ? i + m * m : -1};
+        for (int k = 0; k < 7; k++)
+          if (nb[k] >= 0) {
+            A.cols.push_back(nb[k]);
+            A.elms.push_back(nb[k] == i ? 6.001 : -1.0);
+          }
+        A.row_ptr.push_back((int)A.cols.size());
+      }
+  A.n_nz = (int)A.cols.size();
+  return A;
+}
+
+static void count(void *v) { ++*static_cast<long *>(v); }
+
+int main()
+{
+  ispm_initialize();
+  std::printf("device %s\n", get_device_string().str);
+  csr_matrix<double> A = poisson(12);
+  const int n = A.n_rows;
+  std::vector<double> x(n);
+  for (int i = 0; i < n; i++) x[i] = std::sin(0.37 * i);
+  std::vector<double> yref(n, 0.0);
+  for (int r = 0; r < n; r++)
+    for (int k = A.row_ptr[r]; k < A.row_ptr[r + 1]; k++) yref[r] += A.elms[k] * x[A.cols[k]];
+
+  for (int kind = 0; kind <= 2; kind += 2) {
+    spmv_plan<double> plan(A, (spmv_plan_kind)kind);
+    REQUIRE(A.n_rows % 512 == 0);     /* the plan pads the caller's matrix */
+    vector<double, host_memory_space_tag> hx(A.n_rows, 0.0);
+    for (int i = 0; i < n; i++) hx[i] = x[i];
+    vector<double, device_memory_space_tag> dx(hx), dy(A.n_rows);
+    plan.execute_spmv(dx, dy);
+    vector<double, host_memory_space_tag> hy(dy);
+    double err = 0;
+    for (int i = 0; i < n; i++) err = std::fmax(err, std::fabs(hy[i] - yref[i]));
+    REQUIRE(err < 1e-12);
+    std::printf("ok spmv_plan kind=%d S=%u H=%u V=%u D=%u N=%u err=%.1e\n", kind, plan.mparams.S,
+                plan.mparams.H, plan.mparams.V, plan.mparams.D, plan.params.N, err);
+  }
+  {
+    /* coefficient refresh: same pattern, doubled values, re-converted with the saved
+     * format parameters and copied to the device */
+    spmv_plan<double> plan(A);
+    csr_matrix<double> A2 = A;
+    for (size_t k = 0; k < A2.elms.size(); k++) A2.elms[k] *= 2.0;
+    plan.change_host_matrix(A2);
+    plan.update_device_matrix_coefs();
+    vector<double, host_memory_space_tag> hx(A.n_rows, 0.0);
+    for (int i = 0; i < n; i++) hx[i] = x[i];
+    vector<double, device_memory_space_tag> dx(hx), dy(A.n_rows);
+    plan.execute_spmv(dx.udata(), dy.udata());
+    vector<double, host_memory_space_tag> hy(dy);
+    double err = 0;
+    for (int i = 0; i < n; i++) err = std::fmax(err, std::fabs(hy[i] - 2.0 * yref[i]));
+    REQUIRE(err < 1e-12);
+    /* an invalid launch (N < S/H) leaves y unwritten */
+    spmv_launch_params bad = plan.params;
+    bad.N = 64; bad.S = 512; bad.H = 1;
+    vector<double, device_memory_space_tag> dz(A.n_rows);
+    launch_spmv_kernel<double, double>(bad, *plan.device_matrix, dx, dz);
+    vector<double, host_memory_space_tag> hz(dz);
+    for (int i = 0; i < n; i++) REQUIRE(hz[i] == 0.0);
+    std::printf("ok coefficient refresh err=%.1e\n", err);
+  }
+  {
+    csr_matrix<double> B = poisson(6);
+    csr_matrix<float> a1, a2;
+    long calls = 0;
+    const bool ok = fastainv_sym<double, float>(a1, a2, B, count, &calls, 8192000, 0.05);
+    REQUIRE(ok);
+    REQUIRE(calls == B.n_rows && a1.n_rows == B.n_rows && a1.n_nz == a2.n_nz);
+    std::printf("ok fastainv nnz=%d\n", a1.n_nz);
+  }
+  {
+    const int m = 1000;
+    std::vector<double> a(m), b(m);
+    for (int i = 0; i < m; i++) { a[i] = i * 0.5; b[i] = 1.0 / (1 + i); }
+    vector<double, device_memory_space_tag> da(a.begin(), a.end()), db(b.begin(), b.end()), ds(1);
+    sblas::dot(da.data(), db.data(), ds.data(), m);
+    vector<double, host_memory_space_tag> hs(ds);
+    double ref = 0;
+    for (int i = 0; i < m; i++) ref += a[i] * b[i];
+    REQUIRE(std::fabs(hs[0] - ref) < 1e-12 * std::fabs(ref));
+    /* result written through a mapped pinned scalar */
+    std::vector<double, ispm_pinned_allocator<double> > pinned(1, 0.0);
+    pinmemptr<double> ps(pinned.data());
+    sblas::dot(da.data(), db.data(), ps(), m);
+    ispm_synchronize();
+    REQUIRE(std::fabs(*ps - ref) < 1e-12 * std::fabs(ref));
+    std::printf("ok sblas dot %.6f\n", hs[0]);
+  }
+  {
+    DroptolTuner t;
+    REQUIRE(std::fabs(t.getDroptol() - 0.006231) < 1e-6);
+    std::printf("ok droptol tuner\n");
+  }
+  std::printf("ok all\n");
+  return 0;
+}
