@@ -145,6 +145,16 @@ typedef struct ispm_csr {
   int elm_type;
 } ispm_csr;
 
+/* CSR arrays owned by the library (outputs of ispm_fastainv_sym and ispm_mm_read); release
+ * with ispm_csr_out_free. */
+typedef struct ispm_csr_out {
+  int n_rows, n_cols, n_nz;
+  int *row_ptr;
+  int *cols;
+  void *elms;
+  int elm_type;
+} ispm_csr_out;
+
 /* slell_matrix<E, host_memory_space_tag> (formats/formats.h:44-102) owned by the library;
  * release with ispm_slell_host_free. */
 typedef struct ispm_slell_host {
@@ -215,6 +225,10 @@ int ispm_plan_execute_multi(ispm_plan *p, int k, const void *X, size_t ldx, void
 /* change_host_matrix + update_device_matrix_coefs (plan/plan.h:39-56): new values, same
  * sparsity pattern; conversion reuses the plan's format parameters. */
 int ispm_plan_update_coefs(ispm_plan *p, const ispm_csr *A);
+/* On-device refresh: vals (device pointer, nnz values of val_type F32 or F64, in the CSR
+ * order the plan was built from) are scattered into the element slots through the
+ * conversion's entry -> slot map; no host re-conversion and no full re-upload. */
+int ispm_plan_update_coefs_device(ispm_plan *p, const void *vals, int val_type);
 void ispm_plan_destroy(ispm_plan *p);
 
 /* ------------------------------------------------------------- PCG solver -------------- */
@@ -258,9 +272,31 @@ typedef struct ispm_solver ispm_solver;
  * preconditioner.  Plans must outlive the solver. */
 int ispm_solver_create(ispm_solver **out, ispm_plan *A, ispm_plan *Ainv1, ispm_plan *Ainv2,
                        const ispm_solver_opts *opts);
+/* Mixed precision evaluates the FP64 true residual with A's plan read into FP64
+ * accumulators, which is exact only when A's values are FP32-representable.  Otherwise give
+ * an FP64-element plan of the same matrix here (used only for the residual replacements). */
+int ispm_solver_set_residual_plan(ispm_solver *s, ispm_plan *A64);
 /* b, x: device FP64 vectors of the padded length; x is the initial guess and the result. */
 int ispm_solver_solve(ispm_solver *s, const double *b, double *x, ispm_solve_stats *stats);
 void ispm_solver_destroy(ispm_solver *s);
+
+/* ------------------------------------------------------------- on-disk formats -------- */
+/* No reference counterpart (the reference has no persistence).  MatrixMarket coordinate
+ * files (real / integer / pattern; general or symmetric, expanded) to CSR (duplicates add),
+ * and back; output is library-owned (ispm_csr_out_free). */
+int ispm_mm_read(const char *path, ispm_csr_out *out);
+int ispm_mm_write(const char *path, const ispm_csr *A);
+/* 64-bit FNV-1a fingerprint of (n_rows, n_cols, row_ptr, cols). */
+int ispm_csr_fingerprint(const ispm_csr *A, unsigned long long *fp);
+/* Plan cache: text file of tuned plans keyed by (fingerprint, vector type, element type).
+ * ispm_plan_create(PLAN_EXHAUSTIVE) consults and extends the file named by the environment
+ * variable ISPM_PLAN_CACHE.  lookup returns ISPM_ERR_INVALID when the key is absent. */
+int ispm_plan_cache_lookup(const char *path, unsigned long long fp, int vec_type, int elm_type,
+                           unsigned *S, unsigned *hblock, int *V, int *D, ispm_launch *launch,
+                           double *seconds);
+int ispm_plan_cache_store(const char *path, unsigned long long fp, int vec_type, int elm_type,
+                          unsigned S, unsigned hblock, int V, int D, const ispm_launch *launch,
+                          double seconds);
 
 /* ------------------------------------------------- approximate inverse (CPU) ---------- */
 
@@ -268,13 +304,6 @@ void ispm_solver_destroy(ispm_solver *s);
  * AINV-type factorisation with M = Ainv2 * Ainv1 ~= A^-1.  in type ISPM_F64, out type
  * ISPM_F32 or ISPM_F64.  progress may be NULL.  Outputs are library-owned CSR arrays;
  * release with ispm_csr_out_free.  Returns ISPM_ERR_INVALID for n <= 0 (reference: false). */
-typedef struct ispm_csr_out {
-  int n_rows, n_cols, n_nz;
-  int *row_ptr;
-  int *cols;
-  void *elms;
-  int elm_type;
-} ispm_csr_out;
 
 typedef void (*ispm_progress_fn)(void *);
 

@@ -386,8 +386,16 @@ int ispm_plan_create(ispm_plan **out, int vt, int et, const ispm_csr *A, int kin
     unsigned S = 0, W = 1;
     int V = 0, D = 0;
     ispm_launch lp;
-    st = tune_exhaustive(A, vt, et, S, W, V, D, lp, p->tuned);
-    if (st) { delete p; return st; }
+    const char *cache = std::getenv("ISPM_PLAN_CACHE");
+    unsigned long long fp = 0;
+    bool hit = false;
+    if (cache && cache[0] && !ispm_csr_fingerprint(A, &fp))
+      hit = ispm_plan_cache_lookup(cache, fp, vt, et, &S, &W, &V, &D, &lp, &p->tuned) == ISPM_OK;
+    if (!hit) {
+      st = tune_exhaustive(A, vt, et, S, W, V, D, lp, p->tuned);
+      if (st) { delete p; return st; }
+      if (cache && cache[0]) ispm_plan_cache_store(cache, fp, vt, et, S, W, V, D, &lp, p->tuned);
+    }
     st = make_layout(A, et, S, W, V, D, p->L);
     if (st) { delete p; return st; }
     p->launch = lp;
@@ -505,6 +513,7 @@ void ispm_plan_destroy(ispm_plan *p)
 {
   if (!p) return;
   if (p->has_dev) ispm_slell_device_free(&p->dev);
+  if (p->dev_map) ispm_free(p->dev_map);
   delete p;
 }
 

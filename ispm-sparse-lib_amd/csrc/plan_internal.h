@@ -15,6 +15,7 @@ struct ispm_plan {
   ispm_launch launch{};
   double tuned = 0;
   size_t stored = 0, csr = 0;
+  void *dev_map = nullptr;                /* device copy of the slot map (on-device refresh) */
 };
 
 namespace ispm {

@@ -248,6 +248,7 @@ using namespace ispm;
 
 struct ispm_solver {
   ispm_plan *A = nullptr, *M1 = nullptr, *M2 = nullptr;
+  ispm_plan *Ares = nullptr;  /* plan used for the FP64 true residual (default: A) */
   ispm_solver_opts o{};
   int n = 0;
   int T = ISPM_F64;
@@ -394,7 +395,7 @@ int zero(void *p, size_t bytes) { return ispm_zero(p, bytes); }
 /* r64 = b - A x (FP64 true residual), s->r64 = ||r64||^2 */
 int true_residual(ispm_solver *S, const double *b, const double *x)
 {
-  int st = spmv(S->A, ISPM_F64, x, S->tmp64, nullptr);
+  int st = spmv(S->Ares ? S->Ares : S->A, ISPM_F64, x, S->tmp64, nullptr);
   if (st) return st;
   hipLaunchKernelGGL(k_resid, dim3(S->grid), dim3(256), 0, stream(), b, (const double *)S->tmp64, S->r64,
                      S->n, S->s, S->parts, S->ctr + 2);
@@ -587,6 +588,15 @@ int ispm_solver_solve(ispm_solver *S, const double *b, double *x, ispm_solve_sta
   if (stats) *stats = res;
   if (st) return st;
   return res.converged ? ISPM_OK : fail(ISPM_ERR_NOT_CONVERGED, "solver: relres %.3e after %d iterations", res.relres, res.iterations);
+}
+
+int ispm_solver_set_residual_plan(ispm_solver *S, ispm_plan *A64)
+{
+  if (!S) return fail(ISPM_ERR_INVALID, "solver_set_residual_plan: null solver");
+  if (A64 && (!A64->has_dev || A64->E != ISPM_F64 || A64->L.n_rows != S->n))
+    return fail(ISPM_ERR_INVALID, "solver_set_residual_plan: need an FP64 device plan of the same size");
+  S->Ares = A64;
+  return ISPM_OK;
 }
 
 void ispm_solver_destroy(ispm_solver *S)

@@ -128,13 +128,23 @@ _SIGS = {
     "ispm_plan_execute_multi": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t,
                                           C.c_void_p, C.c_size_t]),
     "ispm_plan_update_coefs": (C.c_int, [C.c_void_p, C.POINTER(Csr)]),
+    "ispm_plan_update_coefs_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "ispm_plan_destroy": (None, [C.c_void_p]),
     "ispm_fastainv_sym": (C.c_int, [C.c_int, C.POINTER(Csr), PROGRESS_FN, C.c_void_p,
                                     C.c_longlong, C.c_double, C.POINTER(CsrOut), C.POINTER(CsrOut)]),
     "ispm_csr_out_free": (None, [C.POINTER(CsrOut)]),
+    "ispm_mm_read": (C.c_int, [C.c_char_p, C.POINTER(CsrOut)]),
+    "ispm_mm_write": (C.c_int, [C.c_char_p, C.POINTER(Csr)]),
+    "ispm_csr_fingerprint": (C.c_int, [C.POINTER(Csr), C.POINTER(C.c_ulonglong)]),
+    "ispm_plan_cache_lookup": (C.c_int, [C.c_char_p, C.c_ulonglong, C.c_int, C.c_int, C.POINTER(C.c_uint),
+                                         C.POINTER(C.c_uint), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                         C.POINTER(Launch), C.POINTER(C.c_double)]),
+    "ispm_plan_cache_store": (C.c_int, [C.c_char_p, C.c_ulonglong, C.c_int, C.c_int, C.c_uint, C.c_uint,
+                                        C.c_int, C.c_int, C.POINTER(Launch), C.c_double]),
     "ispm_solver_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "ispm_solver_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "ispm_solver_destroy": (None, [C.c_void_p]),
+    "ispm_solver_set_residual_plan": (C.c_int, [C.c_void_p, C.c_void_p]),
 }
 
 # Extra entry points appended by later headers (solver); registered when present.
@@ -406,6 +416,10 @@ class Plan:
     def update_coefs(self, A: CSR):
         check(lib().ispm_plan_update_coefs(self.p, C.byref(A.c())))
 
+    def update_coefs_device(self, vals: DeviceArray):
+        t = F64 if vals.dtype == np.float64 else F32
+        check(lib().ispm_plan_update_coefs_device(self.p, vals.vp, t))
+
     def __del__(self):
         try:
             lib().ispm_plan_destroy(self.p)
@@ -443,7 +457,7 @@ def _csr_from_out(o: CsrOut):
     ci = np.ctypeslib.as_array(o.cols, (max(1, o.n_nz),))[:o.n_nz].copy()
     ct = C.c_float if o.elm_type == F32 else C.c_double
     v = np.ctypeslib.as_array(C.cast(o.elms, C.POINTER(ct)), (max(1, o.n_nz),))[:o.n_nz].copy()
-    return CSR(n, n, rp, ci, v)
+    return CSR(n, o.n_cols, rp, ci, v)
 
 
 def fastainv_sym(A: CSR, out_type=F32, max_elts=8192000, droptol=0.125, progress=None):
@@ -459,3 +473,23 @@ def fastainv_sym(A: CSR, out_type=F32, max_elts=8192000, droptol=0.125, progress
     finally:
         lib().ispm_csr_out_free(C.byref(a1))
         lib().ispm_csr_out_free(C.byref(a2))
+
+
+# ------------------------------------------------------------------ on-disk ----------
+def mm_read(path) -> CSR:
+    o = CsrOut()
+    check(lib().ispm_mm_read(str(path).encode(), C.byref(o)))
+    try:
+        return _csr_from_out(o)
+    finally:
+        lib().ispm_csr_out_free(C.byref(o))
+
+
+def mm_write(path, A: CSR):
+    check(lib().ispm_mm_write(str(path).encode(), C.byref(A.c())))
+
+
+def fingerprint(A: CSR) -> int:
+    h = C.c_ulonglong()
+    check(lib().ispm_csr_fingerprint(C.byref(A.c()), C.byref(h)))
+    return h.value

@@ -23,6 +23,11 @@ class SolveStats(C.Structure):
                 ("relres", C.c_double), ("l1_resnorm", C.c_double)]
 
 
+def fp32_exact(A: CSR) -> bool:
+    v = np.asarray(A.vals)
+    return v.dtype == np.float32 or bool(np.all(v.astype(np.float32).astype(np.float64) == v))
+
+
 def _bind():
     L = lib()
     L.ispm_solver_create.restype = C.c_int
@@ -32,6 +37,8 @@ def _bind():
     L.ispm_solver_solve.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(SolveStats)]
     L.ispm_solver_destroy.restype = None
     L.ispm_solver_destroy.argtypes = [C.c_void_p]
+    L.ispm_solver_set_residual_plan.restype = C.c_int
+    L.ispm_solver_set_residual_plan.argtypes = [C.c_void_p, C.c_void_p]
     return L
 
 
@@ -65,6 +72,11 @@ class Solver:
         m2 = self.M_plans[1].p if self.M_plans else None
         check(L.ispm_solver_create(C.byref(p), self.A_plan.p, m1, m2, C.byref(o)))
         self.p = p
+        self.res_plan = None
+        if mixed and A_plan is None and not fp32_exact(A):
+            # FP32 storage changes A: the FP64 true residual needs the FP64 values
+            self.res_plan = Plan(A, F64, elm_type=F64)
+            check(L.ispm_solver_set_residual_plan(self.p, self.res_plan.p))
 
     def solve_device(self, b: DeviceArray, x: DeviceArray, allow_fail=False):
         st = SolveStats()
