@@ -99,10 +99,19 @@ typedef struct ispm_slell_desc {
 /* spmv_launch_params (cuda/spmv/kernel-iface.h:6-18).  N = threads per workgroup
  * (64..1024, power of two), S = slice height the launch is specialised for, H = rows per
  * thread (1 or 2).  Valid iff N >= S/H and S >= H.  use_tex_* are accepted and ignored:
- * x windows are read through the cache hierarchy, never through texture units. */
+ * x windows are read through the cache hierarchy, never through texture units.
+ * mode (new): 0 = one workgroup per slice, threads split a slice's planes when N*H > S (the
+ * reference's mapping); 1 = packed, for fixed slice heights below 64: each 64-lane wave
+ * takes 64/S consecutive slices with one row per lane (N then only sets the workgroup
+ * size); 2 = streamed, for fixed slice heights up to 256 without hblock pairs or diagonal
+ * planes: a 256-thread workgroup reads the contiguous elements of 256 rows with consecutive
+ * lanes on consecutive elements, stages the products in LDS and sums each row in plane
+ * order (other layouts given mode 2 run packed when S < 64, else mode 0).  Any other mode
+ * is rejected. */
 typedef struct ispm_launch {
   unsigned N, S, H;
   int use_tex_sliceptr, use_tex_x;
+  int mode;
 } ispm_launch;
 
 /* Replaces launch_spmv_kernel<T,E> (cuda/spmv/kernel-iface.h:20-74, dispatch.h:184-205):
@@ -213,8 +222,15 @@ int ispm_plan_create_explicit(ispm_plan **out, int vec_type, int elm_type, const
                               const ispm_launch *launch);
 int ispm_plan_info_get(const ispm_plan *p, ispm_plan_info *info);
 /* Launch shape the default plan uses for a format (CDNA4 heuristic: one row per thread,
- * workgroups of max(64, min(1024, S)) threads; 256 threads for variable-height slices). */
+ * workgroups of max(64, min(1024, S)) threads; one 64-lane wave per slice for
+ * variable-height slices). */
 int ispm_plan_default_launch(unsigned slice_height, int var_height, ispm_launch *out);
+/* PLAN_ESTIMATE (in the reference an alias of PLAN_DEFAULT, plan/plan.h:74-96): the format
+ * with the fewest stored bytes (exact for plain ELL at every slice height, fixed and
+ * variable; the diagonal/hblock variant suggested by the sampled statistics converted once
+ * and compared), with the default launch shape. */
+int ispm_plan_estimate(int vec_type, int elm_type, const ispm_csr *A, unsigned *S,
+                       unsigned *hblock, int *V, int *D, ispm_launch *launch);
 /* The PLAN_EXHAUSTIVE search (plan/plan.h:97-144): times every (S, V, D, hblock) format and
  * launch shape on the device and returns the fastest. */
 int ispm_plan_tune(int vec_type, int elm_type, const ispm_csr *A, unsigned *S, unsigned *hblock,
@@ -297,6 +313,34 @@ int ispm_plan_cache_lookup(const char *path, unsigned long long fp, int vec_type
 int ispm_plan_cache_store(const char *path, unsigned long long fp, int vec_type, int elm_type,
                           unsigned S, unsigned hblock, int V, int D, const ispm_launch *launch,
                           double seconds);
+
+/* ------------------------------------------------- multi-RHS and batched solves ------- */
+/* k = 1..8 right-hand sides sharing one matrix: independent FP64 PCG recurrences per column
+ * whose matrix products are SpMMs (one pass over A, Ainv1, Ainv2 per iteration for all
+ * columns).  B, X: device, column j at B + j*ldb / X + j*ldx (padded length); X holds the
+ * initial guesses.  stats: array of k.  No reference counterpart. */
+typedef struct ispm_block_solver ispm_block_solver;
+int ispm_block_solver_create(ispm_block_solver **out, ispm_plan *A, ispm_plan *Ainv1,
+                             ispm_plan *Ainv2, int k, const ispm_solver_opts *opts);
+int ispm_block_solver_solve(ispm_block_solver *s, const double *B, size_t ldb, double *X,
+                            size_t ldx, ispm_solve_stats *stats);
+void ispm_block_solver_destroy(ispm_block_solver *s);
+
+/* Many independent small SPD systems (FP32) solved with Jacobi-preconditioned CG in ONE
+ * persistent launch: workgroups pull whole systems from a device work queue (largest
+ * first).  The systems are given as one block-diagonal CSR (global row/column numbering,
+ * system i owns sys_rows[i] consecutive rows and may not couple to another system).  b and x
+ * use the padded per-system layout reported by ispm_batch_layout (each system padded to a
+ * multiple of 64 rows; padding entries of b must be 0).  iters / relres (host arrays of
+ * n_sys, may be NULL) receive each system's iteration count and FP32 recursive relative
+ * residual.  No reference counterpart. */
+typedef struct ispm_batch ispm_batch;
+int ispm_batch_create(ispm_batch **out, int n_sys, const int *sys_rows, const int *row_ptr,
+                      const int *cols, const float *vals);
+int ispm_batch_layout(const ispm_batch *b, long long *rows_padded, long long *row_offsets);
+int ispm_batch_solve(ispm_batch *b, const float *rhs, float *x, double tol, int maxit,
+                     int *iters, double *relres);
+void ispm_batch_destroy(ispm_batch *b);
 
 /* ------------------------------------------------- approximate inverse (CPU) ---------- */
 

@@ -28,17 +28,17 @@ struct colsc {
 
 constexpr int kMaxK = 8;
 
-enum { OP_RZ, OP_PQ, OP_R64 };
+enum { OP_RZ, OP_PQ, OP_R64, OP_XR };
 
 /* per-column dot (X_j, Y_j) with the scalar epilogue of PCG */
 template<int OP>
 __global__ void __launch_bounds__(256)
-k_mdot(const double *__restrict__ X, const double *__restrict__ Y, size_t ld, int n, colsc *cs,
-       double *parts, unsigned *ctr)
+k_mdot(const double *__restrict__ X, const double *__restrict__ Y, size_t ld, int n, const colsc *cs,
+       double *parts)
 {
   __shared__ double lds[4];
   const int j = blockIdx.y;
-  colsc *c = cs + j;
+  const colsc *c = cs + j;
   if (OP != OP_R64 && !c->active) return;
   const double *x = X + j * ld, *y = Y + j * ld;
   double v[1] = {0.0};
@@ -48,12 +48,29 @@ k_mdot(const double *__restrict__ X, const double *__restrict__ Y, size_t ld, in
     v[0] += a.v[0] * b.v[0] + a.v[1] * b.v[1];
   }
   for (int i = nv * 2 + gt; i < n; i += gs) v[0] += x[i] * y[i];
+  store_partials<1>(v, parts + (size_t)j * kMaxBlocks, lds);
+}
+
+/* Second phase of the per-column reductions (block j sums column j's partials) with the
+ * scalar epilogue of PCG; columns already converged are left as they are. */
+template<int OP>
+__global__ void __launch_bounds__(kFinThreads) k_mfin(const double *parts, int G, colsc *cs)
+{
+  __shared__ double lds[kFinThreads / 64];
+  const int j = blockIdx.x;
+  colsc *c = cs + j;
+  if (OP != OP_R64 && !c->active) return;
   double out[1];
-  if (finish_reduction<1>(v, parts + (size_t)j * kMaxBlocks, ctr + j, lds, out) && threadIdx.x == 0) {
-    const double r = out[0];
-    if (OP == OP_RZ) { c->beta = r / c->rho; c->rho = r; }
-    if (OP == OP_PQ) { c->pq = r; c->alpha = c->rho / r; }
-    if (OP == OP_R64) c->r64 = r;
+  sum_partials<1>(parts + (size_t)j * kMaxBlocks, G, lds, out);
+  if (threadIdx.x) return;
+  const double r = out[0];
+  if (OP == OP_RZ) { c->beta = r / c->rho; c->rho = r; }
+  if (OP == OP_PQ) { c->pq = r; c->alpha = c->rho / r; }
+  if (OP == OP_R64) c->r64 = r;
+  if (OP == OP_XR) {
+    c->rr = r;
+    c->iters++;
+    if (r <= c->tol2) c->active = 0;
   }
 }
 
@@ -72,11 +89,11 @@ __global__ void __launch_bounds__(256) k_mp(double *__restrict__ P, const double
 /* X_j += alpha_j P_j; R_j -= alpha_j Q_j; (R_j, R_j); column done below tol2 */
 __global__ void __launch_bounds__(256)
 k_mxr(double *__restrict__ X, size_t ldx, double *__restrict__ R, const double *__restrict__ P,
-      const double *__restrict__ Q, size_t ld, int n, colsc *cs, double *parts, unsigned *ctr)
+      const double *__restrict__ Q, size_t ld, int n, const colsc *cs, double *parts)
 {
   __shared__ double lds[4];
   const int j = blockIdx.y;
-  colsc *c = cs + j;
+  const colsc *c = cs + j;
   if (!c->active) return;
   const double a = c->alpha;
   double *x = X + j * ldx, *r = R + j * ld;
@@ -89,12 +106,7 @@ k_mxr(double *__restrict__ X, size_t ldx, double *__restrict__ R, const double *
     r[i] = ri;
     v[0] += ri * ri;
   }
-  double out[1];
-  if (finish_reduction<1>(v, parts + (size_t)j * kMaxBlocks, ctr + j, lds, out) && threadIdx.x == 0) {
-    c->rr = out[0];
-    c->iters++;
-    if (out[0] <= c->tol2) c->active = 0;
-  }
+  store_partials<1>(v, parts + (size_t)j * kMaxBlocks, lds);
 }
 
 /* R_j = B_j - AX_j */
@@ -129,7 +141,6 @@ struct ispm_block_solver {
   double *R = nullptr, *Z = nullptr, *P = nullptr, *Q = nullptr, *T = nullptr;
   colsc *cs = nullptr, *hcs = nullptr;
   double *parts = nullptr;
-  unsigned *ctr = nullptr;
 };
 
 namespace {
@@ -137,6 +148,11 @@ namespace {
 int mspmm(ispm_plan *P, int k, const double *X, size_t ldx, double *Y, size_t ldy)
 {
   return spmm_internal(&P->launch, &P->dev, ISPM_F64, k, X, ldx, Y, ldy);
+}
+
+template<int OP> void mfinish(ispm_block_solver *S)
+{
+  hipLaunchKernelGGL(k_mfin<OP>, dim3(S->k), dim3(kFinThreads), 0, stream(), (const double *)S->parts, S->grid, S->cs);
 }
 
 int pull(ispm_block_solver *S)
@@ -165,11 +181,14 @@ int iterate(ispm_block_solver *S, double *X, size_t ldx)
   } else {
     hipLaunchKernelGGL(k_mcopy, g, b, 0, sm, S->Z, (const double *)S->R, ld, n);
   }
-  hipLaunchKernelGGL((k_mdot<OP_RZ>), g, b, 0, sm, (const double *)S->R, (const double *)S->Z, ld, n, S->cs, S->parts, S->ctr);
+  hipLaunchKernelGGL((k_mdot<OP_RZ>), g, b, 0, sm, (const double *)S->R, (const double *)S->Z, ld, n, S->cs, S->parts);
+  mfinish<OP_RZ>(S);
   hipLaunchKernelGGL(k_mp, g, b, 0, sm, S->P, (const double *)S->Z, ld, n, (const colsc *)S->cs);
   if ((st = mspmm(S->A, k, S->P, ld, S->Q, ld))) return st;
-  hipLaunchKernelGGL((k_mdot<OP_PQ>), g, b, 0, sm, (const double *)S->P, (const double *)S->Q, ld, n, S->cs, S->parts, S->ctr);
-  hipLaunchKernelGGL(k_mxr, g, b, 0, sm, X, ldx, S->R, (const double *)S->P, (const double *)S->Q, ld, n, S->cs, S->parts, S->ctr);
+  hipLaunchKernelGGL((k_mdot<OP_PQ>), g, b, 0, sm, (const double *)S->P, (const double *)S->Q, ld, n, S->cs, S->parts);
+  mfinish<OP_PQ>(S);
+  hipLaunchKernelGGL(k_mxr, g, b, 0, sm, X, ldx, S->R, (const double *)S->P, (const double *)S->Q, ld, n, S->cs, S->parts);
+  mfinish<OP_XR>(S);
   ISPM_HIP(hipGetLastError());
   return ISPM_OK;
 }
@@ -181,7 +200,8 @@ int residuals(ispm_block_solver *S, const double *B, size_t ldb, const double *X
   dim3 g(S->grid, S->k), b(256);
   if ((st = mspmm(S->A, S->k, X, ldx, S->Q, S->ld))) return st;
   hipLaunchKernelGGL(k_mres, g, b, 0, stream(), B, ldb, (const double *)S->Q, S->R, S->ld, S->n);
-  hipLaunchKernelGGL((k_mdot<OP_R64>), g, b, 0, stream(), (const double *)S->R, (const double *)S->R, S->ld, S->n, S->cs, S->parts, S->ctr);
+  hipLaunchKernelGGL((k_mdot<OP_R64>), g, b, 0, stream(), (const double *)S->R, (const double *)S->R, S->ld, S->n, S->cs, S->parts);
+  mfinish<OP_R64>(S);
   ISPM_HIP(hipGetLastError());
   return ISPM_OK;
 }
@@ -208,14 +228,13 @@ int ispm_block_solver_create(ispm_block_solver **out, ispm_plan *A, ispm_plan *M
   if (S->o.maxit <= 0) S->o.maxit = 100000;
   S->n = A->L.n_rows;
   S->ld = ((size_t)S->n + 63) / 64 * 64;
-  S->grid = std::max(1, std::min(num_cus() * 4 / k, kMaxBlocks));
+  S->grid = std::max(1, std::min((S->n + 511) / 512, kMaxBlocks));
   const size_t vb = sizeof(double) * S->ld * k;
   double **vs[] = {&S->R, &S->Z, &S->P, &S->Q, &S->T};
   for (double **v : vs)
     if ((st = ispm_malloc((void **)v, vb))) { ispm_block_solver_destroy(S); return st; }
   if ((st = ispm_malloc((void **)&S->cs, sizeof(colsc) * kMaxK)) ||
-      (st = ispm_malloc((void **)&S->parts, sizeof(double) * kMaxBlocks * kMaxK)) ||
-      (st = ispm_malloc((void **)&S->ctr, sizeof(unsigned) * kMaxK))) {
+      (st = ispm_malloc((void **)&S->parts, sizeof(double) * kMaxBlocks * kMaxK))) {
     ispm_block_solver_destroy(S);
     return st;
   }
@@ -240,7 +259,8 @@ int ispm_block_solver_solve(ispm_block_solver *S, const double *B, size_t ldb, d
   if ((st = push(S))) return st;
   {
     dim3 g(S->grid, k), b(256);
-    hipLaunchKernelGGL((k_mdot<OP_R64>), g, b, 0, stream(), B, B, ldb, S->n, S->cs, S->parts, S->ctr);
+    hipLaunchKernelGGL((k_mdot<OP_R64>), g, b, 0, stream(), B, B, ldb, S->n, S->cs, S->parts);
+    mfinish<OP_R64>(S);
     ISPM_HIP(hipGetLastError());
     if ((st = pull(S))) return st;
     for (int j = 0; j < k; j++) bn[j] = S->hcs[j].r64 > 0 ? std::sqrt(S->hcs[j].r64) : 1.0;
@@ -315,7 +335,6 @@ void ispm_block_solver_destroy(ispm_block_solver *S)
   for (double *v : {S->R, S->Z, S->P, S->Q, S->T}) ispm_free(v);
   ispm_free(S->cs);
   ispm_free(S->parts);
-  ispm_free(S->ctr);
   if (S->hcs) (void)hipHostFree(S->hcs);
   delete S;
 }

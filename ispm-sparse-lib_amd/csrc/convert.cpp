@@ -246,6 +246,21 @@ void enc_diag(const csr_in &A, layout &B, int S, int row0, std::vector<int> &dia
 
 }  // namespace
 
+int var_slice_heights(const int *rp_pad, int n_pad, int cap, long long nnz, int min_height,
+                      std::vector<int> &h)
+{
+  csr_in A{n_pad, rp_pad, nullptr, 8};
+  h.clear();
+  for (int i = 0; i < n_pad;) {
+    int sz = var_slice_rows(A, i, cap, nnz);
+    if (sz < min_height) sz = min_height;
+    if (sz <= 0 || i + sz > n_pad) return -1;
+    h.push_back(sz);
+    i += sz;
+  }
+  return 0;
+}
+
 int pad_row_ptr(const int *rp, int n_rows, std::vector<int> &out)
 {
   const int n_pad = (n_rows + kPadRows - 1) & ~(kPadRows - 1);
@@ -308,6 +323,16 @@ int convert(const int *rp_in, int n_rows, const int *ci, int n_cols, int n_nz, i
 int guess(const int *rp_in, int n_rows, const int *ci, int n_cols, int n_nz, int elm_size,
           int examine_ratio, int min_height, layout &B, guess_info *gi)
 {
+  guess_info g;
+  guess_stats(rp_in, n_rows, ci, elm_size, examine_ratio, g);
+  if (gi) *gi = g;
+  return convert(rp_in, n_rows, ci, n_cols, n_nz, elm_size, g.S, g.hblock, g.var_height, g.diags,
+                 min_height, B);
+}
+
+int guess_stats(const int *rp_in, int n_rows, const int *ci, int elm_size, int examine_ratio,
+                guess_info &out)
+{
   enum { ELTS_PER_SLICE = 8 * 256, MIN_HBLOCK_RATIO = 25, MIN_DIAGS_RATIO = 50, TRIAL = 32 };
   std::vector<int> rp;
   const int n = pad_row_ptr(rp_in, n_rows, rp);
@@ -345,19 +370,16 @@ int guess(const int *rp_in, int n_rows, const int *ci, int n_cols, int n_nz, int
   int hb = 1;
   const bool want_diags = (long long)n_diags * TRIAL > (long long)elts * MIN_DIAGS_RATIO / 100;
   if (!want_diags && n_pairs > (long long)elts * MIN_HBLOCK_RATIO / 100) hb = 2;
-  if (gi) {
-    gi->rows_examined = rows;
-    gi->elts_examined = elts;
-    gi->waste = waste;
-    gi->n_pairs = n_pairs;
-    gi->n_diags = n_diags;
-    gi->S = S;
-    gi->hblock = hb;
-    gi->var_height = want_var;
-    gi->diags = want_diags;
-  }
-  return convert(rp_in, n_rows, ci, n_cols, n_nz, elm_size, S, hb, want_var, want_diags,
-                 min_height, B);
+  out.rows_examined = rows;
+  out.elts_examined = elts;
+  out.waste = waste;
+  out.n_pairs = n_pairs;
+  out.n_diags = n_diags;
+  out.S = S;
+  out.hblock = hb;
+  out.var_height = want_var;
+  out.diags = want_diags;
+  return 0;
 }
 
 }  // namespace conv

@@ -30,6 +30,14 @@ int convert(const int *rp, int n_rows, const int *ci, int n_cols, int n_nz, int 
             int req_height, int hblock, bool var_height, bool diags, int min_height,
             layout &out);
 
+/* the guess heuristic's sample statistics and decision, without converting */
+int guess_stats(const int *rp, int n_rows, const int *ci, int elm_size, int examine_ratio,
+                guess_info &out);
+
+/* variable slice heights (estimate_slice_height, raised to min_height) over a padded CSR */
+int var_slice_heights(const int *rp_pad, int n_pad, int cap, long long nnz, int min_height,
+                      std::vector<int> &h);
+
 int guess(const int *rp, int n_rows, const int *ci, int n_cols, int n_nz, int elm_size,
           int examine_ratio, int min_height, layout &out, guess_info *gi);
 

@@ -14,7 +14,7 @@ template<class T, class E>
 void launch_spmv_kernel(spmv_launch_params &params, slell_matrix<E, device_memory_space_tag> &m,
                         T *xptr, T *yptr)
 {
-  ispm_launch p{params.N, params.S, params.H, params.use_tex_sliceptr, params.use_tex_x};
+  ispm_launch p{params.N, params.S, params.H, params.use_tex_sliceptr, params.use_tex_x, params.mode};
   ispm_slell_desc d = m.ispm_desc();
   int st = ispm_spmv(&p, &d, ispm_type_code<T>::value, xptr, yptr);
   /* invalid or unsupported launch parameters leave y unwritten, as in the reference */

@@ -15,33 +15,45 @@ typedef uint16_t bf16_t;
 template<typename T> __device__ __forceinline__ T cvt(float v) { return (T)v; }
 template<typename T> __device__ __forceinline__ T cvt_d(double v) { return (T)v; }
 
+/* Matrix arrays are streamed once per product: they are read with nontemporal loads so
+ * they do not evict the x windows that neighbouring slices reuse from L2. */
+typedef float nt_f2 __attribute__((ext_vector_type(2)));
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+typedef unsigned nt_u2 __attribute__((ext_vector_type(2)));
+
 template<typename T, typename E> struct elt;
 template<typename T> struct elt<T, float> {
-  static __device__ __forceinline__ T get(const float *p, unsigned i) { return (T)p[i]; }
+  static __device__ __forceinline__ T get(const float *p, unsigned i) { return (T)__builtin_nontemporal_load(p + i); }
   static __device__ __forceinline__ void get2(const float *p, unsigned i, T &a, T &b)
   {
-    float2 v = *reinterpret_cast<const float2 *>(p + i);
+    nt_f2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f2 *>(p + i));
     a = (T)v.x; b = (T)v.y;
   }
 };
 template<typename T> struct elt<T, double> {
-  static __device__ __forceinline__ T get(const double *p, unsigned i) { return (T)p[i]; }
+  static __device__ __forceinline__ T get(const double *p, unsigned i) { return (T)__builtin_nontemporal_load(p + i); }
   static __device__ __forceinline__ void get2(const double *p, unsigned i, T &a, T &b)
   {
-    double2 v = *reinterpret_cast<const double2 *>(p + i);
+    nt_d2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_d2 *>(p + i));
     a = (T)v.x; b = (T)v.y;
   }
 };
 __device__ __forceinline__ float bf16_to_f32(unsigned short u) { return __uint_as_float((unsigned)u << 16); }
 template<typename T> struct elt<T, bf16_t> {
-  static __device__ __forceinline__ T get(const bf16_t *p, unsigned i) { return (T)bf16_to_f32(p[i]); }
+  static __device__ __forceinline__ T get(const bf16_t *p, unsigned i) { return (T)bf16_to_f32(__builtin_nontemporal_load(p + i)); }
   static __device__ __forceinline__ void get2(const bf16_t *p, unsigned i, T &a, T &b)
   {
-    unsigned v = *reinterpret_cast<const unsigned *>(p + i);
+    unsigned v = __builtin_nontemporal_load(reinterpret_cast<const unsigned *>(p + i));
     a = (T)__uint_as_float(v << 16);
     b = (T)__uint_as_float(v & 0xffff0000u);
   }
 };
+__device__ __forceinline__ unsigned col_load(const unsigned *p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void col_load2(const unsigned *p, unsigned &a, unsigned &b)
+{
+  nt_u2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_u2 *>(p));
+  a = v.x; b = v.y;
+}
 
 /* ---- wave64 reductions ---- */
 /* ds_swizzle in bit mode: lane l reads lane (l ^ m) within its 32-lane half. */

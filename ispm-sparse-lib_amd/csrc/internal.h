@@ -28,15 +28,13 @@ hipStream_t stream();
 /* Make sure the device is initialised (gfx950 check); returns ISPM_OK or an error. */
 int ensure_init();
 
-/* Per-device reduction scratch: partial sums and a completion counter used by the
- * last-block-finishes reductions.  Allocated once per device at initialisation. */
+/* Per-device reduction scratch: block partial sums of the two-phase reductions of the
+ * sblas entry points.  Allocated once per device at initialisation. */
 struct reduce_scratch {
   double *partials;         /* kMaxBlocks * kMaxSums doubles */
-  unsigned *counter;        /* one completion counter per reduction slot */
 };
-constexpr int kMaxBlocks = 2048;
+constexpr int kMaxBlocks = 8192;
 constexpr int kMaxSums = 8;
-constexpr int kCounterSlots = 16;
 reduce_scratch &scratch();
 
 /* Number of compute units (cached). */

@@ -127,13 +127,13 @@ extern "C" int ispm_plan_cache_lookup(const char *path, unsigned long long fp, i
   int found = 0;
   while (std::fgets(line, sizeof line, f)) {
     unsigned long long h;
-    int a, b, v, d;
+    int a, b, v, d, mode;
     unsigned s, w, n, ls, hh;
     double sec;
-    if (std::sscanf(line, "%llx %d %d %u %u %d %d %u %u %u %lf", &h, &a, &b, &s, &w, &v, &d, &n, &ls, &hh, &sec) != 11) continue;
+    if (std::sscanf(line, "%llx %d %d %u %u %d %d %u %u %u %d %lf", &h, &a, &b, &s, &w, &v, &d, &n, &ls, &hh, &mode, &sec) != 12) continue;
     if (h == fp && a == vt && b == et) {   /* the last matching line wins */
       *S = s; *W = w; *V = v; *D = d;
-      lp->N = n; lp->S = ls; lp->H = hh; lp->use_tex_sliceptr = lp->use_tex_x = 0;
+      lp->N = n; lp->S = ls; lp->H = hh; lp->use_tex_sliceptr = lp->use_tex_x = 0; lp->mode = mode;
       if (seconds) *seconds = sec;
       found = 1;
     }
@@ -149,6 +149,6 @@ extern "C" int ispm_plan_cache_store(const char *path, unsigned long long fp, in
   if (!path || !lp) return set_error_message(ISPM_ERR_INVALID, "plan_cache_store: null argument");
   FILE *f = std::fopen(path, "a");
   if (!f) return set_error_message(ISPM_ERR_INVALID, "plan_cache_store: cannot open cache file");
-  std::fprintf(f, "%016llx %d %d %u %u %d %d %u %u %u %.9g\n", fp, vt, et, S, W, V, D, lp->N, lp->S, lp->H, seconds);
+  std::fprintf(f, "%016llx %d %d %u %u %d %d %u %u %u %d %.9g\n", fp, vt, et, S, W, V, D, lp->N, lp->S, lp->H, lp->mode, seconds);
   return std::fclose(f) == 0 ? ISPM_OK : set_error_message(ISPM_ERR_RUNTIME, "plan_cache_store: write failed");
 }

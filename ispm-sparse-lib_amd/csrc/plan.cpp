@@ -159,15 +159,27 @@ size_t csr_bytes(const ispm_csr *A, int et, int vt)
   return 4 * ((size_t)A->n_rows + 1) + (4 + tsize(et)) * nnz + tsize(vt) * 2 * (size_t)A->n_rows;
 }
 
-/* CDNA4 default launch: one row per thread, wave-multiple workgroups covering a slice. */
+/* CDNA4 default launch: one row per thread, wave-multiple workgroups covering a slice
+ * (fixed height); one wave per slice for variable-height slices. */
 ispm_launch default_launch(const conv::layout &L)
 {
   ispm_launch p;
   p.H = 1;
   p.use_tex_sliceptr = p.use_tex_x = 0;
-  if (L.V) {
+  p.mode = 0;
+  if (!L.V && L.S < 64) {
+    /* small fixed slices, 4 waves per workgroup: streamed elements for plain planes,
+     * packed rows otherwise */
     p.N = 256;
-    p.S = 256;
+    p.S = L.S;
+    p.mode = (L.W == 1 && !L.D) ? 2 : 1;
+    return p;
+  }
+  if (L.V) {
+    /* variable-height slices are mostly small: one wave per slice, and each workgroup walks
+     * several consecutive slices (see the launch in spmv_kernel.h) */
+    p.N = 64;
+    p.S = 64;
   } else {
     unsigned S = L.S;
     p.N = std::min(1024u, std::max(64u, S));
@@ -232,6 +244,49 @@ static int time_candidate(int vt, const ispm_slell_desc &d, const ispm_launch &l
   return ISPM_OK;
 }
 
+/* PLAN_ESTIMATE: smallest stored bytes.  Plain-ELL bytes are exact and cost O(n) per
+ * candidate (row lengths only); the diagonal / hblock variant the sampled statistics
+ * suggest is converted once and compared exactly. */
+int estimate_format(const ispm_csr *A, int vt, int et, unsigned &bS, unsigned &bW, int &bV, int &bD)
+{
+  std::vector<int> rp;
+  const int n = conv::pad_row_ptr(A->row_ptr, A->n_rows, rp);
+  const long long nnz = rp[n];
+  const double es = (double)tsize(et);
+  double best = INFINITY;
+  for (unsigned S = 4; S <= 512; S <<= 1)
+    for (int V = 0; V <= 1; V++) {
+      std::vector<int> h;
+      if (V) {
+        if (conv::var_slice_heights(rp.data(), n, (int)S, nnz, 4, h)) continue;
+      } else {
+        h.assign(n / S, (int)S);
+      }
+      double slots = 0;
+      int row = 0;
+      for (int sz : h) {
+        int w = 0;
+        for (int r = row; r < row + sz; r++) w = std::max(w, rp[r + 1] - rp[r]);
+        slots += (double)sz * w;
+        row += sz;
+      }
+      const double bytes = slots * (4 + es) + 4.0 * (1 + V) * (h.size() + 1);
+      if (bytes < best * 0.999) { best = bytes; bS = S; bV = V; bW = 1; bD = 0; }
+    }
+  conv::guess_info g;
+  conv::guess_stats(A->row_ptr, A->n_rows, A->cols, (int)tsize(et), 1, g);
+  if (g.diags || g.hblock > 1) {
+    conv::layout L;
+    if (!conv::convert(A->row_ptr, A->n_rows, A->cols, A->n_cols, A->n_nz, (int)tsize(et), (int)bS,
+                       g.hblock, bV != 0, g.diags, 4, L)) {
+      const double bytes = 4.0 * L.sp.size() + 4.0 * L.cols.size() + es * L.n_elms;
+      if (bytes < best) { best = bytes; bW = g.hblock; bD = g.diags; }
+    }
+  }
+  (void)vt;
+  return std::isfinite(best) ? ISPM_OK : fail_plain(ISPM_ERR_INVALID, "estimate: no format");
+}
+
 int tune_exhaustive(const ispm_csr *A, int vt, int et, unsigned &bS, unsigned &bW, int &bV,
                     int &bD, ispm_launch &bl, double &best)
 {
@@ -243,8 +298,9 @@ int tune_exhaustive(const ispm_csr *A, int vt, int et, unsigned &bS, unsigned &b
   if ((st = ispm_malloc(&x, tsize(vt) * ncols)) || (st = ispm_malloc(&y, tsize(vt) * nvec)) ||
       (st = ispm_malloc(&x2, tsize(vt) * ncols)) || (st = ispm_malloc(&y2, tsize(vt) * nvec)))
     return st;
-  for (unsigned S = 32; S <= 512; S <<= 1)
-    for (int V = 0; V <= 1; V++)
+  /* the reference's formats (S = 32..512) plus small fixed heights run in packed mode */
+  for (unsigned S = 4; S <= 512; S <<= 1)
+    for (int V = 0; V <= (S >= 32 ? 1 : 0); V++)
       for (int D = 0; D <= 1; D++)
         for (unsigned W = 1; W <= (D ? 1u : 2u); W <<= 1) {
           conv::layout L;
@@ -253,9 +309,20 @@ int tune_exhaustive(const ispm_csr *A, int vt, int et, unsigned &bS, unsigned &b
           fill_values(L, A, et, el.data());
           ispm_slell_desc d;
           if ((st = upload_layout(L, el.data(), et, &d))) return st;
+          /* packed (S < 64) and streamed (plain ELL planes) mappings */
+          for (int mode = 1; mode <= 2; mode++) {
+            if (V || (mode == 1 && S >= 64) || (mode == 2 && (W != 1 || D || S > 256))) continue;
+            ispm_launch lp{256, S, 1, 0, 0, mode};
+            double sec;
+            if (!time_candidate(vt, d, lp, x, y, x2, y2, sec) && sec < best) {
+              best = sec;
+              bS = S; bW = W; bV = V; bD = D;
+              bl = lp;
+            }
+          }
           for (unsigned N = 64; N <= 1024; N <<= 1)
             for (unsigned H = 1; H <= 2; H <<= 1) {
-              ispm_launch lp{N, V ? N : S, H, 0, 0};
+              ispm_launch lp{N, V ? N : S, H, 0, 0, 0};
               if (!(lp.N >= lp.S / lp.H && lp.S >= lp.H)) continue;
               double sec;
               if (time_candidate(vt, d, lp, x, y, x2, y2, sec)) continue;
@@ -399,6 +466,13 @@ int ispm_plan_create(ispm_plan **out, int vt, int et, const ispm_csr *A, int kin
     st = make_layout(A, et, S, W, V, D, p->L);
     if (st) { delete p; return st; }
     p->launch = lp;
+  } else if (kind == ISPM_PLAN_ESTIMATE) {
+    unsigned S = 64, W = 1;
+    int V = 0, D = 0;
+    st = estimate_format(A, vt, et, S, W, V, D);
+    if (!st) st = make_layout(A, et, S, W, V, D, p->L);
+    if (st) { delete p; return st; }
+    p->launch = default_launch(p->L);
   } else {
     conv::guess_info gi;
     st = conv_status(conv::guess(A->row_ptr, A->n_rows, A->cols, A->n_cols, A->n_nz, (int)tsize(et),
@@ -447,6 +521,24 @@ int ispm_plan_default_launch(unsigned slice_height, int var_height, ispm_launch 
   L.S = slice_height;
   L.V = var_height != 0;
   *out = default_launch(L);
+  return ISPM_OK;
+}
+
+int ispm_plan_estimate(int vt, int et, const ispm_csr *A, unsigned *S, unsigned *hblock, int *V,
+                       int *D, ispm_launch *launch)
+{
+  int st = check_csr(A);
+  if (st) return st;
+  if (!S || !hblock || !V || !D || !launch) return fail_plain(ISPM_ERR_INVALID, "plan_estimate: null output");
+  unsigned s = 64, w = 1;
+  int v = 0, d = 0;
+  st = estimate_format(A, vt, et, s, w, v, d);
+  if (st) return st;
+  *S = s; *hblock = w; *V = v; *D = d;
+  conv::layout L;
+  L.S = s;
+  L.V = v != 0;
+  *launch = default_launch(L);
   return ISPM_OK;
 }
 

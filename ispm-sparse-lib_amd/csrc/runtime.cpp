@@ -36,7 +36,7 @@ struct device_state {
   int init_status = ISPM_OK;
   hipStream_t own = nullptr;
   hipStream_t user = nullptr;
-  reduce_scratch red{nullptr, nullptr};
+  reduce_scratch red{nullptr};
   int cus = 0;
 };
 std::mutex g_mu;
@@ -69,10 +69,6 @@ int init_locked(device_state &s)
   if (e != hipSuccess) return s.init_status = hip_status(e, "hipStreamCreate");
   e = hipMalloc((void **)&s.red.partials, sizeof(double) * kMaxBlocks * kMaxSums);
   if (e != hipSuccess) return s.init_status = hip_status(e, "hipMalloc(scratch)");
-  e = hipMalloc((void **)&s.red.counter, sizeof(unsigned) * kCounterSlots);
-  if (e != hipSuccess) return s.init_status = hip_status(e, "hipMalloc(counter)");
-  e = hipMemset(s.red.counter, 0, sizeof(unsigned) * kCounterSlots);
-  if (e != hipSuccess) return s.init_status = hip_status(e, "hipMemset(counter)");
   e = hipDeviceSynchronize();
   if (e != hipSuccess) return s.init_status = hip_status(e, "hipDeviceSynchronize");
   return s.init_status = ISPM_OK;

@@ -1,9 +1,9 @@
 /* sblas.hip -- pipelined-PCG vector kernels (replaces util/cuda/sblas.cu:18-237).
  *
  * Differences in mechanism from the reference, not in results:
- *  - reductions are single-pass: each block reduces with wave64 ds_swizzle butterflies
- *    (device.h) and the last block to finish (acq_rel counter) sums the block partials in
- *    block order, so there is no second launch and the result is deterministic;
+ *  - reductions are two-phase without atomics: each block reduces with wave64 butterflies
+ *    (device.h) and stores its partial; a one-block finishing kernel sums the partials in
+ *    block order (sblas_kernels.h), so the result is deterministic run to run;
  *  - dot products and the three recurrence sums accumulate in FP64 for both FP32 and
  *    FP64 vectors;
  *  - the fused update reads 10 and writes 8 vectors in one pass with 16-byte accesses.
@@ -52,10 +52,9 @@ __global__ void ppcg_scalars_kernel(T *alpha, T *beta, const T *gamma, const T *
 }
 
 template<typename T, bool VEC>
-__global__ void __launch_bounds__(256) dot_kernel(const T *x, const T *y, T *s, int n,
-                                                  double *parts, unsigned *counter)
+__global__ void __launch_bounds__(256) dot_kernel(const T *x, const T *y, int n, double *parts)
 {
-  __shared__ double lds[kMaxSums * 4];
+  __shared__ double lds[4];
   double v[1] = {0.0};
   const int gt = blockIdx.x * 256 + threadIdx.x, gs = gridDim.x * 256;
   if (VEC) {
@@ -71,33 +70,44 @@ __global__ void __launch_bounds__(256) dot_kernel(const T *x, const T *y, T *s, 
   } else {
     for (int i = gt; i < n; i += gs) v[0] += (double)x[i] * (double)y[i];
   }
+  store_partials<1>(v, parts, lds);
+}
+
+template<typename T>
+__global__ void __launch_bounds__(kFinThreads) dot_finish(const double *parts, int G, T *s)
+{
+  __shared__ double lds[kFinThreads / 64];
   double out[1];
-  if (finish_reduction<1>(v, parts, counter, lds, out)) {
-    if (threadIdx.x == 0) *s = (T)out[0];
+  sum_partials<1>(parts, G, lds, out);
+  if (threadIdx.x == 0) *s = (T)out[0];
+}
+
+template<typename T>
+__global__ void __launch_bounds__(kFinThreads)
+ppcg_update_finish(const double *parts, int G, T *resnorm, T *gamma, T *delta)
+{
+  __shared__ double lds[4 * kFinThreads / 64];
+  double out[4];
+  sum_partials<4>(parts, G, lds, out);
+  if (threadIdx.x == 0) {
+    *resnorm = (T)out[0];
+    *gamma = (T)out[1];
+    *delta = (T)out[2];
   }
+}
+
+template<typename T>
+void launch_ppcg_update(bool vec, int grid, T *resnorm, T *gamma, T *delta, const T *alpha,
+                        const T *beta, const T *n, const T *m, T *p, T *s, T *q, T *z, T *x,
+                        T *r, T *u, T *w, int n_elts)
+{
+  double *parts = scratch().partials;
+  if (vec) hipLaunchKernelGGL((ppcg_update_kernel<T, true>), dim3(grid), dim3(256), 0, stream(), alpha, beta, n, m, p, s, q, z, x, r, u, w, n_elts, parts);
+  else     hipLaunchKernelGGL((ppcg_update_kernel<T, false>), dim3(grid), dim3(256), 0, stream(), alpha, beta, n, m, p, s, q, z, x, r, u, w, n_elts, parts);
+  hipLaunchKernelGGL(ppcg_update_finish<T>, dim3(1), dim3(kFinThreads), 0, stream(), (const double *)parts, grid, resnorm, gamma, delta);
 }
 
 }  // namespace
-
-/* Launch of the fused recurrence update on the library stream (also used by the solver). */
-int launch_ppcg_update(int type, bool vec, int grid, void *resnorm, void *gamma, void *delta,
-                       double *r2, const void *alpha, const void *beta, const void *n,
-                       const void *m, void *p, void *s, void *q, void *z, void *x, void *r,
-                       void *u, void *w, int n_elts, unsigned *counter)
-{
-  double *parts = scratch().partials;
-  if (type == ISPM_F64) {
-    typedef double T;
-    if (vec) hipLaunchKernelGGL((ppcg_update_kernel<T, true>), dim3(grid), dim3(256), 0, stream(), (T *)resnorm, (T *)gamma, (T *)delta, r2, (const T *)alpha, (const T *)beta, (const T *)n, (const T *)m, (T *)p, (T *)s, (T *)q, (T *)z, (T *)x, (T *)r, (T *)u, (T *)w, n_elts, parts, counter);
-    else     hipLaunchKernelGGL((ppcg_update_kernel<T, false>), dim3(grid), dim3(256), 0, stream(), (T *)resnorm, (T *)gamma, (T *)delta, r2, (const T *)alpha, (const T *)beta, (const T *)n, (const T *)m, (T *)p, (T *)s, (T *)q, (T *)z, (T *)x, (T *)r, (T *)u, (T *)w, n_elts, parts, counter);
-  } else {
-    typedef float T;
-    if (vec) hipLaunchKernelGGL((ppcg_update_kernel<T, true>), dim3(grid), dim3(256), 0, stream(), (T *)resnorm, (T *)gamma, (T *)delta, r2, (const T *)alpha, (const T *)beta, (const T *)n, (const T *)m, (T *)p, (T *)s, (T *)q, (T *)z, (T *)x, (T *)r, (T *)u, (T *)w, n_elts, parts, counter);
-    else     hipLaunchKernelGGL((ppcg_update_kernel<T, false>), dim3(grid), dim3(256), 0, stream(), (T *)resnorm, (T *)gamma, (T *)delta, r2, (const T *)alpha, (const T *)beta, (const T *)n, (const T *)m, (T *)p, (T *)s, (T *)q, (T *)z, (T *)x, (T *)r, (T *)u, (T *)w, n_elts, parts, counter);
-  }
-  ISPM_HIP(hipGetLastError());
-  return ISPM_OK;
-}
 
 }  // namespace ispm
 
@@ -105,11 +115,11 @@ using namespace ispm;
 
 static int grid_for(int n)
 {
-  int g = num_cus() * 4;
-  if (g > kMaxBlocks) g = kMaxBlocks;
-  int need = (n + 255) / 256;
+  /* about one 16-byte access per thread (two elements for FP64, four for FP32), capped by
+   * the reduction-partials capacity */
+  int need = (n + 511) / 512;
   if (need < 1) need = 1;
-  return need < g ? need : g;
+  return need < kMaxBlocks ? need : kMaxBlocks;
 }
 
 static bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
@@ -172,15 +182,16 @@ extern "C" int ispm_dot(int type, const void *x, const void *y, void *s, int n)
   if (!x || !y || !s) return fail(ISPM_ERR_INVALID, "dot: null pointer");
   if (n < 0) return fail(ISPM_ERR_INVALID, "dot: n < 0");
   const int grid = grid_for(n);
-  reduce_scratch &rs = scratch();
+  double *parts = scratch().partials;
   const bool vec = aligned16(x) && aligned16(y);
-  unsigned *ctr = rs.counter + 0;
   if (type == ISPM_F64) {
-    if (vec) hipLaunchKernelGGL((dot_kernel<double, true>), dim3(grid), dim3(256), 0, stream(), (const double *)x, (const double *)y, (double *)s, n, rs.partials, ctr);
-    else     hipLaunchKernelGGL((dot_kernel<double, false>), dim3(grid), dim3(256), 0, stream(), (const double *)x, (const double *)y, (double *)s, n, rs.partials, ctr);
+    if (vec) hipLaunchKernelGGL((dot_kernel<double, true>), dim3(grid), dim3(256), 0, stream(), (const double *)x, (const double *)y, n, parts);
+    else     hipLaunchKernelGGL((dot_kernel<double, false>), dim3(grid), dim3(256), 0, stream(), (const double *)x, (const double *)y, n, parts);
+    hipLaunchKernelGGL(dot_finish<double>, dim3(1), dim3(kFinThreads), 0, stream(), (const double *)parts, grid, (double *)s);
   } else if (type == ISPM_F32) {
-    if (vec) hipLaunchKernelGGL((dot_kernel<float, true>), dim3(grid), dim3(256), 0, stream(), (const float *)x, (const float *)y, (float *)s, n, rs.partials, ctr);
-    else     hipLaunchKernelGGL((dot_kernel<float, false>), dim3(grid), dim3(256), 0, stream(), (const float *)x, (const float *)y, (float *)s, n, rs.partials, ctr);
+    if (vec) hipLaunchKernelGGL((dot_kernel<float, true>), dim3(grid), dim3(256), 0, stream(), (const float *)x, (const float *)y, n, parts);
+    else     hipLaunchKernelGGL((dot_kernel<float, false>), dim3(grid), dim3(256), 0, stream(), (const float *)x, (const float *)y, n, parts);
+    hipLaunchKernelGGL(dot_finish<float>, dim3(1), dim3(kFinThreads), 0, stream(), (const double *)parts, grid, (float *)s);
   } else return fail(ISPM_ERR_UNSUPPORTED, "dot: type %d", type);
   ISPM_HIP(hipGetLastError());
   return ISPM_OK;
@@ -214,7 +225,18 @@ extern "C" int ispm_ppcg_update_vectors(int type, void *resnorm, void *gamma, vo
   if (type != ISPM_F64 && type != ISPM_F32) return fail(ISPM_ERR_UNSUPPORTED, "ppcg_update_vectors: type %d", type);
   const bool vec = aligned16(n) && aligned16(m) && aligned16(p) && aligned16(s) && aligned16(q) &&
                    aligned16(z) && aligned16(x) && aligned16(r) && aligned16(u) && aligned16(w);
-  reduce_scratch &rs = scratch();
-  return launch_ppcg_update(type, vec, grid_for(n_elts), resnorm, gamma, delta, nullptr, alpha,
-                            beta, n, m, p, s, q, z, x, r, u, w, n_elts, rs.counter + 1);
+  const int grid = grid_for(n_elts);
+  if (type == ISPM_F64) {
+    typedef double T;
+    launch_ppcg_update<T>(vec, grid, (T *)resnorm, (T *)gamma, (T *)delta, (const T *)alpha, (const T *)beta,
+                          (const T *)n, (const T *)m, (T *)p, (T *)s, (T *)q, (T *)z, (T *)x, (T *)r,
+                          (T *)u, (T *)w, n_elts);
+  } else {
+    typedef float T;
+    launch_ppcg_update<T>(vec, grid, (T *)resnorm, (T *)gamma, (T *)delta, (const T *)alpha, (const T *)beta,
+                          (const T *)n, (const T *)m, (T *)p, (T *)s, (T *)q, (T *)z, (T *)x, (T *)r,
+                          (T *)u, (T *)w, n_elts);
+  }
+  ISPM_HIP(hipGetLastError());
+  return ISPM_OK;
 }

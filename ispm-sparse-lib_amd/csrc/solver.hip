@@ -19,6 +19,7 @@
  */
 #include <hip/hip_runtime.h>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 #include "device.h"
@@ -41,7 +42,7 @@ struct scalars {
   unsigned done, iters;
 };
 
-enum { OP_PQ, OP_RZ, OP_GAMMA, OP_DELTA, OP_R64 };
+enum { OP_PQ, OP_RZ, OP_GAMMA, OP_DELTA, OP_R64, OP_XR, OP_PPCG };
 
 template<typename T> struct vtraits {
   static constexpr int L = 16 / sizeof(T);
@@ -49,8 +50,8 @@ template<typename T> struct vtraits {
 
 template<typename T, int OP>
 __global__ void __launch_bounds__(256)
-k_dot(const T *__restrict__ x, const T *__restrict__ y, int n, scalars *s, double *parts,
-      unsigned *ctr, int skip_if_done)
+k_dot(const T *__restrict__ x, const T *__restrict__ y, int n, const scalars *s, double *parts,
+      int skip_if_done)
 {
   __shared__ double lds[4];
   if (skip_if_done && s->done) return;
@@ -63,14 +64,47 @@ k_dot(const T *__restrict__ x, const T *__restrict__ y, int n, scalars *s, doubl
     for (int l = 0; l < L; l++) v[0] += (double)a.v[l] * (double)b.v[l];
   }
   for (int i = nv * L + gt; i < n; i += gs) v[0] += (double)x[i] * (double)y[i];
-  double out[1];
-  if (finish_reduction<1>(v, parts, ctr, lds, out) && threadIdx.x == 0) {
-    const double r = out[0];
-    if (OP == OP_PQ) { s->pq = r; s->alpha = s->rho / r; }
-    if (OP == OP_RZ) { s->beta = r / s->rho; s->rho = r; }
-    if (OP == OP_GAMMA) s->gamma = r;
-    if (OP == OP_DELTA) s->delta = r;
-    if (OP == OP_R64) s->r64 = r;
+  store_partials<1>(v, parts, lds);
+}
+
+/* Second phase of every solver reduction: one block sums the partials of the kernel before
+ * it and applies the scalar part of the recurrence.  A finishing kernel skipped when the
+ * solve is done leaves the scalars as they are, like its reduction kernel. */
+template<int OP>
+__global__ void __launch_bounds__(kFinThreads) k_fin(const double *parts, int G, scalars *s, int skip_if_done)
+{
+  constexpr int NS = OP == OP_PPCG ? 4 : 1;
+  __shared__ double lds[NS * kFinThreads / 64];
+  if (skip_if_done && s->done) return;
+  double out[NS];
+  sum_partials<NS>(parts, G, lds, out);
+  if (threadIdx.x) return;
+  const double r = out[0];
+  if (OP == OP_PQ) { s->pq = r; s->alpha = s->rho / r; }
+  if (OP == OP_RZ) { s->beta = r / s->rho; s->rho = r; }
+  if (OP == OP_GAMMA) s->gamma = r;
+  if (OP == OP_DELTA) s->delta = r;
+  if (OP == OP_R64) s->r64 = r;
+  if (OP == OP_XR) {
+    s->rr = r;
+    s->iters++;
+    if (r <= s->tol2) s->done = 1;
+  }
+  if (OP == OP_PPCG) {
+    /* the same scalar update every block of k_ppcg made from the same inputs */
+    const double g = s->gamma, go = s->gamma_old, dl = s->delta, ao = s->alpha_old;
+    const double bd = g / go;
+    const double ad = g / (dl - bd * g / ao);
+    s->gamma_old = g;
+    s->alpha_old = ad;
+    s->alpha = ad;
+    s->beta = bd;
+    s->l1 = out[0];
+    s->gamma = out[1];
+    s->delta = out[2];
+    s->rr = out[3];
+    s->iters++;
+    if (out[3] <= s->tol2) s->done = 1;
   }
 }
 
@@ -78,7 +112,7 @@ k_dot(const T *__restrict__ x, const T *__restrict__ y, int n, scalars *s, doubl
 template<typename T>
 __global__ void __launch_bounds__(256)
 k_xr(T *__restrict__ x, T *__restrict__ r, const T *__restrict__ p, const T *__restrict__ q, int n,
-     scalars *s, double *parts, unsigned *ctr)
+     const scalars *s, double *parts)
 {
   __shared__ double lds[4];
   if (s->done) return;
@@ -103,12 +137,7 @@ k_xr(T *__restrict__ x, T *__restrict__ r, const T *__restrict__ p, const T *__r
     r[i] = ri;
     v[0] += (double)ri * (double)ri;
   }
-  double out[1];
-  if (finish_reduction<1>(v, parts, ctr, lds, out) && threadIdx.x == 0) {
-    s->rr = out[0];
-    s->iters++;
-    if (out[0] <= s->tol2) s->done = 1;
-  }
+  store_partials<1>(v, parts, lds);
 }
 
 /* p = z + beta p */
@@ -148,7 +177,7 @@ template<typename T>
 __global__ void __launch_bounds__(256)
 k_ppcg(const T *__restrict__ nn, const T *__restrict__ m, T *__restrict__ p, T *__restrict__ s_,
        T *__restrict__ q, T *__restrict__ z, T *__restrict__ x, T *__restrict__ r,
-       T *__restrict__ u, T *__restrict__ w, int n, scalars *s, double *parts, unsigned *ctr)
+       T *__restrict__ u, T *__restrict__ w, int n, const scalars *s, double *parts)
 {
   __shared__ double lds[16];
   if (s->done) return;
@@ -187,25 +216,13 @@ k_ppcg(const T *__restrict__ nn, const T *__restrict__ m, T *__restrict__ p, T *
     one(zi, qi, si, pi, ui, wi, ri, xi, nn[i], m[i]);
     z[i] = zi; q[i] = qi; s_[i] = si; p[i] = pi; u[i] = ui; w[i] = wi; r[i] = ri; x[i] = xi;
   }
-  double out[4];
-  if (finish_reduction<4>(acc, parts, ctr, lds, out) && threadIdx.x == 0) {
-    s->gamma_old = g;
-    s->alpha_old = ad;
-    s->alpha = ad;
-    s->beta = bd;
-    s->l1 = out[0];
-    s->gamma = out[1];
-    s->delta = out[2];
-    s->rr = out[3];
-    s->iters++;
-    if (out[3] <= s->tol2) s->done = 1;
-  }
+  store_partials<4>(acc, parts, lds);
 }
 
 /* r = b - Ax (FP64), (r,r) -> s->r64 */
 __global__ void __launch_bounds__(256)
 k_resid(const double *__restrict__ b, const double *__restrict__ Ax, double *__restrict__ r, int n,
-        scalars *s, double *parts, unsigned *ctr)
+        double *parts)
 {
   __shared__ double lds[4];
   double v[1] = {0.0};
@@ -215,8 +232,7 @@ k_resid(const double *__restrict__ b, const double *__restrict__ Ax, double *__r
     r[i] = ri;
     v[0] += ri * ri;
   }
-  double out[1];
-  if (finish_reduction<1>(v, parts, ctr, lds, out) && threadIdx.x == 0) s->r64 = out[0];
+  store_partials<1>(v, parts, lds);
 }
 
 /* dst = src / ||src|| with ||src||^2 = s->r64; s->rscale = ||src|| */
@@ -260,7 +276,6 @@ struct ispm_solver {
   scalars *s = nullptr;
   scalars *hs = nullptr;   /* pinned host mirror */
   double *parts = nullptr; /* reduction partials, private to this solver */
-  unsigned *ctr = nullptr; /* completion counters, private */
   int grid = 1;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
@@ -277,18 +292,22 @@ int spmv(ispm_plan *P, int vt, const void *x, void *y, const unsigned *skip)
   return spmv_internal(&P->launch, &P->dev, vt, x, y, skip);
 }
 
-template<typename T> int launch_dot(ispm_solver *S, int op, const void *x, const void *y, int skip, int slot)
+template<int OP> void finish(ispm_solver *S, int skip)
 {
-  unsigned *c = S->ctr + slot;
+  hipLaunchKernelGGL(k_fin<OP>, dim3(1), dim3(kFinThreads), 0, stream(), (const double *)S->parts, S->grid, S->s, skip);
+}
+
+template<typename T> int launch_dot(ispm_solver *S, int op, const void *x, const void *y, int skip)
+{
   const T *a = (const T *)x, *b = (const T *)y;
   hipStream_t st = stream();
   dim3 g(S->grid), bl(256);
   switch (op) {
-    case OP_PQ: hipLaunchKernelGGL((k_dot<T, OP_PQ>), g, bl, 0, st, a, b, S->n, S->s, S->parts, c, skip); break;
-    case OP_RZ: hipLaunchKernelGGL((k_dot<T, OP_RZ>), g, bl, 0, st, a, b, S->n, S->s, S->parts, c, skip); break;
-    case OP_GAMMA: hipLaunchKernelGGL((k_dot<T, OP_GAMMA>), g, bl, 0, st, a, b, S->n, S->s, S->parts, c, skip); break;
-    case OP_DELTA: hipLaunchKernelGGL((k_dot<T, OP_DELTA>), g, bl, 0, st, a, b, S->n, S->s, S->parts, c, skip); break;
-    default: hipLaunchKernelGGL((k_dot<T, OP_R64>), g, bl, 0, st, a, b, S->n, S->s, S->parts, c, skip); break;
+    case OP_PQ: hipLaunchKernelGGL((k_dot<T, OP_PQ>), g, bl, 0, st, a, b, S->n, S->s, S->parts, skip); finish<OP_PQ>(S, skip); break;
+    case OP_RZ: hipLaunchKernelGGL((k_dot<T, OP_RZ>), g, bl, 0, st, a, b, S->n, S->s, S->parts, skip); finish<OP_RZ>(S, skip); break;
+    case OP_GAMMA: hipLaunchKernelGGL((k_dot<T, OP_GAMMA>), g, bl, 0, st, a, b, S->n, S->s, S->parts, skip); finish<OP_GAMMA>(S, skip); break;
+    case OP_DELTA: hipLaunchKernelGGL((k_dot<T, OP_DELTA>), g, bl, 0, st, a, b, S->n, S->s, S->parts, skip); finish<OP_DELTA>(S, skip); break;
+    default: hipLaunchKernelGGL((k_dot<T, OP_R64>), g, bl, 0, st, a, b, S->n, S->s, S->parts, skip); finish<OP_R64>(S, skip); break;
   }
   ISPM_HIP(hipGetLastError());
   return ISPM_OK;
@@ -296,7 +315,7 @@ template<typename T> int launch_dot(ispm_solver *S, int op, const void *x, const
 
 int dot(ispm_solver *S, int op, const void *x, const void *y, int skip, int vt)
 {
-  return vt == ISPM_F64 ? launch_dot<double>(S, op, x, y, skip, 0) : launch_dot<float>(S, op, x, y, skip, 0);
+  return vt == ISPM_F64 ? launch_dot<double>(S, op, x, y, skip) : launch_dot<float>(S, op, x, y, skip);
 }
 
 /* z = M r (two SpMVs, or a copy without preconditioner) */
@@ -328,7 +347,8 @@ int iterate(ispm_solver *S)
     hipLaunchKernelGGL(k_ppcg<double>, g, b, 0, sm, (const double *)S->nv, (const double *)S->m,
                        (double *)S->p, (double *)S->sv, (double *)S->q, (double *)S->z,
                        (double *)S->xs, (double *)S->r, (double *)S->u, (double *)S->w, S->n, S->s,
-                       S->parts, S->ctr + 1);
+                       S->parts);
+    finish<OP_PPCG>(S, 1);
     ISPM_HIP(hipGetLastError());
     return ISPM_OK;
   }
@@ -341,10 +361,11 @@ int iterate(ispm_solver *S)
   if ((st = dot(S, OP_PQ, S->p, S->q, 1, S->T))) return st;
   if (S->T == ISPM_F64)
     hipLaunchKernelGGL(k_xr<double>, g, b, 0, sm, (double *)S->xs, (double *)S->r, (const double *)S->p,
-                       (const double *)S->q, S->n, S->s, S->parts, S->ctr + 1);
+                       (const double *)S->q, S->n, S->s, S->parts);
   else
     hipLaunchKernelGGL(k_xr<float>, g, b, 0, sm, (float *)S->xs, (float *)S->r, (const float *)S->p,
-                       (const float *)S->q, S->n, S->s, S->parts, S->ctr + 1);
+                       (const float *)S->q, S->n, S->s, S->parts);
+  finish<OP_XR>(S, 1);
   ISPM_HIP(hipGetLastError());
   return ISPM_OK;
 }
@@ -398,7 +419,8 @@ int true_residual(ispm_solver *S, const double *b, const double *x)
   int st = spmv(S->Ares ? S->Ares : S->A, ISPM_F64, x, S->tmp64, nullptr);
   if (st) return st;
   hipLaunchKernelGGL(k_resid, dim3(S->grid), dim3(256), 0, stream(), b, (const double *)S->tmp64, S->r64,
-                     S->n, S->s, S->parts, S->ctr + 2);
+                     S->n, S->parts);
+  finish<OP_R64>(S, 0);
   ISPM_HIP(hipGetLastError());
   return ISPM_OK;
 }
@@ -431,13 +453,21 @@ int ispm_solver_create(ispm_solver **out, ispm_plan *A, ispm_plan *M1, ispm_plan
   if (S->o.check_every <= 0) S->o.check_every = 16;
   if (S->o.maxit <= 0) S->o.maxit = 100000;
   if (S->o.reliable_delta <= 0 || S->o.reliable_delta >= 1) S->o.reliable_delta = 0.1;
+  /* ISPM_NO_GRAPH=1 launches iterations eagerly (profilers that cannot follow graph capture) */
+  if (const char *ng = std::getenv("ISPM_NO_GRAPH"))
+    if (ng[0] == '1') S->o.use_graph = 0;
   S->mixed = mixed;
   S->pipelined = o->method == ISPM_SOLVER_PPCG;
   S->T = T;
   S->ts = mixed ? 4 : 8;
   S->n = A->L.n_rows;
   const size_t vb = S->ts * (size_t)S->n + 64;
-  S->grid = std::min(num_cus() * 4, kMaxBlocks);
+  /* one 16-byte vector access per thread where possible (the vector kernels are latency
+   * bound with fewer, longer grid-stride loops) */
+  {
+    const long long per_block = 256LL * (16 / (long long)S->ts);
+    S->grid = (int)std::max(1LL, std::min((long long)kMaxBlocks, ((long long)S->n + per_block - 1) / per_block));
+  }
   void **vecs[] = {&S->r, &S->z, &S->q, &S->p, &S->t, &S->xs};
   for (void **v : vecs) if ((st = alloc(v, vb))) { ispm_solver_destroy(S); return st; }
   if (S->pipelined) {
@@ -445,8 +475,7 @@ int ispm_solver_create(ispm_solver **out, ispm_plan *A, ispm_plan *M1, ispm_plan
     for (void **v : pv) if ((st = alloc(v, vb))) { ispm_solver_destroy(S); return st; }
   }
   if ((st = alloc((void **)&S->r64, 8 * (size_t)S->n + 64)) || (st = alloc((void **)&S->tmp64, 8 * (size_t)S->n + 64)) ||
-      (st = alloc((void **)&S->s, sizeof(scalars))) || (st = alloc((void **)&S->parts, sizeof(double) * kMaxBlocks * 4)) ||
-      (st = alloc((void **)&S->ctr, sizeof(unsigned) * 4))) {
+      (st = alloc((void **)&S->s, sizeof(scalars))) || (st = alloc((void **)&S->parts, sizeof(double) * kMaxBlocks * 4))) {
     ispm_solver_destroy(S);
     return st;
   }
@@ -471,7 +500,7 @@ int ispm_solver_solve(ispm_solver *S, const double *b, double *x, ispm_solve_sta
   /* ||b|| */
   std::memset(S->hs, 0, sizeof(scalars));
   if ((st = push(S))) return st;
-  if ((st = launch_dot<double>(S, OP_R64, b, b, 0, 3))) return st;
+  if ((st = launch_dot<double>(S, OP_R64, b, b, 0))) return st;
   if ((st = pull(S))) return st;
   const double bnorm = std::sqrt(S->hs->r64) > 0 ? std::sqrt(S->hs->r64) : 1.0;
   const double tol = S->o.tol > 0 ? S->o.tol : 1e-10;
@@ -504,8 +533,8 @@ int ispm_solver_solve(ispm_solver *S, const double *b, double *x, ispm_solve_sta
       /* u = M r; w = A u; gamma = (r,u); delta = (w,u); recurrences restart */
       if ((e = apply_M(S, S->r, S->u, nullptr))) return e;
       if ((e = spmv(S->A, ISPM_F64, S->u, S->w, nullptr))) return e;
-      if ((e = launch_dot<double>(S, OP_GAMMA, S->r, S->u, 0, 3))) return e;
-      if ((e = launch_dot<double>(S, OP_DELTA, S->w, S->u, 0, 3))) return e;
+      if ((e = launch_dot<double>(S, OP_GAMMA, S->r, S->u, 0))) return e;
+      if ((e = launch_dot<double>(S, OP_DELTA, S->w, S->u, 0))) return e;
       for (void *v : {S->z, S->q, S->p, S->sv}) if ((e = zero(v, vb))) return e;
       if ((e = pull(S))) return e;
       h.gamma_old = INFINITY;
@@ -609,7 +638,6 @@ void ispm_solver_destroy(ispm_solver *S)
   ispm_free(S->tmp64);
   ispm_free(S->s);
   ispm_free(S->parts);
-  ispm_free(S->ctr);
   if (S->hs) (void)hipHostFree(S->hs);
   delete S;
 }

@@ -16,7 +16,16 @@ int spmv_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, 
   int st = spmvk::check_args(p, A, vec_type, x, y, H);
   if (st) return st;
   if (A->n_slices == 0) return ISPM_OK;
+  if (spmvk::use_stream(p, A)) return spmvk::launch_stream_types(A, vec_type, x, y, skip);
+  if (spmvk::use_packed(p, A)) return spmvk::launch_packed_types(p, A, vec_type, x, y, skip);
   return spmvk::launch_types<1>(p, A, vec_type, x, 0, y, 0, H, skip);
+}
+
+/* SpMM for the block solver: same checks as ispm_spmm */
+int spmm_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, int k,
+                  const void *X, size_t ldx, void *Y, size_t ldy)
+{
+  return ispm_spmm(p, A, vec_type, k, X, ldx, Y, ldy);
 }
 }  // namespace ispm
 
@@ -32,6 +41,8 @@ extern "C" int ispm_spmv(const ispm_launch *p, const ispm_slell_desc *A, int vec
   st = check_args(p, A, vec_type, x, y, H);
   if (st) return st;
   if (A->n_slices == 0) return ISPM_OK;
+  if (use_stream(p, A)) return launch_stream_types(A, vec_type, x, y, nullptr);
+  if (use_packed(p, A)) return launch_packed_types(p, A, vec_type, x, y, nullptr);
   return launch_types<1>(p, A, vec_type, x, 0, y, 0, H);
 }
 

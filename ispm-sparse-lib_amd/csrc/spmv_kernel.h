@@ -29,6 +29,7 @@
  */
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <cstdlib>
 #include "device.h"
 #include "internal.h"
 
@@ -48,7 +49,8 @@ struct spmv_args {
   unsigned n_slices;
   unsigned log2S;        /* fixed-height slices */
   unsigned n_cols;
-  unsigned per_xcd;      /* slices per XCD range */
+  unsigned per_xcd;      /* workgroups per XCD range */
+  unsigned spb;          /* consecutive slices each workgroup processes */
   const unsigned *skip;  /* if non-null and *skip != 0 the launch does nothing (solver done flag) */
 };
 
@@ -78,10 +80,11 @@ __global__ void __launch_bounds__(1024) slell_kernel(spmv_args a)
   const unsigned N = blockDim.x;
   const unsigned t = threadIdx.x;
   const unsigned b = blockIdx.x;
-  const unsigned slice = (b & 7u) * a.per_xcd + (b >> 3);
-  if (slice >= a.n_slices) return;
+  const unsigned group = (b & 7u) * a.per_xcd + (b >> 3);
   if (a.skip && *a.skip) return;
-
+  for (unsigned it = 0; it < a.spb; it++) {
+  const unsigned slice = group * a.spb + it;
+  if (slice >= a.n_slices) return;
   constexpr unsigned P = 1 + (W > 1) + (V ? 1 : 0) + (D ? 2 : 0);
   const unsigned *rec = a.sp + P * slice;
   const unsigned e0 = rec[0], e1 = rec[P];
@@ -144,11 +147,10 @@ __global__ void __launch_bounds__(1024) slell_kernel(spmv_args a)
       unsigned c[H];
       if (H == 2) {
         elt<T, E>::get2(el, e0 + off, e[0], e[H - 1]);
-        uint2 cc = *reinterpret_cast<const uint2 *>(cl + c0 + off);
-        c[0] = cc.x; c[H - 1] = cc.y;
+        dev::col_load2(cl + c0 + off, c[0], c[H - 1]);
       } else {
         e[0] = elt<T, E>::get(el, e0 + off);
-        c[0] = cl[c0 + off];
+        c[0] = dev::col_load(cl + c0 + off);
       }
 #pragma unroll
       for (int j = 0; j < K; j++)
@@ -170,12 +172,8 @@ __global__ void __launch_bounds__(1024) slell_kernel(spmv_args a)
         const unsigned boff = (nell << log2sz);
         const unsigned coff = ((nell + d) << log2sz) + row;
         unsigned c[H];
-        if (H == 2) {
-          uint2 cc = *reinterpret_cast<const uint2 *>(cl + c0 + coff);
-          c[0] = cc.x; c[H - 1] = cc.y;
-        } else {
-          c[0] = cl[c0 + coff];
-        }
+        if (H == 2) dev::col_load2(cl + c0 + coff, c[0], c[H - 1]);
+        else c[0] = dev::col_load(cl + c0 + coff);
         T ev[H], od[H];
         const unsigned eo = e0 + boff + ((2 * d) << log2sz) + row;
         if (H == 2) {
@@ -234,6 +232,7 @@ __global__ void __launch_bounds__(1024) slell_kernel(spmv_args a)
               acc[j][h] = s;
             }
         }
+        __syncthreads();   /* the next slice reuses the LDS buffer */
       }
     }
     if (t < g) {
@@ -241,6 +240,235 @@ __global__ void __launch_bounds__(1024) slell_kernel(spmv_args a)
       for (int j = 0; j < K; j++) vstore<T, H>::st(yv + j * a.ldy + y0 + pass * R + t * H, acc[j]);
     }
   }
+  }
+}
+
+
+/* Packed mapping for fixed slice heights S < 64: each wave covers 64/S consecutive slices,
+ * one row per lane, so every lane works even when a slice has only one or two planes (the
+ * slice-per-workgroup mapping would give each row 64/S lanes and leave most of them idle).
+ * Lanes of one slice read contiguous runs; consecutive slices follow each other in memory. */
+template<typename T, typename E, int W, bool D>
+__global__ void __launch_bounds__(256) slell_packed_kernel(spmv_args a)
+{
+  constexpr int U = 4;                     /* independent rows per lane */
+  const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned wpb = blockDim.x >> 6;
+  const unsigned log2S = a.log2S, S = 1u << log2S;
+  const unsigned spw = 64u >> log2S;
+  const unsigned group = (blockIdx.x & 7u) * a.per_xcd + (blockIdx.x >> 3);
+  if (a.skip && *a.skip) return;
+  constexpr unsigned P = 1 + (W > 1) + (D ? 2 : 0);
+  const E *__restrict__ el = static_cast<const E *>(a.elms);
+  const unsigned *__restrict__ cl = a.cols;
+  const T *__restrict__ xv = static_cast<const T *>(a.x);
+  T *__restrict__ yv = static_cast<T *>(a.y);
+  const unsigned ncols = a.n_cols;
+  const unsigned r = lane & (S - 1);
+  for (unsigned it = 0; it < a.spb; it++) {
+    /* this wave handles U consecutive groups of 64 rows */
+    const unsigned wg0 = ((group * a.spb + it) * wpb + wave) * U;
+    if (wg0 * spw >= a.n_slices) return;   /* wave-uniform; no workgroup barriers below */
+    unsigned slice[U], e0[U], c0[U], nell[U], nsec[U], d0[U];
+    bool on[U];
+    unsigned kmax = 0;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      slice[u] = (wg0 + u) * spw + (lane >> log2S);
+      on[u] = slice[u] < a.n_slices;
+      nell[u] = nsec[u] = 0;
+      e0[u] = c0[u] = d0[u] = 0;
+      if (on[u]) {
+        const unsigned *rec = a.sp + P * slice[u];
+        e0[u] = rec[0];
+        const unsigned e1 = rec[P];
+        if (D) {
+          c0[u] = rec[1]; d0[u] = rec[2];
+          nell[u] = (rec[1 + P] - c0[u]) >> log2S;
+          nsec[u] = ((e1 - e0[u]) >> log2S) - nell[u];
+        } else if (W > 1) {
+          c0[u] = rec[1];
+          const unsigned c1 = rec[1 + P];
+          const unsigned nse = (W * (c1 - c0[u]) - (e1 - e0[u])) / (W - 1);
+          nell[u] = nse >> log2S;
+          nsec[u] = ((c1 - c0[u]) >> log2S) - nell[u];
+        } else {
+          c0[u] = e0[u];
+          nell[u] = (e1 - e0[u]) >> log2S;
+        }
+      }
+      const unsigned tot = nell[u] + nsec[u];
+      kmax = tot > kmax ? tot : kmax;
+    }
+    T acc[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) acc[u] = 0;
+    for (unsigned k = 0; k < kmax; k++) {
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (k < nell[u]) {
+          const unsigned off = (k << log2S) + r;
+          acc[u] += elt<T, E>::get(el, e0[u] + off) * xload(xv, dev::col_load(cl + c0[u] + off), ncols);
+        } else if (k < nell[u] + nsec[u]) {
+          const unsigned d = k - nell[u];
+          if (D) {
+            const unsigned off = (k << log2S) + r;
+            acc[u] += elt<T, E>::get(el, e0[u] + off) * xload(xv, a.sp[d0[u] + d] + r - S, ncols);
+          } else if (W > 1) {
+            const unsigned c = 2 * dev::col_load(cl + c0[u] + (k << log2S) + r);
+            const unsigned eo = e0[u] + (nell[u] << log2S) + ((2 * d) << log2S) + r;
+            acc[u] += elt<T, E>::get(el, eo) * xload(xv, c, ncols) + elt<T, E>::get(el, eo + S) * xload(xv, c + 1, ncols);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (on[u]) yv[(slice[u] << log2S) + r] = acc[u];
+  }
+}
+
+/* Streamed mapping (mode 2) for fixed-height slices without hblock pairs or diagonal
+ * planes.  A 256-thread workgroup takes 256 consecutive rows (256/S slices), whose elements
+ * form one contiguous range of the element and column arrays.  Phase 1 reads that range
+ * with consecutive lanes on consecutive elements, so every fetched line is used in full by
+ * one instruction, and leaves the products e*x[col] in LDS in storage order; phase 2 gives
+ * each thread one row and adds that row's products plane by plane, in storage order.  A
+ * workgroup whose range exceeds the LDS capacity reads its rows directly.  Suited to
+ * irregular, short rows (approximate-inverse factors), where the packed mapping's lanes
+ * touch one short run per slice per instruction. */
+template<typename T, typename E>
+__global__ void __launch_bounds__(256) slell_stream_kernel(spmv_args a)
+{
+  constexpr unsigned CAP = 2048;
+  __shared__ T prod[CAP];
+  __shared__ unsigned soff[257];
+  if (a.skip && *a.skip) return;
+  const unsigned log2S = a.log2S, S = 1u << log2S;
+  const unsigned spg = 256u >> log2S;
+  const unsigned t = threadIdx.x;
+  const unsigned i = t >> log2S, r = t & (S - 1);
+  const E *__restrict__ el = static_cast<const E *>(a.elms);
+  const unsigned *__restrict__ cl = a.cols;
+  const T *__restrict__ xv = static_cast<const T *>(a.x);
+  T *__restrict__ yv = static_cast<T *>(a.y);
+  const unsigned ncols = a.n_cols;
+  for (unsigned it = 0; it < a.spb; it++) {
+    const unsigned group = ((blockIdx.x & 7u) * a.per_xcd + (blockIdx.x >> 3)) * a.spb + it;
+    const unsigned s0 = group * spg;
+    if (s0 >= a.n_slices) return;   /* uniform over the workgroup */
+    const unsigned ns = min(spg, a.n_slices - s0);
+    if (t <= ns) soff[t] = a.sp[s0 + t];
+    __syncthreads();
+    const unsigned E0 = soff[0], n = soff[ns] - E0;
+    T acc = 0;
+    if (n <= CAP) {
+      for (unsigned b = t; b < n; b += 1024) {
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const unsigned j = b + u * 256;
+          v[u] = 0;
+          if (j < n) v[u] = elt<T, E>::get(el, E0 + j) * xload(xv, dev::col_load(cl + E0 + j), ncols);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          if (b + u * 256 < n) prod[b + u * 256] = v[u];
+      }
+      __syncthreads();
+      if (i < ns) {
+        const unsigned o = soff[i] - E0, nk = (soff[i + 1] - soff[i]) >> log2S;
+        for (unsigned k = 0; k < nk; k++) acc += prod[o + (k << log2S) + r];
+      }
+    } else if (i < ns) {
+      const unsigned o = soff[i], nk = (soff[i + 1] - o) >> log2S;
+      for (unsigned k = 0; k < nk; k++) {
+        const unsigned e = o + (k << log2S) + r;
+        acc += elt<T, E>::get(el, e) * xload(xv, dev::col_load(cl + e), ncols);
+      }
+    }
+    if (i < ns) yv[((s0 + i) << log2S) + r] = acc;
+    __syncthreads();   /* soff and prod are rewritten by the next group */
+  }
+}
+
+template<typename T, typename E>
+int launch_stream(const ispm_slell_desc *A, const void *x, void *y, const unsigned *skip)
+{
+  spmv_args a;
+  a.skip = skip;
+  a.sp = A->slice_ptr;
+  a.cols = A->cols;
+  a.elms = A->elms;
+  a.x = x;
+  a.y = y;
+  a.ldx = a.ldy = 0;
+  a.n_slices = A->n_slices;
+  a.log2S = (unsigned)(31 - __builtin_clz(A->slice_height));
+  a.n_cols = (unsigned)A->n_cols;
+  const unsigned spg = 256u >> a.log2S;
+  const unsigned groups = (A->n_slices + spg - 1) / spg;
+  unsigned spb = 1;
+  if (const char *e = std::getenv("ISPM_SPMV_SPB")) spb = (unsigned)std::max(1, std::atoi(e));
+  a.spb = spb;
+  a.per_xcd = ((groups + spb - 1) / spb + 7) / 8;
+  hipLaunchKernelGGL((slell_stream_kernel<T, E>), dim3(a.per_xcd * 8), dim3(256), 0, stream(), a);
+  ISPM_HIP(hipGetLastError());
+  return ISPM_OK;
+}
+
+inline int launch_stream_types(const ispm_slell_desc *A, int vt, const void *x, void *y,
+                               const unsigned *skip)
+{
+  const int et = A->elm_type;
+  if (vt == ISPM_F64 && et == ISPM_F64) return launch_stream<double, double>(A, x, y, skip);
+  if (vt == ISPM_F64 && et == ISPM_F32) return launch_stream<double, float>(A, x, y, skip);
+  if (vt == ISPM_F32 && et == ISPM_F32) return launch_stream<float, float>(A, x, y, skip);
+  if (vt == ISPM_F32 && et == ISPM_BF16) return launch_stream<float, bf16_t>(A, x, y, skip);
+  return fail(ISPM_ERR_UNSUPPORTED, "spmv: unsupported (vector, element) types (%d, %d)", vt, et);
+}
+
+template<typename T, typename E>
+int launch_packed(const ispm_launch *p, const ispm_slell_desc *A, const void *x, void *y,
+                  const unsigned *skip)
+{
+  spmv_args a;
+  a.skip = skip;
+  a.sp = A->slice_ptr;
+  a.cols = A->cols;
+  a.elms = A->elms;
+  a.x = x;
+  a.y = y;
+  a.ldx = a.ldy = 0;
+  a.n_slices = A->n_slices;
+  a.log2S = (unsigned)(31 - __builtin_clz(A->slice_height));
+  a.n_cols = (unsigned)A->n_cols;
+  const unsigned wpb = 4;                          /* 256-thread workgroups */
+  const unsigned spw = 64u >> a.log2S;
+  const unsigned waves = ((A->n_slices + spw - 1) / spw + 3) / 4;   /* 4 row groups per wave */
+  unsigned spb = 1;
+  if (const char *e = std::getenv("ISPM_SPMV_SPB")) spb = (unsigned)std::max(1, std::atoi(e));
+  a.spb = spb;
+  const unsigned groups = (waves + wpb * spb - 1) / (wpb * spb);
+  a.per_xcd = (groups + 7) / 8;
+  const unsigned grid = a.per_xcd * 8;
+  hipStream_t s = stream();
+  if (A->hblock == 2) hipLaunchKernelGGL((slell_packed_kernel<T, E, 2, false>), dim3(grid), dim3(64 * wpb), 0, s, a);
+  else if (A->diags) hipLaunchKernelGGL((slell_packed_kernel<T, E, 1, true>), dim3(grid), dim3(64 * wpb), 0, s, a);
+  else hipLaunchKernelGGL((slell_packed_kernel<T, E, 1, false>), dim3(grid), dim3(64 * wpb), 0, s, a);
+  ISPM_HIP(hipGetLastError());
+  return ISPM_OK;
+}
+
+inline int launch_packed_types(const ispm_launch *p, const ispm_slell_desc *A, int vt, const void *x,
+                               void *y, const unsigned *skip)
+{
+  const int et = A->elm_type;
+  if (vt == ISPM_F64 && et == ISPM_F64) return launch_packed<double, double>(p, A, x, y, skip);
+  if (vt == ISPM_F64 && et == ISPM_F32) return launch_packed<double, float>(p, A, x, y, skip);
+  if (vt == ISPM_F32 && et == ISPM_F32) return launch_packed<float, float>(p, A, x, y, skip);
+  if (vt == ISPM_F32 && et == ISPM_BF16) return launch_packed<float, bf16_t>(p, A, x, y, skip);
+  return fail(ISPM_ERR_UNSUPPORTED, "spmv: unsupported (vector, element) types (%d, %d)", vt, et);
 }
 
 template<typename T, typename E, int H, int K>
@@ -259,7 +487,14 @@ int launch_tv(const ispm_launch *p, const ispm_slell_desc *A, const void *x, siz
   a.n_slices = A->n_slices;
   a.log2S = A->var_height ? 0u : (unsigned)(31 - __builtin_clz(A->slice_height));
   a.n_cols = (unsigned)A->n_cols;
-  a.per_xcd = (A->n_slices + 7) / 8;
+  /* slices per workgroup: keep about 64 workgroups of work per CU in flight and let each
+   * workgroup walk consecutive slices (amortises wave launch for small slices) */
+  unsigned spb = 1;
+  if (const char *e = std::getenv("ISPM_SPMV_SPB")) spb = (unsigned)std::max(1, std::atoi(e));
+  else if (p->N == 64 && A->var_height) spb = A->n_slices > (1u << 16) ? 8u : 1u;
+  a.spb = spb;
+  const unsigned groups = (A->n_slices + spb - 1) / spb;
+  a.per_xcd = (groups + 7) / 8;
   const unsigned N = p->N;
   const unsigned grid = a.per_xcd * 8;
   /* LDS is needed only when partial sums of one row live in different waves. */
@@ -322,6 +557,22 @@ int launch_types(const ispm_launch *p, const ispm_slell_desc *A, int vt, const v
 
 inline bool pow2(unsigned v) { return v && !(v & (v - 1)); }
 
+/* streamed mapping: requested by the launch (mode 2), fixed heights up to 256 rows without
+ * hblock pairs or diagonal planes */
+inline bool use_stream(const ispm_launch *p, const ispm_slell_desc *A)
+{
+  return p->mode == 2 && !A->var_height && A->slice_height <= 256 && A->slice_height >= 1 &&
+         A->hblock == 1 && !A->diags;
+}
+
+/* packed mapping: requested by the launch (mode 1; also mode 2 on layouts the streamed
+ * mapping does not take), fixed-height slices below 64 rows */
+inline bool use_packed(const ispm_launch *p, const ispm_slell_desc *A)
+{
+  return (p->mode == 1 || p->mode == 2) && !A->var_height && A->slice_height < 64 &&
+         A->slice_height >= 1;
+}
+
 inline int check_args(const ispm_launch *p, const ispm_slell_desc *A, int vt, const void *x, void *y,
                unsigned &H)
 {
@@ -332,6 +583,8 @@ inline int check_args(const ispm_launch *p, const ispm_slell_desc *A, int vt, co
     return fail(ISPM_ERR_UNSUPPORTED, "spmv: invalid launch parameters N=%u S=%u H=%u", p->N, p->S, p->H);
   if (!pow2(p->N) || p->N < 64 || p->N > 1024 || (p->H != 1 && p->H != 2))
     return fail(ISPM_ERR_UNSUPPORTED, "spmv: unsupported launch N=%u H=%u", p->N, p->H);
+  if (p->mode < 0 || p->mode > 2)
+    return fail(ISPM_ERR_UNSUPPORTED, "spmv: launch mode %d unsupported", p->mode);
   if (A->hblock != 1 && A->hblock != 2)
     return fail(ISPM_ERR_UNSUPPORTED, "spmv: hblock %u unsupported", A->hblock);
   if (A->hblock == 2 && A->diags)

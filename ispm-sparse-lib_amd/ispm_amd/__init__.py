@@ -46,7 +46,7 @@ class SlellDesc(C.Structure):
 
 class Launch(C.Structure):
     _fields_ = [("N", C.c_uint), ("S", C.c_uint), ("H", C.c_uint),
-                ("use_tex_sliceptr", C.c_int), ("use_tex_x", C.c_int)]
+                ("use_tex_sliceptr", C.c_int), ("use_tex_x", C.c_int), ("mode", C.c_int)]
 
 
 class Csr(C.Structure):
@@ -121,6 +121,8 @@ _SIGS = {
                                             C.c_uint, C.c_uint, C.c_int, C.c_int, C.POINTER(Launch)]),
     "ispm_plan_info_get": (C.c_int, [C.c_void_p, C.POINTER(PlanInfo)]),
     "ispm_plan_default_launch": (C.c_int, [C.c_uint, C.c_int, C.POINTER(Launch)]),
+    "ispm_plan_estimate": (C.c_int, [C.c_int, C.c_int, C.POINTER(Csr), C.POINTER(C.c_uint), C.POINTER(C.c_uint),
+                                     C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(Launch)]),
     "ispm_plan_tune": (C.c_int, [C.c_int, C.c_int, C.POINTER(Csr), C.POINTER(C.c_uint), C.POINTER(C.c_uint),
                                  C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(Launch),
                                  C.POINTER(C.c_double)]),
@@ -145,6 +147,13 @@ _SIGS = {
     "ispm_solver_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "ispm_solver_destroy": (None, [C.c_void_p]),
     "ispm_solver_set_residual_plan": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "ispm_block_solver_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "ispm_block_solver_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "ispm_block_solver_destroy": (None, [C.c_void_p]),
+    "ispm_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ispm_batch_layout": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.c_void_p]),
+    "ispm_batch_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_void_p, C.c_void_p]),
+    "ispm_batch_destroy": (None, [C.c_void_p]),
 }
 
 # Extra entry points appended by later headers (solver); registered when present.
@@ -373,9 +382,9 @@ class DeviceSlell:
         check(lib().ispm_slell_upload(C.byref(h), C.byref(self.desc)))
         self.host = hs
 
-    def spmv(self, x: DeviceArray, y: DeviceArray, vec_type, N=64, S=None, H=1):
+    def spmv(self, x: DeviceArray, y: DeviceArray, vec_type, N=64, S=None, H=1, mode=0):
         S = self.desc.slice_height if S is None else S
-        lp = Launch(N, S, H, 0, 0)
+        lp = Launch(N, S, H, 0, 0, mode)
         return lib().ispm_spmv(C.byref(lp), C.byref(self.desc), vec_type, x.vp, y.vp)
 
     def __del__(self):

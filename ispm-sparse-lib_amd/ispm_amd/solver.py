@@ -6,7 +6,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import (BF16, CSR, F32, F64, DeviceArray, Plan, check, lib)
+from . import (BF16, CSR, F32, F64, PLAN_ESTIMATE, DeviceArray, Plan, check, lib)
 
 PPCG, PCG = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
@@ -63,7 +63,9 @@ class Solver:
             self.M_plans = M_plans
         elif M is not None:
             ft = factor_type if factor_type is not None else (BF16 if mixed else F64)
-            self.M_plans = (Plan(M[0], vt, elm_type=ft), Plan(M[1], vt, elm_type=ft))
+            # factor rows are irregular: take the byte-minimal format
+            self.M_plans = (Plan(M[0], vt, elm_type=ft, kind=PLAN_ESTIMATE),
+                            Plan(M[1], vt, elm_type=ft, kind=PLAN_ESTIMATE))
         self.n = self.A_plan.n_rows_padded
         o = SolverOpts(method, PREC_MIXED if mixed else PREC_FP64, tol, maxit, check_every,
                        int(use_graph), reliable_delta)

@@ -1,5 +1,6 @@
 /* plan.h -- SpMV plan (plan/plan.h:1-161 of the reference), same members and methods.
- * PLAN_DEFAULT / PLAN_ESTIMATE take the guessed format with a CDNA4 launch shape;
+ * PLAN_DEFAULT takes the guessed format with a CDNA4 launch shape; PLAN_ESTIMATE the format
+ * with the fewest stored bytes (ispm_plan_estimate);
  * PLAN_EXHAUSTIVE times every format and launch shape on the device (ispm_plan_tune). */
 #ifndef ISPM_PLAN_H
 #define ISPM_PLAN_H
@@ -80,6 +81,14 @@ spmv_plan<T, E>::spmv_plan(const csr_matrix<E> &m, spmv_plan_kind kind, unsigned
     mparams = slell_params{S, W, (unsigned)V, (unsigned)D};
     host_matrix = new slell_matrix<E, host_memory_space_tag>(csr_to_slell(m, S, W, V != 0, D != 0));
     upload();
+  } else if (kind == PLAN_ESTIMATE) {
+    ispm_csr c = ispm_detail::csr_view(m);
+    unsigned S, W;
+    int V, D;
+    ISPM_CHECK(ispm_plan_estimate(ispm_type_code<T>::value, ispm_type_code<E>::value, &c, &S, &W, &V, &D, &lp));
+    mparams = slell_params{S, W, (unsigned)V, (unsigned)D};
+    host_matrix = new slell_matrix<E, host_memory_space_tag>(csr_to_slell(m, S, W, V != 0, D != 0));
+    if (up) upload();
   } else {
     host_matrix = new slell_matrix<E, host_memory_space_tag>(csr_to_slell_guess_format(m));
     if (up) upload();
@@ -87,7 +96,7 @@ spmv_plan<T, E>::spmv_plan(const csr_matrix<E> &m, spmv_plan_kind kind, unsigned
                            (unsigned)host_matrix->var_height, (unsigned)host_matrix->diags};
     ISPM_CHECK(ispm_plan_default_launch(host_matrix->slice_height, host_matrix->var_height, &lp));
   }
-  params = spmv_launch_params{lp.N, lp.S, lp.H, usetex, usetex};
+  params = spmv_launch_params{lp.N, lp.S, lp.H, usetex, usetex, lp.mode};
 }
 
 #endif

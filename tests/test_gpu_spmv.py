@@ -15,7 +15,8 @@ pytestmark = pytest.mark.gpu
 
 FORMATS = [dict(S=32), dict(S=64), dict(S=256), dict(S=64, D=True), dict(S=256, D=True),
            dict(S=128, hblock=2), dict(S=32, hblock=2), dict(S=256, V=True),
-           dict(S=64, V=True, D=True), dict(S=128, V=True, hblock=2), dict(S=512)]
+           dict(S=64, V=True, D=True), dict(S=128, V=True, hblock=2), dict(S=512),
+           dict(S=4), dict(S=8, D=True), dict(S=16, hblock=2)]
 LAUNCH = [(64, 1), (128, 1), (256, 2), (512, 1), (1024, 2), (64, 2)]
 TYPES = [(I.F64, I.F64), (I.F64, I.F32), (I.F32, I.F32), (I.F32, I.BF16)]
 
@@ -31,7 +32,7 @@ def rounded_vals(A, et):
     return v
 
 
-def run_spmv(hs, x, vt, N, H):
+def run_spmv(hs, x, vt, N, H, mode=0):
     dm = I.DeviceSlell(hs)
     dt = np.float64 if vt == I.F64 else np.float32
     xv = np.zeros(max(hs.n_rows, hs.n_cols), dt)
@@ -39,7 +40,7 @@ def run_spmv(hs, x, vt, N, H):
     xd = I.DeviceArray.from_numpy(xv)
     yd = I.DeviceArray(hs.n_rows, dt)
     yd.upload(np.full(hs.n_rows, 7.0, dt))
-    st = dm.spmv(xd, yd, vt, N=N, S=hs.slice_height if not hs.var_height else N * H, H=H)
+    st = dm.spmv(xd, yd, vt, N=N, S=hs.slice_height if not hs.var_height else N * H, H=H, mode=mode)
     I.synchronize()
     return st, yd.numpy()
 
@@ -67,7 +68,24 @@ def mats(gpu):
         "irregular": G.random_sparse(3000, 11, min_len=1, max_len=60),
         "pairs": G.random_sparse(2500, 12, min_len=2, max_len=30, pair_bias=0.7),
         "skewed": G.random_sparse(1800, 13, min_len=0, max_len=200),
+        "short_long": short_long_rows(),
     }
+
+
+def short_long_rows(n=6000, seed=14):
+    """Rows of 1-3 entries with every 700th row holding 2500: workgroups of the streamed
+    mapping (256 rows) fall on both sides of its LDS capacity."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(1, 4, n)
+    lens[::700] = 2500
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    cols = np.concatenate([np.sort(rng.choice(n, l, replace=False)) for l in lens]).astype(np.int32)
+    vals = rng.uniform(-1, 1, rp[-1])
+    return I.CSR(n, n, rp, cols, vals)
+
+
+def stream_ok(hs):
+    return not hs.var_height and hs.hblock == 1 and not hs.diags and hs.slice_height <= 256
 
 
 @pytest.mark.parametrize("fmt", FORMATS, ids=lambda f: "-".join(f"{k}{int(v)}" for k, v in f.items()))
@@ -85,18 +103,42 @@ def test_spmv_formats_fp64(mats, name, fmt):
         assert st == 0, I.lib().ispm_last_error()
         check(A, hs, x, y, I.F64, I.F64)
         ran += 1
+    if not hs.var_height and hs.slice_height < 64:
+        st, y = run_spmv(hs, x, I.F64, 256, 1, mode=1)      # packed rows
+        assert st == 0, I.lib().ispm_last_error()
+        check(A, hs, x, y, I.F64, I.F64)
+    if not hs.var_height and hs.slice_height <= 256:
+        # streamed rows (plain planes), else the packed / slice-per-workgroup fallback
+        st, y = run_spmv(hs, x, I.F64, 256, 1, mode=2)
+        assert st == 0, I.lib().ispm_last_error()
+        check(A, hs, x, y, I.F64, I.F64)
     assert ran > 0
 
 
+@pytest.mark.parametrize("S", [4, 8, 32, 256])
 @pytest.mark.parametrize("vt,et", TYPES)
-@pytest.mark.parametrize("fmt", [dict(S=64, D=True), dict(S=128, hblock=2), dict(S=256, V=True), dict(S=32)],
+def test_spmv_stream_capacity(mats, S, vt, et):
+    """Streamed mapping on rows that put some workgroups over the LDS capacity."""
+    A = mats["short_long"]
+    hs = I.csr_to_slell(A, S, out_type=et)
+    assert stream_ok(hs)
+    x = G.uniform(103, A.n_cols)
+    st, y = run_spmv(hs, x, vt, 256, 1, mode=2)
+    assert st == 0, I.lib().ispm_last_error()
+    check(A, hs, x, y, vt, et)
+
+
+@pytest.mark.parametrize("vt,et", TYPES)
+@pytest.mark.parametrize("fmt", [dict(S=64, D=True), dict(S=128, hblock=2), dict(S=256, V=True), dict(S=32),
+                                 dict(S=8, mode=1), dict(S=4, D=True, mode=1), dict(S=4, mode=2),
+                                 dict(S=16, mode=2)],
                          ids=lambda f: "-".join(f"{k}{int(v)}" for k, v in f.items()))
 def test_spmv_types(mats, vt, et, fmt):
     A = mats["pairs"] if fmt.get("hblock") else mats["irregular"]
     hs = I.csr_to_slell(A, fmt["S"], fmt.get("hblock", 1), fmt.get("V", False), fmt.get("D", False), out_type=et)
     x = G.uniform(101, A.n_cols)
     N, H = (256, 1)
-    st, y = run_spmv(hs, x, vt, N, H)
+    st, y = run_spmv(hs, x, vt, N, H, mode=fmt.get("mode", 0))
     assert st == 0, I.lib().ispm_last_error()
     check(A, hs, x, y, vt, et)
 
@@ -111,6 +153,10 @@ def test_spmv_integer_exact(mats, fmt):
         if not (N >= S // H and S >= H):
             continue
         st, y = run_spmv(hs, x, I.F64, N, H)
+        assert st == 0
+        check(A0, hs, x, y, I.F64, I.F64, integer=True)
+    if stream_ok(hs):
+        st, y = run_spmv(hs, x, I.F64, 256, 1, mode=2)
         assert st == 0
         check(A0, hs, x, y, I.F64, I.F64, integer=True)
 
