@@ -65,6 +65,26 @@ def spmv_leg(I, G, A, vt, et, steps, warmup):
     }
 
 
+def roofline(leg, key, kernel):
+    """HBM roofline entry of one SpMV leg: achieved = stored (algorithmic) bytes per launch over
+    the live average launch duration; traffic = the HBM bytes per launch that rocprofv3
+    FETCH_SIZE / WRITE_SIZE measured for the same plan (profiles/spmv_27pt_192_traffic.json,
+    counters scaled by the factors a 1 GiB copy gave), null when no measurement of this
+    format is on record."""
+    traffic = None
+    path = os.path.join(ROOT, "profiles", "spmv_27pt_192_traffic.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            rec = json.load(f).get(key)
+        same = rec and rec.get("stored_bytes") == leg["stored_bytes"] and all(
+            rec["format"].get(k) == leg["format"][k] for k in ("S", "hblock", "V", "D"))
+        if same:
+            traffic = rec["traffic_bytes"]
+    return {"bound": "hbm", "kernel": kernel, "achieved": leg["GBps"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": leg["frac_hbm"], "traffic": traffic,
+            "algorithmic_bytes": leg["stored_bytes"]}
+
+
 def cpu_spmv_baseline(O, A, seconds=10.0):
     threads = O.set_threads(0)
     x = np.random.default_rng(0).uniform(-1, 1, A.n_rows)
@@ -137,6 +157,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--droptol", type=float, default=0.125)   # fastainv_sym default (fastainv.h:14)
     ap.add_argument("--spmv-only", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="first copy 1 GiB device-to-device 3 times (a plain copy of known size, "
+                         "the reference for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the library has no "
@@ -147,6 +170,14 @@ def main():
     import oracle as O
     I.init()
     log("device", I.device_string())
+    if args.calibrate:
+        nb = 1 << 30
+        src, dst = I.DeviceArray(nb // 8), I.DeviceArray(nb // 8)
+        for _ in range(3):
+            I.check(I.lib().ispm_memcpy(dst.vp, src.vp, nb, 2))   # ISPM_D2D
+        I.synchronize()
+        del src, dst
+        log("calibration: 3 device-to-device copies of 1 GiB")
     t = time.perf_counter()
     A = G.stencil27_contrast(192)
     log(f"generated 27-point 192^3: n={A.n_rows} nnz={A.nnz} in {time.perf_counter() - t:.1f}s")
@@ -180,7 +211,6 @@ def main():
             f"{v['replacements']} repl, oracle relres {v['oracle_relres']:.2e} [{time.perf_counter() - t:.1f}s]")
     fp64_best = min(solves["fp64_ppcg"]["seconds"], solves["fp64_pcg"]["seconds"])
     mixed = solves["mixed_pcg"]
-    dom = legs["fp64"]
     cpu = None
     if not args.no_cpu:
         cpu = cpu_solve_baseline(O, A, b, factors, solves["fp64_ppcg"]["iterations"])
@@ -205,9 +235,11 @@ def main():
         "fsai_nnz_per_row": factors[0].nnz / A.n_rows,
         "solves": solves,
         "spmv": legs,
-        "roofline": {"bound": "hbm", "kernel": "slell_kernel<double,double> (27-point 192^3 SpMV)",
-                     "achieved": dom["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": dom["frac_hbm"], "traffic": None},
+        "roofline": roofline(legs["fp32"], "fp32",
+                             "slell_diagpat_kernel<float,float> (FP32 27-point 192^3 SpMV, the "
+                             "largest share of the mixed solve)"),
+        "roofline_fp64_spmv": roofline(legs["fp64"], "fp64",
+                                       "slell_diagpat_kernel<double,double> (FP64 27-point 192^3 SpMV)"),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

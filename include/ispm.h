@@ -210,6 +210,8 @@ typedef struct ispm_plan_info {
   size_t stored_bytes;         /* 4|slice_ptr| + 4|cols| + sizeof(E)|elms| + sizeof(T)(n_cols+n_rows) */
   size_t csr_bytes;            /* 4(n+1) + (4+sizeof(E))nnz + sizeof(T)*2n */
   ispm_slell_desc device;      /* device matrix (valid unless NO_DEVICE) */
+  unsigned pattern_planes;     /* diagonal planes of the pattern most slices share (0: none) */
+  unsigned pattern_slices;     /* slices that have exactly that pattern */
 } ispm_plan_info;
 
 /* Replaces spmv_plan<T,E>::spmv_plan(csr, kind, flags) (plan/plan.h:64-146):

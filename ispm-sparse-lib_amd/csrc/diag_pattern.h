@@ -1,0 +1,30 @@
+/* diag_pattern.h -- the diagonal pattern that most slices of a plan share (host and device
+ * code; no HIP dependencies).
+ *
+ * In a diagonal slice (fixed height S) plane d reads x[pos_d - S + r] for its rows r < S,
+ * pos_d being the plane's stored diagonal position.  For a stencil matrix the positions
+ * relative to the slice's first row are the same in almost every slice, and they come in
+ * runs of nearby values (three per (dy, dz) line of a 27-point stencil).  The plan records
+ * that shared list once, with one LDS window per run -- x[first row + rrel[r] + i] for
+ * i < rlen[r], staged at rbase[r] -- and marks the slices whose positions equal it, so the
+ * SpMV reads those slices' x from LDS without loading any position. */
+#ifndef ISPM_DIAG_PATTERN_H
+#define ISPM_DIAG_PATTERN_H
+
+namespace ispm {
+
+constexpr int kPatPlanes = 32;
+
+struct diag_pattern {
+  unsigned n;                    /* diagonal planes of the shared list */
+  unsigned nrun;                 /* runs of nearby diagonals */
+  unsigned poff[kPatPlanes];     /* LDS index of plane d's x entry for row 0 */
+  int rrel[kPatPlanes];          /* run r's window starts at x[first row + rrel[r]] */
+  unsigned rbase[kPatPlanes];    /* LDS index of run r's window */
+  unsigned rlen[kPatPlanes];     /* window length (S + span of the run) */
+  unsigned lds_len;              /* sum of the window lengths */
+};
+
+}  // namespace ispm
+
+#endif

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <string>
 #include "ispm.h"
+#include "diag_pattern.h"
 
 namespace ispm {
 
@@ -41,6 +42,12 @@ reduce_scratch &scratch();
 int num_cus();
 
 size_t type_size(int t);
+
+/* SpMV with the checks of ispm_spmv, an optional device skip flag (nothing is done while
+ * *skip != 0) and a plan's shared diagonal pattern with its slice mask (both may be null). */
+int spmv_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, const void *x,
+                  void *y, const unsigned *skip, const diag_pattern *pat = nullptr,
+                  const unsigned *uni = nullptr);
 
 }  // namespace ispm
 

@@ -4,6 +4,7 @@
 
 #include <vector>
 #include "convert.h"
+#include "diag_pattern.h"
 #include "ispm.h"
 
 struct ispm_plan {
@@ -16,6 +17,10 @@ struct ispm_plan {
   double tuned = 0;
   size_t stored = 0, csr = 0;
   void *dev_map = nullptr;                /* device copy of the slot map (on-device refresh) */
+  bool has_pat = false;                   /* diagonal pattern shared by most slices */
+  ispm::diag_pattern pat{};
+  unsigned *dev_uni = nullptr;            /* device bit mask of the slices that have it */
+  unsigned pat_slices = 0;
 };
 
 namespace ispm {
@@ -24,6 +29,7 @@ unsigned short f32_to_bf16(float f);
 int fill_values(const conv::layout &L, const ispm_csr *A, int et, void *elms);
 int upload_layout(const conv::layout &L, const void *elms, int et, ispm_slell_desc *d);
 ispm_launch default_launch(const conv::layout &L);
+bool build_pattern(const conv::layout &L, diag_pattern &pat, std::vector<unsigned> &mask);
 }
 
 #endif

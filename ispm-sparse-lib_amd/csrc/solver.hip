@@ -29,8 +29,6 @@
 
 namespace ispm {
 
-int spmv_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, const void *x,
-                  void *y, const unsigned *skip);
 
 namespace {
 
@@ -289,7 +287,7 @@ int alloc(void **p, size_t bytes) { return ispm_malloc(p, bytes); }
 
 int spmv(ispm_plan *P, int vt, const void *x, void *y, const unsigned *skip)
 {
-  return spmv_internal(&P->launch, &P->dev, vt, x, y, skip);
+  return spmv_internal(&P->launch, &P->dev, vt, x, y, skip, P->has_pat ? &P->pat : nullptr, P->dev_uni);
 }
 
 template<int OP> void finish(ispm_solver *S, int skip)

@@ -8,14 +8,16 @@ int spmm_dispatch(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, 
 }
 
 namespace ispm {
-/* SpMV for the solver: same checks as ispm_spmv, optional device skip flag. */
+/* SpMV for the solver and plans: same checks as ispm_spmv, optional device skip flag, and
+ * the plan's shared diagonal pattern when it has one. */
 int spmv_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, const void *x,
-                  void *y, const unsigned *skip)
+                  void *y, const unsigned *skip, const diag_pattern *pat, const unsigned *uni)
 {
   unsigned H;
   int st = spmvk::check_args(p, A, vec_type, x, y, H);
   if (st) return st;
   if (A->n_slices == 0) return ISPM_OK;
+  if (spmvk::use_diagpat(p, A, vec_type, H, pat, uni)) return spmvk::launch_diagpat_types(A, vec_type, *pat, uni, x, y, skip);
   if (spmvk::use_stream(p, A)) return spmvk::launch_stream_types(A, vec_type, x, y, skip);
   if (spmvk::use_packed(p, A)) return spmvk::launch_packed_types(p, A, vec_type, x, y, skip);
   return spmvk::launch_types<1>(p, A, vec_type, x, 0, y, 0, H, skip);

@@ -32,6 +32,7 @@
 #include <cstdlib>
 #include "device.h"
 #include "internal.h"
+#include "diag_pattern.h"
 
 namespace ispm {
 namespace spmvk {
@@ -243,6 +244,141 @@ __global__ void __launch_bounds__(1024) slell_kernel(spmv_args a)
   }
 }
 
+
+/* Diagonal slices with one row per thread (fixed S, N == S, H == 1, one vector) of a plan
+ * whose slices mostly share one diagonal pattern (diag_pattern.h).  A slice marked in `uni`
+ * has no ELL planes and exactly the pattern's diagonals: its elements for all planes are
+ * requested at once, the pattern's x windows are loaded into LDS with coalesced loads
+ * (addresses known from the slice index alone), and every plane reads its x from LDS.  Other
+ * slices take the generic path, reading positions and x directly.  Both paths add the ELL
+ * planes, then the diagonal planes in storage order, as slell_kernel does, so the results
+ * are identical to it. */
+template<typename T, typename E, unsigned LOG2S, int H>
+__global__ void __launch_bounds__((1u << LOG2S) / H)
+slell_diagpat_kernel(spmv_args a, diag_pattern pat, const unsigned *__restrict__ uni)
+{
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *win = reinterpret_cast<T *>(lds_raw);
+  constexpr unsigned S = 1u << LOG2S, NT = S / H;
+  const unsigned t = threadIdx.x, r0 = t * H;   /* this thread's first row in the slice */
+  /* one slice per workgroup */
+  const unsigned slice = (blockIdx.x & 7u) * a.per_xcd + (blockIdx.x >> 3);
+  if (slice >= a.n_slices) return;
+  if (a.skip && *a.skip) return;
+  const E *__restrict__ el = static_cast<const E *>(a.elms);
+  const unsigned *__restrict__ sp = a.sp;
+  const T *__restrict__ xv = static_cast<const T *>(a.x);
+  T *__restrict__ yv = static_cast<T *>(a.y);
+  const unsigned ncols = a.n_cols;
+  const unsigned row0 = slice << LOG2S;
+  const unsigned e0 = sp[3 * slice];
+  T acc[H];
+#pragma unroll
+  for (int h = 0; h < H; h++) acc[h] = 0;
+  if ((uni[slice >> 5] >> (slice & 31u)) & 1u) {
+    const unsigned np = pat.n;
+    T ev[kPatPlanes][H];
+    const E *ep = el + e0 + r0;
+#pragma unroll
+    for (unsigned j = 0; j < kPatPlanes; j++)
+      if (j < np) {
+        if (H == 2) elt<T, E>::get2(ep, j << LOG2S, ev[j][0], ev[j][H - 1]);
+        else ev[j][0] = elt<T, E>::get(ep, j << LOG2S);
+      }
+    for (unsigned r = 0; r < pat.nrun; r++) {
+      const unsigned x0 = row0 + (unsigned)pat.rrel[r], L = pat.rlen[r], b = pat.rbase[r];
+      for (unsigned i = t; i < L; i += NT) win[b + i] = xload(xv, x0 + i, ncols);
+    }
+    __syncthreads();
+#pragma unroll
+    for (unsigned j = 0; j < kPatPlanes; j++)
+      if (j < np)
+#pragma unroll
+        for (int h = 0; h < H; h++) acc[h] += ev[j][h] * win[pat.poff[j] + r0 + h];
+  } else {
+    const unsigned *__restrict__ cl = a.cols;
+    const unsigned c0 = sp[3 * slice + 1], d0 = sp[3 * slice + 2];
+    const unsigned e1 = sp[3 * slice + 3], c1 = sp[3 * slice + 4];
+    const unsigned nell = (c1 - c0) >> LOG2S;
+    const unsigned nsec = ((e1 - e0) >> LOG2S) - nell;
+#pragma unroll 4
+    for (unsigned k = 0; k < nell; k++)
+#pragma unroll
+      for (int h = 0; h < H; h++) {
+        const unsigned off = (k << LOG2S) + r0 + h;
+        acc[h] += elt<T, E>::get(el, e0 + off) * xload(xv, dev::col_load(cl + c0 + off), ncols);
+      }
+#pragma unroll 8
+    for (unsigned d = 0; d < nsec; d++) {
+      const unsigned pos = sp[d0 + d];
+#pragma unroll
+      for (int h = 0; h < H; h++) {
+        const unsigned off = ((nell + d) << LOG2S) + r0 + h;
+        acc[h] += elt<T, E>::get(el, e0 + off) * xload(xv, pos + r0 + h - S, ncols);
+      }
+    }
+  }
+  vstore<T, H>::st(yv + row0 + r0, acc);
+}
+
+template<typename T, typename E>
+int launch_diagpat(const ispm_slell_desc *A, const diag_pattern &pat, const unsigned *uni,
+                   const void *x, void *y, const unsigned *skip)
+{
+  /* Rows per thread is the kernel's choice (each row's sum is the same either way): two for
+   * FP32 vectors (half the load instructions; 205 -> 184 us on a 27-point 192^3 stencil),
+   * one for FP64, where two rows per thread run no faster. */
+  const unsigned H = (sizeof(T) == 4 && A->slice_height >= 128 &&
+                      (reinterpret_cast<uintptr_t>(y) % (2 * sizeof(T))) == 0) ? 2u : 1u;
+  spmv_args a;
+  a.skip = skip;
+  a.sp = A->slice_ptr;
+  a.cols = A->cols;
+  a.elms = A->elms;
+  a.x = x;
+  a.y = y;
+  a.ldx = a.ldy = 0;
+  a.n_slices = A->n_slices;
+  a.log2S = (unsigned)(31 - __builtin_clz(A->slice_height));
+  a.n_cols = (unsigned)A->n_cols;
+  a.spb = 1;
+  a.per_xcd = (A->n_slices + 7) / 8;
+  const size_t lds = (size_t)pat.lds_len * sizeof(T);
+  if (lds > 65536) return fail(ISPM_ERR_UNSUPPORTED, "spmv: pattern windows need %zu bytes of LDS", lds);
+  const dim3 g(a.per_xcd * 8), b(A->slice_height / H);
+  hipStream_t st = stream();
+  switch (A->slice_height * 4 + H) {
+    case 64 * 4 + 1: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 6, 1>), g, b, lds, st, a, pat, uni); break;
+    case 128 * 4 + 1: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 7, 1>), g, b, lds, st, a, pat, uni); break;
+    case 256 * 4 + 1: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 8, 1>), g, b, lds, st, a, pat, uni); break;
+    case 64 * 4 + 2: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 6, 2>), g, b, lds, st, a, pat, uni); break;
+    case 128 * 4 + 2: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 7, 2>), g, b, lds, st, a, pat, uni); break;
+    case 256 * 4 + 2: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 8, 2>), g, b, lds, st, a, pat, uni); break;
+    default: return fail(ISPM_ERR_UNSUPPORTED, "spmv: pattern kernel for slice height %u", A->slice_height);
+  }
+  ISPM_HIP(hipGetLastError());
+  return ISPM_OK;
+}
+
+/* pattern path: a plan's diagonal fixed-height slices, one row per thread, single vector */
+inline bool use_diagpat(const ispm_launch *p, const ispm_slell_desc *A, int vt, unsigned H,
+                        const diag_pattern *pat, const unsigned *uni)
+{
+  return pat && uni && pat->n && p->mode == 0 && !A->var_height && A->diags && A->hblock == 1 &&
+         (H == 1 || H == 2) && p->N * H == A->slice_height && A->slice_height >= 64 &&
+         A->slice_height <= 256 && (size_t)pat->lds_len * type_size(vt) <= 65536;
+}
+
+inline int launch_diagpat_types(const ispm_slell_desc *A, int vt, const diag_pattern &pat,
+                                const unsigned *uni, const void *x, void *y, const unsigned *skip)
+{
+  const int et = A->elm_type;
+  if (vt == ISPM_F64 && et == ISPM_F64) return launch_diagpat<double, double>(A, pat, uni, x, y, skip);
+  if (vt == ISPM_F64 && et == ISPM_F32) return launch_diagpat<double, float>(A, pat, uni, x, y, skip);
+  if (vt == ISPM_F32 && et == ISPM_F32) return launch_diagpat<float, float>(A, pat, uni, x, y, skip);
+  if (vt == ISPM_F32 && et == ISPM_BF16) return launch_diagpat<float, bf16_t>(A, pat, uni, x, y, skip);
+  return fail(ISPM_ERR_UNSUPPORTED, "spmv: unsupported (vector, element) types (%d, %d)", vt, et);
+}
 
 /* Packed mapping for fixed slice heights S < 64: each wave covers 64/S consecutive slices,
  * one row per lane, so every lane works even when a slice has only one or two planes (the

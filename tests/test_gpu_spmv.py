@@ -115,6 +115,42 @@ def test_spmv_formats_fp64(mats, name, fmt):
     assert ran > 0
 
 
+def stencil_then_irregular():
+    """27-point rows (16^3 grid) followed by irregular rows in their own column block:
+    slices with the shared diagonal pattern and slices without it."""
+    A = G.stencil27_contrast(16)
+    B = G.random_sparse(4096, 15, min_len=1, max_len=30)
+    rp = np.concatenate([A.row_ptr, A.row_ptr[-1] + B.row_ptr[1:]]).astype(np.int64)
+    cols = np.concatenate([A.cols, B.cols + A.n_cols]).astype(np.int32)
+    vals = np.concatenate([A.vals, B.vals])
+    return I.CSR(A.n_rows + B.n_rows, A.n_cols + B.n_cols, rp, cols, vals)
+
+
+@pytest.mark.parametrize("S,H", [(64, 1), (128, 1), (256, 1), (128, 2), (256, 2)])
+@pytest.mark.parametrize("vt,et", TYPES)
+def test_spmv_diag_pattern(mats, S, H, vt, et):
+    """Plans whose diagonal slices mostly share one pattern: those slices stage the x windows
+    of the pattern in LDS, the others read x directly (one kernel, both paths)."""
+    for name, A, planes in (("poisson7", mats["poisson7"], 7), ("stencil27", G.stencil27_contrast(16), 27),
+                            ("mixed", stencil_then_irregular(), 27)):
+        p = I.Plan(A, vt, elm_type=et, explicit=dict(S=S, D=True), launch=I.Launch(S // H, S, H, 0, 0, 0))
+        inf = p.info()
+        assert inf.pattern_planes == planes, name
+        n_slices = p.n_rows_padded // S
+        assert 0 < inf.pattern_slices <= n_slices
+        if name == "mixed":
+            assert inf.pattern_slices < n_slices
+        n = p.n_rows_padded
+        dt = np.float64 if vt == I.F64 else np.float32
+        x = G.uniform(104, A.n_cols)
+        xd = I.DeviceArray.from_numpy(np.concatenate([x, np.zeros(n - A.n_cols)]).astype(dt))
+        yd = I.DeviceArray(n, dt)
+        yd.upload(np.full(n, 7.0, dt))
+        p.execute(xd, yd)
+        I.synchronize()
+        check(A, None, x, yd.numpy(), vt, et)
+
+
 @pytest.mark.parametrize("S", [4, 8, 32, 256])
 @pytest.mark.parametrize("vt,et", TYPES)
 def test_spmv_stream_capacity(mats, S, vt, et):
