@@ -193,8 +193,9 @@ ispm_launch default_launch(const conv::layout &L)
 static int make_layout(const ispm_csr *A, int et, unsigned S, unsigned W, int V, int D,
                        conv::layout &L)
 {
+  const int min_height = (!V && S < 4) ? (int)S : 4;
   int r = conv::convert(A->row_ptr, A->n_rows, A->cols, A->n_cols, A->n_nz, (int)tsize(et),
-                        (int)S, (int)W, V != 0, D != 0, 4, L);
+                        (int)S, (int)W, V != 0, D != 0, min_height, L);
   return conv_status(r);
 }
 
@@ -333,8 +334,8 @@ int estimate_format(const ispm_csr *A, int vt, int et, unsigned &bS, unsigned &b
   const long long nnz = rp[n];
   const double es = (double)tsize(et);
   double best = INFINITY;
-  for (unsigned S = 4; S <= 512; S <<= 1)
-    for (int V = 0; V <= 1; V++) {
+  for (unsigned S = 1; S <= 512; S <<= 1)
+    for (int V = 0; V <= (S >= 4 ? 1 : 0); V++) {
       std::vector<int> h;
       if (V) {
         if (conv::var_slice_heights(rp.data(), n, (int)S, nnz, 4, h)) continue;
@@ -378,7 +379,7 @@ int tune_exhaustive(const ispm_csr *A, int vt, int et, unsigned &bS, unsigned &b
       (st = ispm_malloc(&x2, tsize(vt) * ncols)) || (st = ispm_malloc(&y2, tsize(vt) * nvec)))
     return st;
   /* the reference's formats (S = 32..512) plus small fixed heights run in packed mode */
-  for (unsigned S = 4; S <= 512; S <<= 1)
+  for (unsigned S = 1; S <= 512; S <<= 1)
     for (int V = 0; V <= (S >= 32 ? 1 : 0); V++)
       for (int D = 0; D <= 1; D++)
         for (unsigned W = 1; W <= (D ? 1u : 2u); W <<= 1) {
@@ -444,6 +445,10 @@ int ispm_csr_to_slell(const ispm_csr *A, int S, int hblock, int var_height, int 
   if (st) return st;
   if (!out) return fail_plain(ISPM_ERR_INVALID, "csr_to_slell: null output");
   if (S <= 0 || hblock <= 0) return fail_plain(ISPM_ERR_INVALID, "csr_to_slell: bad slice height / hblock");
+  /* conversion.h:383-384 raises a fixed height below min_height to min_height rows per slice
+   * while recording the requested height; such a matrix would not describe its own slices */
+  if (!var_height && S < min_height)
+    return fail_plain(ISPM_ERR_INVALID, "csr_to_slell: fixed slice height below min_height");
   int et = out_elm_type < 0 ? A->elm_type : out_elm_type;
   if (!tsize(et)) return fail_plain(ISPM_ERR_INVALID, "csr_to_slell: bad output type");
   conv::layout L;

@@ -465,66 +465,87 @@ __global__ void __launch_bounds__(256) slell_packed_kernel(spmv_args a)
 }
 
 /* Streamed mapping (mode 2) for fixed-height slices without hblock pairs or diagonal
- * planes.  A 256-thread workgroup takes 256 consecutive rows (256/S slices), whose elements
- * form one contiguous range of the element and column arrays.  Phase 1 reads that range
- * with consecutive lanes on consecutive elements, so every fetched line is used in full by
- * one instruction, and leaves the products e*x[col] in LDS in storage order; phase 2 gives
- * each thread one row and adds that row's products plane by plane, in storage order.  A
- * workgroup whose range exceeds the LDS capacity reads its rows directly.  Suited to
- * irregular, short rows (approximate-inverse factors), where the packed mapping's lanes
- * touch one short run per slice per instruction. */
-template<typename T, typename E>
+ * planes.  A 256-thread workgroup takes 256*RPT consecutive rows, whose elements form one
+ * contiguous range of the element and column arrays.  Phase 1 reads that range with
+ * consecutive lanes on consecutive elements (every fetched line used in full by one
+ * instruction), eight elements per thread in flight, and leaves the products e*x[col] in LDS
+ * in storage order; phase 2 gives each thread RPT rows and adds each row's products plane
+ * by plane, in storage order.  A workgroup whose range exceeds the LDS capacity reads its
+ * rows directly.  Suited to irregular, short rows (approximate-inverse factors): the work of
+ * a workgroup is three dependent memory round trips (offsets, elements and columns, x), so
+ * larger groups keep more bytes in flight per round trip. */
+template<typename T, typename E, int RPT>
 __global__ void __launch_bounds__(256) slell_stream_kernel(spmv_args a)
 {
-  constexpr unsigned CAP = 2048;
+  constexpr unsigned ROWS = 256u * RPT, CAP = 2048u * RPT, U = 8;
   __shared__ T prod[CAP];
-  __shared__ unsigned soff[257];
+  __shared__ unsigned soff[ROWS + 1];
   if (a.skip && *a.skip) return;
   const unsigned log2S = a.log2S, S = 1u << log2S;
-  const unsigned spg = 256u >> log2S;
+  const unsigned spg = ROWS >> log2S;
   const unsigned t = threadIdx.x;
-  const unsigned i = t >> log2S, r = t & (S - 1);
   const E *__restrict__ el = static_cast<const E *>(a.elms);
   const unsigned *__restrict__ cl = a.cols;
   const T *__restrict__ xv = static_cast<const T *>(a.x);
   T *__restrict__ yv = static_cast<T *>(a.y);
   const unsigned ncols = a.n_cols;
-  for (unsigned it = 0; it < a.spb; it++) {
-    const unsigned group = ((blockIdx.x & 7u) * a.per_xcd + (blockIdx.x >> 3)) * a.spb + it;
-    const unsigned s0 = group * spg;
-    if (s0 >= a.n_slices) return;   /* uniform over the workgroup */
-    const unsigned ns = min(spg, a.n_slices - s0);
-    if (t <= ns) soff[t] = a.sp[s0 + t];
-    __syncthreads();
-    const unsigned E0 = soff[0], n = soff[ns] - E0;
-    T acc = 0;
-    if (n <= CAP) {
-      for (unsigned b = t; b < n; b += 1024) {
-        T v[4];
+  const unsigned group = (blockIdx.x & 7u) * a.per_xcd + (blockIdx.x >> 3);
+  const unsigned s0 = group * spg;
+  if (s0 >= a.n_slices) return;   /* uniform over the workgroup */
+  const unsigned ns = min(spg, a.n_slices - s0);
+  /* the element range comes from two (scalar) loads, so the element loads need not wait for
+   * the offsets table, which only phase 2 reads */
+  const unsigned E0 = a.sp[s0], n = a.sp[s0 + ns] - E0;
+  for (unsigned q = t; q <= ns; q += 256) soff[q] = a.sp[s0 + q];   /* ns + 1 <= ROWS + 1 */
+  T acc[RPT];
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const unsigned j = b + u * 256;
-          v[u] = 0;
-          if (j < n) v[u] = elt<T, E>::get(el, E0 + j) * xload(xv, dev::col_load(cl + E0 + j), ncols);
+  for (int k = 0; k < RPT; k++) acc[k] = 0;
+  if (n <= CAP) {
+    for (unsigned b = t; b < n; b += 256 * U) {
+      T e[U];
+      unsigned c[U];
+#pragma unroll
+      for (unsigned u = 0; u < U; u++) {
+        const unsigned j = b + u * 256;
+        e[u] = 0;
+        c[u] = 0;
+        if (j < n) {
+          e[u] = elt<T, E>::get(el, E0 + j);
+          c[u] = dev::col_load(cl + E0 + j);
         }
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-          if (b + u * 256 < n) prod[b + u * 256] = v[u];
       }
-      __syncthreads();
+#pragma unroll
+      for (unsigned u = 0; u < U; u++)
+        if (b + u * 256 < n) prod[b + u * 256] = e[u] * xload(xv, c[u], ncols);
+    }
+  }
+  __syncthreads();   /* offsets table and products */
+  if (n <= CAP) {
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+      const unsigned row = t + k * 256u, i = row >> log2S, r = row & (S - 1);
       if (i < ns) {
         const unsigned o = soff[i] - E0, nk = (soff[i + 1] - soff[i]) >> log2S;
-        for (unsigned k = 0; k < nk; k++) acc += prod[o + (k << log2S) + r];
-      }
-    } else if (i < ns) {
-      const unsigned o = soff[i], nk = (soff[i + 1] - o) >> log2S;
-      for (unsigned k = 0; k < nk; k++) {
-        const unsigned e = o + (k << log2S) + r;
-        acc += elt<T, E>::get(el, e) * xload(xv, dev::col_load(cl + e), ncols);
+        for (unsigned q = 0; q < nk; q++) acc[k] += prod[o + (q << log2S) + r];
       }
     }
-    if (i < ns) yv[((s0 + i) << log2S) + r] = acc;
-    __syncthreads();   /* soff and prod are rewritten by the next group */
+  } else {
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+      const unsigned row = t + k * 256u, i = row >> log2S, r = row & (S - 1);
+      if (i < ns) {
+        const unsigned o = soff[i], nk = (soff[i + 1] - o) >> log2S;
+        for (unsigned q = 0; q < nk; q++) {
+          const unsigned e = o + (q << log2S) + r;
+          acc[k] += elt<T, E>::get(el, e) * xload(xv, dev::col_load(cl + e), ncols);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < RPT; k++) {
+    const unsigned row = t + k * 256u;
+    if ((row >> log2S) < ns) yv[(s0 << log2S) + row] = acc[k];
   }
 }
 
@@ -542,13 +563,19 @@ int launch_stream(const ispm_slell_desc *A, const void *x, void *y, const unsign
   a.n_slices = A->n_slices;
   a.log2S = (unsigned)(31 - __builtin_clz(A->slice_height));
   a.n_cols = (unsigned)A->n_cols;
-  const unsigned spg = 256u >> a.log2S;
+  a.spb = 1;
+  /* rows per thread: LDS per workgroup grows with it and limits occupancy; measured best
+   * on approximate-inverse factors: one for FP64 vectors, two for FP32 */
+  unsigned rpt = sizeof(T) == 8 ? 1u : 2u;
+  if (const char *e = std::getenv("ISPM_STREAM_RPT")) rpt = (unsigned)std::atoi(e);
+  if (rpt != 1 && rpt != 2 && rpt != 4) rpt = sizeof(T) == 8 ? 1u : 2u;
+  const unsigned spg = (256u * rpt) >> a.log2S;
   const unsigned groups = (A->n_slices + spg - 1) / spg;
-  unsigned spb = 1;
-  if (const char *e = std::getenv("ISPM_SPMV_SPB")) spb = (unsigned)std::max(1, std::atoi(e));
-  a.spb = spb;
-  a.per_xcd = ((groups + spb - 1) / spb + 7) / 8;
-  hipLaunchKernelGGL((slell_stream_kernel<T, E>), dim3(a.per_xcd * 8), dim3(256), 0, stream(), a);
+  a.per_xcd = (groups + 7) / 8;
+  const dim3 g(a.per_xcd * 8), b(256);
+  if (rpt == 1) hipLaunchKernelGGL((slell_stream_kernel<T, E, 1>), g, b, 0, stream(), a);
+  else if (rpt == 2) hipLaunchKernelGGL((slell_stream_kernel<T, E, 2>), g, b, 0, stream(), a);
+  else hipLaunchKernelGGL((slell_stream_kernel<T, E, 4>), g, b, 0, stream(), a);
   ISPM_HIP(hipGetLastError());
   return ISPM_OK;
 }

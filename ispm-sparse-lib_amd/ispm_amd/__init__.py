@@ -350,7 +350,9 @@ class HostSlell:
         return dict(S=self.slice_height, hblock=self.hblock, V=self.var_height, D=self.diags)
 
 
-def csr_to_slell(A: CSR, S, hblock=1, var_height=False, diags=False, min_height=4, out_type=-1):
+def csr_to_slell(A: CSR, S, hblock=1, var_height=False, diags=False, min_height=None, out_type=-1):
+    if min_height is None:
+        min_height = 4 if var_height or S >= 4 else int(S)
     h = SlellHost()
     check(lib().ispm_csr_to_slell(C.byref(A.c()), int(S), int(hblock), int(var_height), int(diags),
                                   int(min_height), int(out_type), C.byref(h)))

@@ -145,3 +145,14 @@ def test_bf16_and_f32_narrowing():
     ref = A.to_scipy().toarray()
     m = ref != 0
     assert np.all(np.abs(got[m] - ref[m]) <= 2.0 ** -8 * np.abs(ref[m]))
+
+
+def test_fixed_height_below_min_height_rejected():
+    """A fixed slice height below min_height would give slices of min_height rows that the
+    recorded height does not describe (conversion.h:383-384): rejected."""
+    A = G.random_sparse(300, 31, min_len=0, max_len=5)
+    with pytest.raises(I.IspmError):
+        I.csr_to_slell(A, 2, min_height=4)
+    for S in (1, 2):
+        hs = I.csr_to_slell(A, S)          # min_height follows S
+        assert hs.slice_height == S and hs.n_slices * S == hs.n_rows

@@ -151,7 +151,7 @@ def test_spmv_diag_pattern(mats, S, H, vt, et):
         check(A, None, x, yd.numpy(), vt, et)
 
 
-@pytest.mark.parametrize("S", [4, 8, 32, 256])
+@pytest.mark.parametrize("S", [1, 2, 4, 8, 32, 256])
 @pytest.mark.parametrize("vt,et", TYPES)
 def test_spmv_stream_capacity(mats, S, vt, et):
     """Streamed mapping on rows that put some workgroups over the LDS capacity."""
@@ -162,6 +162,10 @@ def test_spmv_stream_capacity(mats, S, vt, et):
     st, y = run_spmv(hs, x, vt, 256, 1, mode=2)
     assert st == 0, I.lib().ispm_last_error()
     check(A, hs, x, y, vt, et)
+    if S < 64:
+        st, y = run_spmv(hs, x, vt, 256, 1, mode=1)
+        assert st == 0, I.lib().ispm_last_error()
+        check(A, hs, x, y, vt, et)
 
 
 @pytest.mark.parametrize("vt,et", TYPES)
