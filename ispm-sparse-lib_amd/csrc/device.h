@@ -48,6 +48,38 @@ template<typename T> struct elt<T, bf16_t> {
     b = (T)__uint_as_float(v & 0xffff0000u);
   }
 };
+/* VEC consecutive elements (16 bytes, 8 for BF16) from a VEC-aligned index, converted to T */
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+typedef unsigned nt_u4 __attribute__((ext_vector_type(4)));
+template<typename E> struct evec { static constexpr unsigned n = 4; };
+template<> struct evec<double> { static constexpr unsigned n = 2; };
+template<typename T> __device__ __forceinline__ void elt_vec(const double *p, unsigned i, T (&v)[2])
+{
+  nt_d2 w = __builtin_nontemporal_load(reinterpret_cast<const nt_d2 *>(p + i));
+  v[0] = (T)w.x; v[1] = (T)w.y;
+}
+template<typename T> __device__ __forceinline__ void elt_vec(const float *p, unsigned i, T (&v)[4])
+{
+  nt_f4 w = __builtin_nontemporal_load(reinterpret_cast<const nt_f4 *>(p + i));
+  v[0] = (T)w.x; v[1] = (T)w.y; v[2] = (T)w.z; v[3] = (T)w.w;
+}
+template<typename T> __device__ __forceinline__ void elt_vec(const bf16_t *p, unsigned i, T (&v)[4])
+{
+  nt_u2 w = __builtin_nontemporal_load(reinterpret_cast<const nt_u2 *>(p + i));
+  v[0] = (T)__uint_as_float(w.x << 16); v[1] = (T)__uint_as_float(w.x & 0xffff0000u);
+  v[2] = (T)__uint_as_float(w.y << 16); v[3] = (T)__uint_as_float(w.y & 0xffff0000u);
+}
+__device__ __forceinline__ void col_vec(const unsigned *p, unsigned i, unsigned (&c)[2])
+{
+  nt_u2 w = __builtin_nontemporal_load(reinterpret_cast<const nt_u2 *>(p + i));
+  c[0] = w.x; c[1] = w.y;
+}
+__device__ __forceinline__ void col_vec(const unsigned *p, unsigned i, unsigned (&c)[4])
+{
+  nt_u4 w = __builtin_nontemporal_load(reinterpret_cast<const nt_u4 *>(p + i));
+  c[0] = w.x; c[1] = w.y; c[2] = w.z; c[3] = w.w;
+}
+
 __device__ __forceinline__ unsigned col_load(const unsigned *p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ void col_load2(const unsigned *p, unsigned &a, unsigned &b)
 {

@@ -253,13 +253,15 @@ __global__ void __launch_bounds__(1024) slell_kernel(spmv_args a)
  * slices take the generic path, reading positions and x directly.  Both paths add the ELL
  * planes, then the diagonal planes in storage order, as slell_kernel does, so the results
  * are identical to it. */
+template<typename T> using vecx = T __attribute__((ext_vector_type(16 / sizeof(T))));
+
 template<typename T, typename E, unsigned LOG2S, int H>
 __global__ void __launch_bounds__((1u << LOG2S) / H)
 slell_diagpat_kernel(spmv_args a, diag_pattern pat, const unsigned *__restrict__ uni)
 {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *win = reinterpret_cast<T *>(lds_raw);
-  constexpr unsigned S = 1u << LOG2S, NT = S / H;
+  constexpr unsigned S = 1u << LOG2S, NT = S / H, VX = 16 / sizeof(T);
   const unsigned t = threadIdx.x, r0 = t * H;   /* this thread's first row in the slice */
   /* one slice per workgroup */
   const unsigned slice = (blockIdx.x & 7u) * a.per_xcd + (blockIdx.x >> 3);
@@ -270,6 +272,7 @@ slell_diagpat_kernel(spmv_args a, diag_pattern pat, const unsigned *__restrict__
   const T *__restrict__ xv = static_cast<const T *>(a.x);
   T *__restrict__ yv = static_cast<T *>(a.y);
   const unsigned ncols = a.n_cols;
+  const bool xvec = (reinterpret_cast<uintptr_t>(xv) % 16) == 0;
   const unsigned row0 = slice << LOG2S;
   const unsigned e0 = sp[3 * slice];
   T acc[H];
@@ -285,9 +288,35 @@ slell_diagpat_kernel(spmv_args a, diag_pattern pat, const unsigned *__restrict__
         if (H == 2) elt<T, E>::get2(ep, j << LOG2S, ev[j][0], ev[j][H - 1]);
         else ev[j][0] = elt<T, E>::get(ep, j << LOG2S);
       }
-    for (unsigned r = 0; r < pat.nrun; r++) {
-      const unsigned x0 = row0 + (unsigned)pat.rrel[r], L = pat.rlen[r], b = pat.rbase[r];
-      for (unsigned i = t; i < L; i += NT) win[b + i] = xload(xv, x0 + i, ncols);
+    if (xvec) {
+      /* 16-byte loads from the aligned cover of each window; entries outside the window
+       * are not stored (they belong to neighbouring windows) */
+      for (unsigned r = 0; r < pat.nrun; r++) {
+        const unsigned x0 = row0 + (unsigned)pat.rrel[r], L = pat.rlen[r], b = pat.rbase[r];
+        const unsigned xa = x0 & ~(VX - 1), nvec = (x0 - xa + L + VX - 1) / VX;
+        for (unsigned v = t; v < nvec; v += NT) {
+          const unsigned xi = xa + v * VX;
+          T w[VX];
+          if (xi < ncols && xi + VX <= ncols) {
+            const vecx<T> q = *reinterpret_cast<const vecx<T> *>(xv + xi);
+#pragma unroll
+            for (unsigned k = 0; k < VX; k++) w[k] = q[k];
+          } else {
+#pragma unroll
+            for (unsigned k = 0; k < VX; k++) w[k] = xload(xv, xi + k, ncols);
+          }
+#pragma unroll
+          for (unsigned k = 0; k < VX; k++) {
+            const unsigned pos = xi + k - x0;   /* modulo 2^32, like the indices */
+            if (pos < L) win[b + pos] = w[k];
+          }
+        }
+      }
+    } else {
+      for (unsigned r = 0; r < pat.nrun; r++) {
+        const unsigned x0 = row0 + (unsigned)pat.rrel[r], L = pat.rlen[r], b = pat.rbase[r];
+        for (unsigned i = t; i < L; i += NT) win[b + i] = xload(xv, x0 + i, ncols);
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -477,8 +506,8 @@ __global__ void __launch_bounds__(256) slell_packed_kernel(spmv_args a)
 template<typename T, typename E, int RPT>
 __global__ void __launch_bounds__(256) slell_stream_kernel(spmv_args a)
 {
-  constexpr unsigned ROWS = 256u * RPT, CAP = 2048u * RPT, U = 8;
-  __shared__ T prod[CAP];
+  constexpr unsigned ROWS = 256u * RPT, CAP = 2048u * RPT, U = 4, VEC = dev::evec<E>::n;
+  __shared__ __align__(16) T prod[CAP];
   __shared__ unsigned soff[ROWS + 1];
   if (a.skip && *a.skip) return;
   const unsigned log2S = a.log2S, S = 1u << log2S;
@@ -495,37 +524,55 @@ __global__ void __launch_bounds__(256) slell_stream_kernel(spmv_args a)
   const unsigned ns = min(spg, a.n_slices - s0);
   /* the element range comes from two (scalar) loads, so the element loads need not wait for
    * the offsets table, which only phase 2 reads */
-  const unsigned E0 = a.sp[s0], n = a.sp[s0 + ns] - E0;
+  const unsigned E0 = a.sp[s0], E1 = a.sp[s0 + ns], n = E1 - E0;
   for (unsigned q = t; q <= ns; q += 256) soff[q] = a.sp[s0 + q];   /* ns + 1 <= ROWS + 1 */
+  /* products are stored at prod[e - base], base = E0 rounded down to VEC elements, so the
+   * VEC-aligned body [A0, A1) of the range is loaded and stored as whole vectors; the head
+   * [E0, min(A0, E1)) and tail [A1, E1) (fewer than VEC elements each) one by one */
+  const unsigned base = E0 & ~(VEC - 1);
+  const bool fits = n + (E0 - base) <= CAP;
   T acc[RPT];
 #pragma unroll
   for (int k = 0; k < RPT; k++) acc[k] = 0;
-  if (n <= CAP) {
-    for (unsigned b = t; b < n; b += 256 * U) {
-      T e[U];
-      unsigned c[U];
-#pragma unroll
-      for (unsigned u = 0; u < U; u++) {
-        const unsigned j = b + u * 256;
-        e[u] = 0;
-        c[u] = 0;
-        if (j < n) {
-          e[u] = elt<T, E>::get(el, E0 + j);
-          c[u] = dev::col_load(cl + E0 + j);
-        }
-      }
+  if (fits) {
+    const unsigned A0 = (E0 + VEC - 1) & ~(VEC - 1);
+    const unsigned A1 = max(A0, E1 & ~(VEC - 1));
+    const unsigned hend = min(A0, E1);
+    if (E0 + t < hend) {
+      const unsigned e = E0 + t;
+      prod[e - base] = elt<T, E>::get(el, e) * xload(xv, dev::col_load(cl + e), ncols);
+    }
+    if (A1 + t < E1 && A1 >= hend) {
+      const unsigned e = A1 + t;
+      prod[e - base] = elt<T, E>::get(el, e) * xload(xv, dev::col_load(cl + e), ncols);
+    }
+    const unsigned nv = (A1 - A0) / VEC;
+    for (unsigned v0 = t; v0 < nv; v0 += 256 * U) {
+      T ev[U][VEC];
+      unsigned cv[U][VEC];
 #pragma unroll
       for (unsigned u = 0; u < U; u++)
-        if (b + u * 256 < n) prod[b + u * 256] = e[u] * xload(xv, c[u], ncols);
+        if (v0 + u * 256 < nv) {
+          const unsigned e = A0 + (v0 + u * 256) * VEC;
+          dev::elt_vec<T>(el, e, ev[u]);
+          dev::col_vec(cl, e, cv[u]);
+        }
+#pragma unroll
+      for (unsigned u = 0; u < U; u++)
+        if (v0 + u * 256 < nv) {
+          const unsigned o = A0 - base + (v0 + u * 256) * VEC;
+#pragma unroll
+          for (unsigned q = 0; q < VEC; q++) prod[o + q] = ev[u][q] * xload(xv, cv[u][q], ncols);
+        }
     }
   }
   __syncthreads();   /* offsets table and products */
-  if (n <= CAP) {
+  if (fits) {
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
       const unsigned row = t + k * 256u, i = row >> log2S, r = row & (S - 1);
       if (i < ns) {
-        const unsigned o = soff[i] - E0, nk = (soff[i + 1] - soff[i]) >> log2S;
+        const unsigned o = soff[i] - base, nk = (soff[i + 1] - soff[i]) >> log2S;
         for (unsigned q = 0; q < nk; q++) acc[k] += prod[o + (q << log2S) + r];
       }
     }
@@ -724,8 +771,11 @@ inline bool pow2(unsigned v) { return v && !(v & (v - 1)); }
  * hblock pairs or diagonal planes */
 inline bool use_stream(const ispm_launch *p, const ispm_slell_desc *A)
 {
+  /* the kernel reads elements and columns as 16-byte vectors */
+  const bool aligned = (reinterpret_cast<uintptr_t>(A->elms) % 16) == 0 &&
+                       (reinterpret_cast<uintptr_t>(A->cols) % 16) == 0;
   return p->mode == 2 && !A->var_height && A->slice_height <= 256 && A->slice_height >= 1 &&
-         A->hblock == 1 && !A->diags;
+         A->hblock == 1 && !A->diags && aligned;
 }
 
 /* packed mapping: requested by the launch (mode 1; also mode 2 on layouts the streamed
