@@ -273,7 +273,7 @@ typedef struct ispm_solver_opts {
   int maxit;
   int check_every;        /* iterations per graph replay / host check (default 16) */
   int use_graph;          /* 1: capture batches in a hipGraph */
-  double reliable_delta;  /* mixed: FP32 residual reduction between FP64 replacements (0.1) */
+  double reliable_delta;  /* mixed: FP32 residual reduction between FP64 replacements (0.01) */
 } ispm_solver_opts;
 
 typedef struct ispm_solve_stats {

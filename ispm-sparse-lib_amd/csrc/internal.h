@@ -49,6 +49,16 @@ int spmv_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, 
                   void *y, const unsigned *skip, const diag_pattern *pat = nullptr,
                   const unsigned *uni = nullptr);
 
+/* Partial sums a fused SpMV + dot may store (one per workgroup). */
+constexpr unsigned kMaxDotParts = 32768;
+
+/* spmv_internal that also asks for the dot product (dotv, y): when the kernel chosen can fuse
+ * it, its workgroups store their partial sums at dparts[0 .. *dgrid) and *dgrid is set;
+ * otherwise *dgrid is 0 and the caller computes the dot product itself. */
+int spmv_dot_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, const void *x,
+                      void *y, const unsigned *skip, const diag_pattern *pat, const unsigned *uni,
+                      const void *dotv, double *dparts, unsigned *dgrid);
+
 }  // namespace ispm
 
 #endif

@@ -58,13 +58,14 @@ __device__ __forceinline__ void store_partials(double (&v)[NS], double *parts, d
 constexpr int kFinThreads = 1024;
 static_assert(kMaxBlocks % kFinThreads == 0, "partials per finishing thread");
 
-/* Sum G <= kMaxBlocks partials of each of NS sums (block of kFinThreads threads; lds holds
+/* Sum G <= GMAX partials of each of NS sums (block of kFinThreads threads; lds holds
  * NS * kFinThreads/64 doubles).  All loads are issued before the adds; the order of the adds
  * is fixed, so the result is reproducible run to run.  out[] is valid in every thread. */
-template<int NS>
+template<int NS, unsigned GMAX = kMaxBlocks>
 __device__ __forceinline__ void sum_partials(const double *parts, unsigned G, double *lds, double (&out)[NS])
 {
-  constexpr int K = kMaxBlocks / kFinThreads;
+  static_assert(GMAX % kFinThreads == 0, "partials per finishing thread");
+  constexpr int K = GMAX / kFinThreads;
   double t[NS][K];
 #pragma unroll
   for (int s = 0; s < NS; s++)

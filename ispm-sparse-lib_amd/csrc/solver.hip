@@ -68,14 +68,14 @@ k_dot(const T *__restrict__ x, const T *__restrict__ y, int n, const scalars *s,
 /* Second phase of every solver reduction: one block sums the partials of the kernel before
  * it and applies the scalar part of the recurrence.  A finishing kernel skipped when the
  * solve is done leaves the scalars as they are, like its reduction kernel. */
-template<int OP>
+template<int OP, unsigned GMAX>
 __global__ void __launch_bounds__(kFinThreads) k_fin(const double *parts, int G, scalars *s, int skip_if_done)
 {
   constexpr int NS = OP == OP_PPCG ? 4 : 1;
   __shared__ double lds[NS * kFinThreads / 64];
   if (skip_if_done && s->done) return;
   double out[NS];
-  sum_partials<NS>(parts, G, lds, out);
+  sum_partials<NS, GMAX>(parts, G, lds, out);
   if (threadIdx.x) return;
   const double r = out[0];
   if (OP == OP_PQ) { s->pq = r; s->alpha = s->rho / r; }
@@ -290,9 +290,15 @@ int spmv(ispm_plan *P, int vt, const void *x, void *y, const unsigned *skip)
   return spmv_internal(&P->launch, &P->dev, vt, x, y, skip, P->has_pat ? &P->pat : nullptr, P->dev_uni);
 }
 
-template<int OP> void finish(ispm_solver *S, int skip)
+/* G partials of the reduction before (the solver's grid when 0) */
+template<int OP> void finish(ispm_solver *S, int skip, unsigned G = 0)
 {
-  hipLaunchKernelGGL(k_fin<OP>, dim3(1), dim3(kFinThreads), 0, stream(), (const double *)S->parts, S->grid, S->s, skip);
+  if (G > (unsigned)kMaxBlocks)
+    hipLaunchKernelGGL((k_fin<OP, kMaxDotParts>), dim3(1), dim3(kFinThreads), 0, stream(),
+                       (const double *)S->parts, (int)G, S->s, skip);
+  else
+    hipLaunchKernelGGL((k_fin<OP, kMaxBlocks>), dim3(1), dim3(kFinThreads), 0, stream(),
+                       (const double *)S->parts, G ? (int)G : S->grid, S->s, skip);
 }
 
 template<typename T> int launch_dot(ispm_solver *S, int op, const void *x, const void *y, int skip)
@@ -330,6 +336,23 @@ int apply_M(ispm_solver *S, const void *rin, void *zout, const unsigned *skip)
   return spmv(S->M2, S->T, S->t, zout, skip);
 }
 
+/* y = P x with the partials of (d, y) in S->parts: fused into the SpMV when its kernel can,
+ * else a separate dot kernel.  Returns the number of partials in G. */
+template<typename T>
+int spmv_dot(ispm_solver *S, ispm_plan *P, const void *x, void *y, const void *d,
+             const unsigned *skip, unsigned &G)
+{
+  G = 0;
+  int st = spmv_dot_internal(&P->launch, &P->dev, S->T, x, y, skip, P->has_pat ? &P->pat : nullptr,
+                             P->dev_uni, d, S->parts, &G);
+  if (st || G) return st;
+  hipLaunchKernelGGL((k_dot<T, OP_PQ>), dim3(S->grid), dim3(256), 0, stream(), (const T *)d, (const T *)y,
+                     S->n, S->s, S->parts, skip ? 1 : 0);
+  ISPM_HIP(hipGetLastError());
+  G = (unsigned)S->grid;
+  return ISPM_OK;
+}
+
 const unsigned *done_flag(ispm_solver *S) { return &S->s->done; }
 
 /* one iteration (enqueued; may be captured) */
@@ -350,13 +373,26 @@ int iterate(ispm_solver *S)
     ISPM_HIP(hipGetLastError());
     return ISPM_OK;
   }
-  if ((st = apply_M(S, S->r, S->z, sk))) return st;
-  if ((st = dot(S, OP_RZ, S->r, S->z, 1, S->T))) return st;
+  unsigned G = 0;
+  if (S->M1) {
+    /* z = M2 (M1 r) with (r, z) fused into the second product where possible */
+    if ((st = spmv(S->M1, S->T, S->r, S->t, sk))) return st;
+    st = S->T == ISPM_F64 ? spmv_dot<double>(S, S->M2, S->t, S->z, S->r, sk, G)
+                          : spmv_dot<float>(S, S->M2, S->t, S->z, S->r, sk, G);
+    if (st) return st;
+    finish<OP_RZ>(S, 1, G);
+  } else {
+    if ((st = apply_M(S, S->r, S->z, sk))) return st;
+    if ((st = dot(S, OP_RZ, S->r, S->z, 1, S->T))) return st;
+  }
   if (S->T == ISPM_F64) hipLaunchKernelGGL(k_p<double>, g, b, 0, sm, (double *)S->p, (const double *)S->z, S->n, S->s);
   else hipLaunchKernelGGL(k_p<float>, g, b, 0, sm, (float *)S->p, (const float *)S->z, S->n, S->s);
   ISPM_HIP(hipGetLastError());
-  if ((st = spmv(S->A, S->T, S->p, S->q, sk))) return st;
-  if ((st = dot(S, OP_PQ, S->p, S->q, 1, S->T))) return st;
+  /* q = A p with (p, q) fused where possible */
+  st = S->T == ISPM_F64 ? spmv_dot<double>(S, S->A, S->p, S->q, S->p, sk, G)
+                        : spmv_dot<float>(S, S->A, S->p, S->q, S->p, sk, G);
+  if (st) return st;
+  finish<OP_PQ>(S, 1, G);
   if (S->T == ISPM_F64)
     hipLaunchKernelGGL(k_xr<double>, g, b, 0, sm, (double *)S->xs, (double *)S->r, (const double *)S->p,
                        (const double *)S->q, S->n, S->s, S->parts);
@@ -450,7 +486,7 @@ int ispm_solver_create(ispm_solver **out, ispm_plan *A, ispm_plan *M1, ispm_plan
   S->o = *o;
   if (S->o.check_every <= 0) S->o.check_every = 16;
   if (S->o.maxit <= 0) S->o.maxit = 100000;
-  if (S->o.reliable_delta <= 0 || S->o.reliable_delta >= 1) S->o.reliable_delta = 0.1;
+  if (S->o.reliable_delta <= 0 || S->o.reliable_delta >= 1) S->o.reliable_delta = 0.01;
   /* ISPM_NO_GRAPH=1 launches iterations eagerly (profilers that cannot follow graph capture) */
   if (const char *ng = std::getenv("ISPM_NO_GRAPH"))
     if (ng[0] == '1') S->o.use_graph = 0;
@@ -473,7 +509,7 @@ int ispm_solver_create(ispm_solver **out, ispm_plan *A, ispm_plan *M1, ispm_plan
     for (void **v : pv) if ((st = alloc(v, vb))) { ispm_solver_destroy(S); return st; }
   }
   if ((st = alloc((void **)&S->r64, 8 * (size_t)S->n + 64)) || (st = alloc((void **)&S->tmp64, 8 * (size_t)S->n + 64)) ||
-      (st = alloc((void **)&S->s, sizeof(scalars))) || (st = alloc((void **)&S->parts, sizeof(double) * kMaxBlocks * 4))) {
+      (st = alloc((void **)&S->s, sizeof(scalars))) || (st = alloc((void **)&S->parts, sizeof(double) * std::max<size_t>(kMaxBlocks * 4, kMaxDotParts)))) {
     ispm_solver_destroy(S);
     return st;
   }

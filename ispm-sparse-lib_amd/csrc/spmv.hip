@@ -23,6 +23,21 @@ int spmv_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, 
   return spmvk::launch_types<1>(p, A, vec_type, x, 0, y, 0, H, skip);
 }
 
+int spmv_dot_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, const void *x,
+                      void *y, const unsigned *skip, const diag_pattern *pat, const unsigned *uni,
+                      const void *dotv, double *dparts, unsigned *dgrid)
+{
+  *dgrid = 0;
+  unsigned H;
+  int st = spmvk::check_args(p, A, vec_type, x, y, H);
+  if (st) return st;
+  if (A->n_slices == 0) return ISPM_OK;
+  if (spmvk::use_diagpat(p, A, vec_type, H, pat, uni))
+    return spmvk::launch_diagpat_types(A, vec_type, *pat, uni, x, y, skip, dotv, dparts, dgrid);
+  if (spmvk::use_stream(p, A)) return spmvk::launch_stream_types(A, vec_type, x, y, skip, dotv, dparts, dgrid);
+  return spmv_internal(p, A, vec_type, x, y, skip);
+}
+
 /* SpMM for the block solver: same checks as ispm_spmm */
 int spmm_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, int k,
                   const void *X, size_t ldx, void *Y, size_t ldy)

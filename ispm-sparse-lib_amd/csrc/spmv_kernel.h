@@ -53,6 +53,8 @@ struct spmv_args {
   unsigned per_xcd;      /* workgroups per XCD range */
   unsigned spb;          /* consecutive slices each workgroup processes */
   const unsigned *skip;  /* if non-null and *skip != 0 the launch does nothing (solver done flag) */
+  const void *dotv = nullptr;   /* fused dot: workgroup w stores sum_rows dotv[row]*y[row] */
+  double *dparts = nullptr;     /* at dparts[w] (kernels that support it) */
 };
 
 template<typename T> __device__ __forceinline__ T xload(const T *x, unsigned i, unsigned n)
@@ -69,6 +71,17 @@ template<> struct vstore<double, 2> {
 template<> struct vstore<float, 2> {
   static __device__ __forceinline__ void st(float *y, const float (&v)[2])
   { *reinterpret_cast<float2 *>(y) = make_float2(v[0], v[1]); }
+};
+template<> struct vstore<float, 4> {
+  static __device__ __forceinline__ void st(float *y, const float (&v)[4])
+  { *reinterpret_cast<float4 *>(y) = make_float4(v[0], v[1], v[2], v[3]); }
+};
+template<> struct vstore<double, 4> {
+  static __device__ __forceinline__ void st(double *y, const double (&v)[4])
+  {
+    reinterpret_cast<double2 *>(y)[0] = make_double2(v[0], v[1]);
+    reinterpret_cast<double2 *>(y)[1] = make_double2(v[2], v[3]);
+  }
 };
 template<typename T> struct vstore<T, 1> {
   static __device__ __forceinline__ void st(T *y, const T (&v)[1]) { *y = v[0]; }
@@ -255,6 +268,26 @@ __global__ void __launch_bounds__(1024) slell_kernel(spmv_args a)
  * are identical to it. */
 template<typename T> using vecx = T __attribute__((ext_vector_type(16 / sizeof(T))));
 
+/* H consecutive elements (aligned to H) of one plane, converted to T */
+template<typename T, typename E, int H> struct rows_load;
+template<typename T, typename E> struct rows_load<T, E, 1> {
+  static __device__ __forceinline__ void get(const E *p, unsigned i, T (&v)[1]) { v[0] = elt<T, E>::get(p, i); }
+};
+template<typename T, typename E> struct rows_load<T, E, 2> {
+  static __device__ __forceinline__ void get(const E *p, unsigned i, T (&v)[2]) { elt<T, E>::get2(p, i, v[0], v[1]); }
+};
+template<typename T, typename E> struct rows_load<T, E, 4> {
+  static __device__ __forceinline__ void get(const E *p, unsigned i, T (&v)[4])
+  {
+    if constexpr (sizeof(E) == 8) {
+      elt<T, E>::get2(p, i, v[0], v[1]);
+      elt<T, E>::get2(p, i + 2, v[2], v[3]);
+    } else {
+      dev::elt_vec<T>(p, i, v);
+    }
+  }
+};
+
 template<typename T, typename E, unsigned LOG2S, int H>
 __global__ void __launch_bounds__((1u << LOG2S) / H)
 slell_diagpat_kernel(spmv_args a, diag_pattern pat, const unsigned *__restrict__ uni)
@@ -265,8 +298,11 @@ slell_diagpat_kernel(spmv_args a, diag_pattern pat, const unsigned *__restrict__
   const unsigned t = threadIdx.x, r0 = t * H;   /* this thread's first row in the slice */
   /* one slice per workgroup */
   const unsigned slice = (blockIdx.x & 7u) * a.per_xcd + (blockIdx.x >> 3);
-  if (slice >= a.n_slices) return;
   if (a.skip && *a.skip) return;
+  if (slice >= a.n_slices) {
+    if (a.dparts && threadIdx.x == 0) a.dparts[blockIdx.x] = 0.0;
+    return;
+  }
   const E *__restrict__ el = static_cast<const E *>(a.elms);
   const unsigned *__restrict__ sp = a.sp;
   const T *__restrict__ xv = static_cast<const T *>(a.x);
@@ -285,8 +321,7 @@ slell_diagpat_kernel(spmv_args a, diag_pattern pat, const unsigned *__restrict__
 #pragma unroll
     for (unsigned j = 0; j < kPatPlanes; j++)
       if (j < np) {
-        if (H == 2) elt<T, E>::get2(ep, j << LOG2S, ev[j][0], ev[j][H - 1]);
-        else ev[j][0] = elt<T, E>::get(ep, j << LOG2S);
+        rows_load<T, E, H>::get(ep, j << LOG2S, ev[j]);
       }
     if (xvec) {
       /* 16-byte loads from the aligned cover of each window; entries outside the window
@@ -348,17 +383,31 @@ slell_diagpat_kernel(spmv_args a, diag_pattern pat, const unsigned *__restrict__
     }
   }
   vstore<T, H>::st(yv + row0 + r0, acc);
+  if (a.dparts) {
+    __shared__ double red[NT / 64];
+    const T *__restrict__ dv = static_cast<const T *>(a.dotv);
+    double d[1] = {0.0};
+#pragma unroll
+    for (int h = 0; h < H; h++) d[0] += (double)dv[row0 + r0 + h] * (double)acc[h];
+    dev::block_sum<NT, 1>(d, red);
+    if (t == 0) a.dparts[blockIdx.x] = d[0];
+  }
 }
 
 template<typename T, typename E>
 int launch_diagpat(const ispm_slell_desc *A, const diag_pattern &pat, const unsigned *uni,
-                   const void *x, void *y, const unsigned *skip)
+                   const void *x, void *y, const unsigned *skip, const void *dotv = nullptr,
+                   double *dparts = nullptr, unsigned *dgrid = nullptr)
 {
   /* Rows per thread is the kernel's choice (each row's sum is the same either way): two for
    * FP32 vectors (half the load instructions; 205 -> 184 us on a 27-point 192^3 stencil),
    * one for FP64, where two rows per thread run no faster. */
-  const unsigned H = (sizeof(T) == 4 && A->slice_height >= 128 &&
-                      (reinterpret_cast<uintptr_t>(y) % (2 * sizeof(T))) == 0) ? 2u : 1u;
+  unsigned H = (sizeof(T) == 4 && A->slice_height >= 128) ? 2u : 1u;
+  if (const char *e = std::getenv("ISPM_DIAGPAT_H")) H = (unsigned)std::atoi(e);
+  if ((H != 1 && H != 2 && H != 4) || A->slice_height / H < 64 ||
+      (reinterpret_cast<uintptr_t>(y) % (H * sizeof(T))) != 0 ||
+      (reinterpret_cast<uintptr_t>(A->elms) % 16) != 0)
+    H = 1;
   spmv_args a;
   a.skip = skip;
   a.sp = A->slice_ptr;
@@ -372,6 +421,11 @@ int launch_diagpat(const ispm_slell_desc *A, const diag_pattern &pat, const unsi
   a.n_cols = (unsigned)A->n_cols;
   a.spb = 1;
   a.per_xcd = (A->n_slices + 7) / 8;
+  if (dparts && dgrid && a.per_xcd * 8 <= kMaxDotParts) {
+    a.dotv = dotv;
+    a.dparts = dparts;
+    *dgrid = a.per_xcd * 8;
+  }
   const size_t lds = (size_t)pat.lds_len * sizeof(T);
   if (lds > 65536) return fail(ISPM_ERR_UNSUPPORTED, "spmv: pattern windows need %zu bytes of LDS", lds);
   const dim3 g(a.per_xcd * 8), b(A->slice_height / H);
@@ -380,9 +434,9 @@ int launch_diagpat(const ispm_slell_desc *A, const diag_pattern &pat, const unsi
     case 64 * 4 + 1: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 6, 1>), g, b, lds, st, a, pat, uni); break;
     case 128 * 4 + 1: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 7, 1>), g, b, lds, st, a, pat, uni); break;
     case 256 * 4 + 1: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 8, 1>), g, b, lds, st, a, pat, uni); break;
-    case 64 * 4 + 2: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 6, 2>), g, b, lds, st, a, pat, uni); break;
     case 128 * 4 + 2: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 7, 2>), g, b, lds, st, a, pat, uni); break;
     case 256 * 4 + 2: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 8, 2>), g, b, lds, st, a, pat, uni); break;
+    case 256 * 4 + 4: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 8, 4>), g, b, lds, st, a, pat, uni); break;
     default: return fail(ISPM_ERR_UNSUPPORTED, "spmv: pattern kernel for slice height %u", A->slice_height);
   }
   ISPM_HIP(hipGetLastError());
@@ -399,13 +453,15 @@ inline bool use_diagpat(const ispm_launch *p, const ispm_slell_desc *A, int vt, 
 }
 
 inline int launch_diagpat_types(const ispm_slell_desc *A, int vt, const diag_pattern &pat,
-                                const unsigned *uni, const void *x, void *y, const unsigned *skip)
+                                const unsigned *uni, const void *x, void *y, const unsigned *skip,
+                                const void *dotv = nullptr, double *dparts = nullptr,
+                                unsigned *dgrid = nullptr)
 {
   const int et = A->elm_type;
-  if (vt == ISPM_F64 && et == ISPM_F64) return launch_diagpat<double, double>(A, pat, uni, x, y, skip);
-  if (vt == ISPM_F64 && et == ISPM_F32) return launch_diagpat<double, float>(A, pat, uni, x, y, skip);
-  if (vt == ISPM_F32 && et == ISPM_F32) return launch_diagpat<float, float>(A, pat, uni, x, y, skip);
-  if (vt == ISPM_F32 && et == ISPM_BF16) return launch_diagpat<float, bf16_t>(A, pat, uni, x, y, skip);
+  if (vt == ISPM_F64 && et == ISPM_F64) return launch_diagpat<double, double>(A, pat, uni, x, y, skip, dotv, dparts, dgrid);
+  if (vt == ISPM_F64 && et == ISPM_F32) return launch_diagpat<double, float>(A, pat, uni, x, y, skip, dotv, dparts, dgrid);
+  if (vt == ISPM_F32 && et == ISPM_F32) return launch_diagpat<float, float>(A, pat, uni, x, y, skip, dotv, dparts, dgrid);
+  if (vt == ISPM_F32 && et == ISPM_BF16) return launch_diagpat<float, bf16_t>(A, pat, uni, x, y, skip, dotv, dparts, dgrid);
   return fail(ISPM_ERR_UNSUPPORTED, "spmv: unsupported (vector, element) types (%d, %d)", vt, et);
 }
 
@@ -520,7 +576,10 @@ __global__ void __launch_bounds__(256) slell_stream_kernel(spmv_args a)
   const unsigned ncols = a.n_cols;
   const unsigned group = (blockIdx.x & 7u) * a.per_xcd + (blockIdx.x >> 3);
   const unsigned s0 = group * spg;
-  if (s0 >= a.n_slices) return;   /* uniform over the workgroup */
+  if (s0 >= a.n_slices) {   /* uniform over the workgroup */
+    if (a.dparts && t == 0) a.dparts[blockIdx.x] = 0.0;
+    return;
+  }
   const unsigned ns = min(spg, a.n_slices - s0);
   /* the element range comes from two (scalar) loads, so the element loads need not wait for
    * the offsets table, which only phase 2 reads */
@@ -594,10 +653,23 @@ __global__ void __launch_bounds__(256) slell_stream_kernel(spmv_args a)
     const unsigned row = t + k * 256u;
     if ((row >> log2S) < ns) yv[(s0 << log2S) + row] = acc[k];
   }
+  if (a.dparts) {
+    __shared__ double red[4];
+    const T *__restrict__ dv = static_cast<const T *>(a.dotv);
+    double d[1] = {0.0};
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+      const unsigned row = t + k * 256u;
+      if ((row >> log2S) < ns) d[0] += (double)dv[(s0 << log2S) + row] * (double)acc[k];
+    }
+    dev::block_sum<256, 1>(d, red);
+    if (t == 0) a.dparts[blockIdx.x] = d[0];
+  }
 }
 
 template<typename T, typename E>
-int launch_stream(const ispm_slell_desc *A, const void *x, void *y, const unsigned *skip)
+int launch_stream(const ispm_slell_desc *A, const void *x, void *y, const unsigned *skip,
+                  const void *dotv = nullptr, double *dparts = nullptr, unsigned *dgrid = nullptr)
 {
   spmv_args a;
   a.skip = skip;
@@ -619,6 +691,11 @@ int launch_stream(const ispm_slell_desc *A, const void *x, void *y, const unsign
   const unsigned spg = (256u * rpt) >> a.log2S;
   const unsigned groups = (A->n_slices + spg - 1) / spg;
   a.per_xcd = (groups + 7) / 8;
+  if (dparts && dgrid && a.per_xcd * 8 <= kMaxDotParts) {
+    a.dotv = dotv;
+    a.dparts = dparts;
+    *dgrid = a.per_xcd * 8;
+  }
   const dim3 g(a.per_xcd * 8), b(256);
   if (rpt == 1) hipLaunchKernelGGL((slell_stream_kernel<T, E, 1>), g, b, 0, stream(), a);
   else if (rpt == 2) hipLaunchKernelGGL((slell_stream_kernel<T, E, 2>), g, b, 0, stream(), a);
@@ -628,13 +705,14 @@ int launch_stream(const ispm_slell_desc *A, const void *x, void *y, const unsign
 }
 
 inline int launch_stream_types(const ispm_slell_desc *A, int vt, const void *x, void *y,
-                               const unsigned *skip)
+                               const unsigned *skip, const void *dotv = nullptr,
+                               double *dparts = nullptr, unsigned *dgrid = nullptr)
 {
   const int et = A->elm_type;
-  if (vt == ISPM_F64 && et == ISPM_F64) return launch_stream<double, double>(A, x, y, skip);
-  if (vt == ISPM_F64 && et == ISPM_F32) return launch_stream<double, float>(A, x, y, skip);
-  if (vt == ISPM_F32 && et == ISPM_F32) return launch_stream<float, float>(A, x, y, skip);
-  if (vt == ISPM_F32 && et == ISPM_BF16) return launch_stream<float, bf16_t>(A, x, y, skip);
+  if (vt == ISPM_F64 && et == ISPM_F64) return launch_stream<double, double>(A, x, y, skip, dotv, dparts, dgrid);
+  if (vt == ISPM_F64 && et == ISPM_F32) return launch_stream<double, float>(A, x, y, skip, dotv, dparts, dgrid);
+  if (vt == ISPM_F32 && et == ISPM_F32) return launch_stream<float, float>(A, x, y, skip, dotv, dparts, dgrid);
+  if (vt == ISPM_F32 && et == ISPM_BF16) return launch_stream<float, bf16_t>(A, x, y, skip, dotv, dparts, dgrid);
   return fail(ISPM_ERR_UNSUPPORTED, "spmv: unsupported (vector, element) types (%d, %d)", vt, et);
 }
 

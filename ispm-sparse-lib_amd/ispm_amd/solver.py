@@ -50,7 +50,7 @@ class Solver:
     vectors, FP64 scalars, reliable FP64 residual replacement."""
 
     def __init__(self, A: CSR, M=None, precision="fp64", method=None, tol=1e-10, maxit=100000,
-                 check_every=16, use_graph=True, reliable_delta=0.1, factor_type=None,
+                 check_every=16, use_graph=True, reliable_delta=0.01, factor_type=None,
                  A_plan=None, M_plans=None):
         L = _bind()
         mixed = precision == "mixed"
