@@ -120,7 +120,10 @@ def solve_leg(I, S, O, A, b, factors, precision, method, steps, warmup, tol):
     x = xd.numpy()[:A.n_rows]
     r = b - O.csr_spmv(A, x)
     rel = float(np.linalg.norm(r) / np.linalg.norm(b))
+    pre = solver.M_plans or ()
     return {"precision": precision, "method": {0: "pipelined-pcg", 1: "pcg"}[method],
+            "preconditioner": ("product Ainv2*Ainv1" if len(pre) == 1 else "factors Ainv1, Ainv2"),
+            "preconditioner_nnz_per_row": sum(p.info().device.n_nz for p in pre) / A.n_rows,
             "seconds": float(np.mean(times)), "seconds_min": float(np.min(times)),
             "iterations": stats.iterations, "replacements": stats.replacements,
             "converged": bool(stats.converged), "gpu_relres": stats.relres,

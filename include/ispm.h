@@ -286,8 +286,10 @@ typedef struct ispm_solver ispm_solver;
 
 /* A: plan of the system matrix (E = F64 for FP64, E = F32 for mixed; the mixed solver uses
  * it both with FP32 and FP64 vectors, so its values must be those of A).  Ainv1 / Ainv2:
- * plans of the factors (T = F64 for FP64, T = F32 for mixed), or both NULL for no
- * preconditioner.  Plans must outlive the solver. */
+ * plans of the factors (T = F64 for FP64, T = F32 for mixed) applied as z = Ainv2 (Ainv1 r);
+ * Ainv1 alone is a one-matrix preconditioner z = Ainv1 r (for instance the product
+ * Ainv2 * Ainv1 formed by ispm_csr_multiply); both NULL: no preconditioner.  Plans must
+ * outlive the solver. */
 int ispm_solver_create(ispm_solver **out, ispm_plan *A, ispm_plan *Ainv1, ispm_plan *Ainv2,
                        const ispm_solver_opts *opts);
 /* Mixed precision evaluates the FP64 true residual with A's plan read into FP64
@@ -356,6 +358,12 @@ typedef void (*ispm_progress_fn)(void *);
 int ispm_fastainv_sym(int out_type, const ispm_csr *A, ispm_progress_fn progress, void *v,
                       long long max_elts, double droptol,
                       ispm_csr_out *Ainv1, ispm_csr_out *Ainv2);
+
+/* No reference counterpart (its solver applies the two factors in turn).  C = A * B on the
+ * host with FP64 accumulation, output type ISPM_F64 or ISPM_F32, sorted columns; forms the
+ * one-matrix preconditioner Ainv2 * Ainv1 when that has fewer nonzeros than the two factors
+ * together.  Output is library-owned (ispm_csr_out_free). */
+int ispm_csr_multiply(const ispm_csr *A, const ispm_csr *B, int out_type, ispm_csr_out *C);
 void ispm_csr_out_free(ispm_csr_out *m);
 
 #ifdef __cplusplus

@@ -176,8 +176,12 @@ int iterate(ispm_block_solver *S, double *X, size_t ldx)
   dim3 g(S->grid, k), b(256);
   hipStream_t sm = stream();
   if (S->M1) {
-    if ((st = mspmm(S->M1, k, S->R, ld, S->T, ld))) return st;
-    if ((st = mspmm(S->M2, k, S->T, ld, S->Z, ld))) return st;
+    if (S->M2) {
+      if ((st = mspmm(S->M1, k, S->R, ld, S->T, ld))) return st;
+      if ((st = mspmm(S->M2, k, S->T, ld, S->Z, ld))) return st;
+    } else if ((st = mspmm(S->M1, k, S->R, ld, S->Z, ld))) {
+      return st;
+    }
   } else {
     hipLaunchKernelGGL(k_mcopy, g, b, 0, sm, S->Z, (const double *)S->R, ld, n);
   }
@@ -218,10 +222,11 @@ int ispm_block_solver_create(ispm_block_solver **out, ispm_plan *A, ispm_plan *M
   int st = ensure_init();
   if (st) return st;
   if (k < 1 || k > kMaxK) return fail(ISPM_ERR_UNSUPPORTED, "block_solver_create: k=%d outside 1..8", k);
-  if ((M1 == nullptr) != (M2 == nullptr)) return fail(ISPM_ERR_INVALID, "block_solver_create: give both factors or none");
+  if (!M1 && M2) return fail(ISPM_ERR_INVALID, "block_solver_create: second factor without the first");
   if (!A->has_dev || A->T != ISPM_F64) return fail(ISPM_ERR_INVALID, "block_solver_create: A needs an FP64-vector device plan");
-  if (M1 && (!M1->has_dev || !M2->has_dev || M1->T != ISPM_F64 || M2->T != ISPM_F64))
-    return fail(ISPM_ERR_INVALID, "block_solver_create: factor plans need FP64 vectors on the device");
+  for (const ispm_plan *M : {M1, M2})
+    if (M && (!M->has_dev || M->T != ISPM_F64 || M->L.n_rows != A->L.n_rows))
+      return fail(ISPM_ERR_INVALID, "block_solver_create: preconditioner plans must be FP64 device plans of A's size");
   ispm_block_solver *S = new ispm_block_solver;
   S->A = A; S->M1 = M1; S->M2 = M2; S->o = *o; S->k = k;
   if (S->o.check_every <= 0) S->o.check_every = 16;

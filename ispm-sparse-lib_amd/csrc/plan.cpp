@@ -357,8 +357,7 @@ int estimate_format(const ispm_csr *A, int vt, int et, unsigned &bS, unsigned &b
   conv::guess_stats(A->row_ptr, A->n_rows, A->cols, (int)tsize(et), 1, g);
   if (g.diags || g.hblock > 1) {
     conv::layout L;
-    if (!conv::convert(A->row_ptr, A->n_rows, A->cols, A->n_cols, A->n_nz, (int)tsize(et), (int)bS,
-                       g.hblock, bV != 0, g.diags, 4, L)) {
+    if (!make_layout(A, et, bS, (unsigned)g.hblock, bV, g.diags ? 1 : 0, L)) {
       const double bytes = 4.0 * L.sp.size() + 4.0 * L.cols.size() + es * L.n_elms;
       if (bytes < best) { best = bytes; bW = g.hblock; bD = g.diags; }
     }

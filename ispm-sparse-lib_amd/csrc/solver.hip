@@ -332,6 +332,7 @@ int apply_M(ispm_solver *S, const void *rin, void *zout, const unsigned *skip)
     ISPM_HIP(hipGetLastError());
     return ISPM_OK;
   }
+  if (!S->M2) return spmv(S->M1, S->T, rin, zout, skip);
   if ((st = spmv(S->M1, S->T, rin, S->t, skip))) return st;
   return spmv(S->M2, S->T, S->t, zout, skip);
 }
@@ -375,10 +376,16 @@ int iterate(ispm_solver *S)
   }
   unsigned G = 0;
   if (S->M1) {
-    /* z = M2 (M1 r) with (r, z) fused into the second product where possible */
-    if ((st = spmv(S->M1, S->T, S->r, S->t, sk))) return st;
-    st = S->T == ISPM_F64 ? spmv_dot<double>(S, S->M2, S->t, S->z, S->r, sk, G)
-                          : spmv_dot<float>(S, S->M2, S->t, S->z, S->r, sk, G);
+    /* z = M2 (M1 r), or M1 r for a one-matrix preconditioner, with (r, z) fused into the
+     * last product where possible */
+    ispm_plan *last = S->M2 ? S->M2 : S->M1;
+    const void *in = S->r;
+    if (S->M2) {
+      if ((st = spmv(S->M1, S->T, S->r, S->t, sk))) return st;
+      in = S->t;
+    }
+    st = S->T == ISPM_F64 ? spmv_dot<double>(S, last, in, S->z, S->r, sk, G)
+                          : spmv_dot<float>(S, last, in, S->z, S->r, sk, G);
     if (st) return st;
     finish<OP_RZ>(S, 1, G);
   } else {
@@ -470,17 +477,19 @@ int ispm_solver_create(ispm_solver **out, ispm_plan *A, ispm_plan *M1, ispm_plan
   *out = nullptr;
   int st = ensure_init();
   if (st) return st;
-  if ((M1 == nullptr) != (M2 == nullptr)) return fail(ISPM_ERR_INVALID, "solver_create: give both factors or none");
+  if (!M1 && M2) return fail(ISPM_ERR_INVALID, "solver_create: second factor without the first");
   if (!A->has_dev) return fail(ISPM_ERR_INVALID, "solver_create: A plan has no device matrix");
   const bool mixed = o->precision == ISPM_PREC_MIXED;
   if (mixed && o->method != ISPM_SOLVER_PCG) return fail(ISPM_ERR_UNSUPPORTED, "solver_create: mixed precision uses method PCG");
   if (!mixed && A->E != ISPM_F64) return fail(ISPM_ERR_INVALID, "solver_create: FP64 solve needs an FP64 A plan");
   if (mixed && A->E != ISPM_F32) return fail(ISPM_ERR_INVALID, "solver_create: mixed solve needs an FP32 A plan");
   const int T = mixed ? ISPM_F32 : ISPM_F64;
-  if (M1 && (!M1->has_dev || !M2->has_dev || M1->T != T || M2->T != T))
-    return fail(ISPM_ERR_INVALID, "solver_create: factor plans must be on the device with vector type %s", mixed ? "F32" : "F64");
-  if (M1 && (M1->L.n_rows != A->L.n_rows || M2->L.n_rows != A->L.n_rows))
-    return fail(ISPM_ERR_INVALID, "solver_create: factor sizes differ from A");
+  for (const ispm_plan *M : {M1, M2}) {
+    if (!M) continue;
+    if (!M->has_dev || M->T != T)
+      return fail(ISPM_ERR_INVALID, "solver_create: preconditioner plans must be on the device with vector type %s", mixed ? "F32" : "F64");
+    if (M->L.n_rows != A->L.n_rows) return fail(ISPM_ERR_INVALID, "solver_create: preconditioner size differs from A");
+  }
   ispm_solver *S = new ispm_solver;
   S->A = A; S->M1 = M1; S->M2 = M2;
   S->o = *o;

@@ -562,7 +562,8 @@ __global__ void __launch_bounds__(256) slell_packed_kernel(spmv_args a)
 template<typename T, typename E, int RPT>
 __global__ void __launch_bounds__(256) slell_stream_kernel(spmv_args a)
 {
-  constexpr unsigned ROWS = 256u * RPT, CAP = 2048u * RPT, U = 4, VEC = dev::evec<E>::n;
+  constexpr unsigned ROWS = 256u * RPT, CAP = (sizeof(T) == 4 ? 1024u : 2048u) * RPT, U = 4,
+                     VEC = dev::evec<E>::n;
   __shared__ __align__(16) T prod[CAP];
   __shared__ unsigned soff[ROWS + 1];
   if (a.skip && *a.skip) return;

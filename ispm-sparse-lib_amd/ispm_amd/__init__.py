@@ -135,6 +135,7 @@ _SIGS = {
     "ispm_fastainv_sym": (C.c_int, [C.c_int, C.POINTER(Csr), PROGRESS_FN, C.c_void_p,
                                     C.c_longlong, C.c_double, C.POINTER(CsrOut), C.POINTER(CsrOut)]),
     "ispm_csr_out_free": (None, [C.POINTER(CsrOut)]),
+    "ispm_csr_multiply": (C.c_int, [C.POINTER(Csr), C.POINTER(Csr), C.c_int, C.POINTER(CsrOut)]),
     "ispm_mm_read": (C.c_int, [C.c_char_p, C.POINTER(CsrOut)]),
     "ispm_mm_write": (C.c_int, [C.c_char_p, C.POINTER(Csr)]),
     "ispm_csr_fingerprint": (C.c_int, [C.POINTER(Csr), C.POINTER(C.c_ulonglong)]),
@@ -484,6 +485,16 @@ def fastainv_sym(A: CSR, out_type=F32, max_elts=8192000, droptol=0.125, progress
     finally:
         lib().ispm_csr_out_free(C.byref(a1))
         lib().ispm_csr_out_free(C.byref(a2))
+
+
+def csr_multiply(A: CSR, B: CSR, out_type=F64) -> CSR:
+    """C = A * B on the host (FP64 accumulation, sorted columns)."""
+    o = CsrOut()
+    check(lib().ispm_csr_multiply(C.byref(A.c()), C.byref(B.c()), out_type, C.byref(o)))
+    try:
+        return _csr_from_out(o)
+    finally:
+        lib().ispm_csr_out_free(C.byref(o))
 
 
 # ------------------------------------------------------------------ on-disk ----------

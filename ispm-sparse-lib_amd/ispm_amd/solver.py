@@ -6,7 +6,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import (BF16, CSR, F32, F64, PLAN_ESTIMATE, DeviceArray, Plan, check, lib)
+from . import (BF16, CSR, F32, F64, PLAN_ESTIMATE, DeviceArray, Plan, check, csr_multiply, lib)
 
 PPCG, PCG = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
@@ -42,16 +42,56 @@ def _bind():
     return L
 
 
-class Solver:
-    """Owns the plans of A and of the factors and a device solver object.
+def preconditioner_matrices(M, product=None):
+    """The matrices the solver applies for M: (Ainv1, Ainv2) as given, or the one matrix
+    Ainv2 * Ainv1 when `product` is True, or (None) when it has fewer than 90% of the two
+    factors' nonzeros together (one sparse product streams fewer bytes than two)."""
+    if M is None or isinstance(M, CSR):
+        return M if M is None else (M,)
+    a1, a2 = M
+    if product is False:
+        return (a1, a2)
+    P = csr_multiply(a2, a1, F64)
+    if product or P.nnz < 0.9 * (a1.nnz + a2.nnz):
+        return (P,)
+    return (a1, a2)
 
-    precision="fp64": A in FP64, factors in FP64 (or the element type of the given factor
-    CSRs), FP64 vectors.  precision="mixed": A in FP32, factors in BF16 (factor_type), FP32
-    vectors, FP64 scalars, reliable FP64 residual replacement."""
+
+def apply_seconds(plans, vt, n, reps=9):
+    """Median time of applying the preconditioner plans in turn (device events)."""
+    from . import Event, synchronize
+    dt = np.float64 if vt == F64 else np.float32
+    bufs = [DeviceArray.from_numpy(np.ones(n, dt)), DeviceArray(n, dt), DeviceArray(n, dt)]
+
+    def run():
+        src = 0
+        for k, p in enumerate(plans):
+            dst = 1 + (k & 1)
+            p.execute(bufs[src], bufs[dst])
+            src = dst
+    run()
+    synchronize()
+    e0, e1, ts = Event(), Event(), []
+    for _ in range(reps):
+        e0.record()
+        run()
+        e1.record()
+        e1.synchronize()
+        ts.append(e0.elapsed_ms(e1))
+    return float(np.median(ts))
+
+
+class Solver:
+    """Owns the plans of A and of the preconditioner and a device solver object.
+
+    precision="fp64": A in FP64, preconditioner in FP64, FP64 vectors.  precision="mixed": A
+    in FP32, preconditioner in BF16 (factor_type), FP32 vectors, FP64 scalars, reliable FP64
+    residual replacement.  M: (Ainv1, Ainv2) from fastainv_sym, applied as Ainv2 (Ainv1 r) or
+    as their product (see preconditioner_matrices; fsai_product chooses), or one CSR matrix."""
 
     def __init__(self, A: CSR, M=None, precision="fp64", method=None, tol=1e-10, maxit=100000,
                  check_every=16, use_graph=True, reliable_delta=0.01, factor_type=None,
-                 A_plan=None, M_plans=None):
+                 A_plan=None, M_plans=None, fsai_product=None):
         L = _bind()
         mixed = precision == "mixed"
         vt = F32 if mixed else F64
@@ -63,15 +103,28 @@ class Solver:
             self.M_plans = M_plans
         elif M is not None:
             ft = factor_type if factor_type is not None else (BF16 if mixed else F64)
-            # factor rows are irregular: take the byte-minimal format
-            self.M_plans = (Plan(M[0], vt, elm_type=ft, kind=PLAN_ESTIMATE),
-                            Plan(M[1], vt, elm_type=ft, kind=PLAN_ESTIMATE))
+            # preconditioner rows are irregular: take the byte-minimal format
+            def plans(mats):
+                return tuple(Plan(m, vt, elm_type=ft, kind=PLAN_ESTIMATE) for m in mats)
+            if fsai_product is None and not isinstance(M, CSR):
+                # two factors or their product: whichever applies faster here (the product
+                # has fewer nonzeros but its off-diagonal entries gather x from further away)
+                a1, a2 = M
+                P = csr_multiply(a2, a1, F64)
+                self.M_plans = plans((a1, a2))
+                if P.nnz < a1.nnz + a2.nnz:
+                    cand = plans((P,))
+                    n = self.A_plan.n_rows_padded
+                    if apply_seconds(cand, vt, n) < apply_seconds(self.M_plans, vt, n):
+                        self.M_plans = cand
+            else:
+                self.M_plans = plans(preconditioner_matrices(M, fsai_product))
         self.n = self.A_plan.n_rows_padded
         o = SolverOpts(method, PREC_MIXED if mixed else PREC_FP64, tol, maxit, check_every,
                        int(use_graph), reliable_delta)
         p = C.c_void_p()
         m1 = self.M_plans[0].p if self.M_plans else None
-        m2 = self.M_plans[1].p if self.M_plans else None
+        m2 = self.M_plans[1].p if self.M_plans and len(self.M_plans) > 1 else None
         check(L.ispm_solver_create(C.byref(p), self.A_plan.p, m1, m2, C.byref(o)))
         self.p = p
         self.res_plan = None

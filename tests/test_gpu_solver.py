@@ -18,13 +18,15 @@ def oracle_relres(A, b, x):
     return float(np.linalg.norm(b - O.csr_spmv(A, x)) / np.linalg.norm(b))
 
 
+@pytest.mark.parametrize("product", [False, True], ids=["factors", "product"])
 @pytest.mark.parametrize("prec,method", VARIANTS)
 @pytest.mark.parametrize("tol", [1e-8, 1e-10])
-def test_poisson_fsai(gpu, prec, method, tol):
+def test_poisson_fsai(gpu, prec, method, tol, product):
+    """Preconditioner applied as its two factors or as their product Ainv2 * Ainv1."""
     A = G.poisson7(24)
     b = O.csr_spmv(A, G.uniform(1, A.n_rows))
     M = I.fastainv_sym(A, I.F64, droptol=0.05)
-    res = S.solve(A, b, M=M, tol=tol, precision=prec, method=method)
+    res = S.solve(A, b, M=M, tol=tol, precision=prec, method=method, fsai_product=product)
     assert res.converged
     assert oracle_relres(A, b, res.x) <= tol
     # iteration count close to the oracle's own PCG with the same preconditioner
@@ -32,12 +34,13 @@ def test_poisson_fsai(gpu, prec, method, tol):
     assert res.iterations <= 1.5 * it_ref + 20
 
 
+@pytest.mark.parametrize("product", [False, True], ids=["factors", "product"])
 @pytest.mark.parametrize("prec,method", VARIANTS)
-def test_contrast_stencil(gpu, prec, method):
+def test_contrast_stencil(gpu, prec, method, product):
     A = G.stencil27_contrast(32)
     b = O.csr_spmv(A, G.uniform(2, A.n_rows))
     M = I.fastainv_sym(A, I.F64, droptol=0.03)
-    res = S.solve(A, b, M=M, tol=1e-10, precision=prec, method=method)
+    res = S.solve(A, b, M=M, tol=1e-10, precision=prec, method=method, fsai_product=product)
     assert res.converged and oracle_relres(A, b, res.x) <= 1e-10
 
 
