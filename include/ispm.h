@@ -233,6 +233,12 @@ int ispm_plan_default_launch(unsigned slice_height, int var_height, ispm_launch 
  * and compared), with the default launch shape. */
 int ispm_plan_estimate(int vec_type, int elm_type, const ispm_csr *A, unsigned *S,
                        unsigned *hblock, int *V, int *D, ispm_launch *launch);
+/* No reference counterpart (the learned planner's training labels): SpMV seconds (median
+ * of 5 launches, default launch shape of each format) of `count` formats (S[c], hblock[c],
+ * V[c], D[c]); +inf where a format cannot be built for A. */
+int ispm_plan_time_formats(int vec_type, int elm_type, const ispm_csr *A, int count,
+                           const unsigned *S, const unsigned *hblock, const int *V, const int *D,
+                           double *seconds);
 /* The PLAN_EXHAUSTIVE search (plan/plan.h:97-144): times every (S, V, D, hblock) format and
  * launch shape on the device and returns the fastest. */
 int ispm_plan_tune(int vec_type, int elm_type, const ispm_csr *A, unsigned *S, unsigned *hblock,
@@ -364,6 +370,25 @@ int ispm_fastainv_sym(int out_type, const ispm_csr *A, ispm_progress_fn progress
  * one-matrix preconditioner Ainv2 * Ainv1 when that has fewer nonzeros than the two factors
  * together.  Output is library-owned (ispm_csr_out_free). */
 int ispm_csr_multiply(const ispm_csr *A, const ispm_csr *B, int out_type, ispm_csr_out *C);
+
+/* No reference counterpart: the learned planner's network (32 features -> 64 -> 64 ->
+ * C plan scores + 1 log2 drop tolerance, ReLU, FP32), evaluated and trained by HIP kernels.
+ * All pointers are device pointers; params holds ispm_mlp_param_count(C) floats laid out
+ * W1[64x32] b1[64] W2[64x64] b2[64] W3[(C+1)x64] b3[C+1].  X is B x 32 (row-major); Tp B x C
+ * target distributions over the formats, Mp / Md B masks (> 0: the sample has a label in
+ * that head), Td B target log2 drop tolerances.  Loss = mean over samples of the masked soft
+ * cross-entropy of the plan scores plus drop_weight * masked squared error of the drop
+ * tolerance head.  One set of activation buffers per process: calls must not overlap. */
+int ispm_mlp_param_count(int n_plan_classes);
+int ispm_mlp_forward(int n_plan_classes, const float *params, const float *X, int B, float *out);
+int ispm_mlp_gradient(int n_plan_classes, const float *params, const float *X, const float *Tp,
+                      const float *Mp, const float *Td, const float *Md, int B, float drop_weight,
+                      float *grad, float *loss);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8, bias-corrected for step >= 1) on params with
+ * moment buffers m, v (zero before step 1). */
+int ispm_mlp_train_step(int n_plan_classes, float *params, float *m, float *v, int step,
+                        const float *X, const float *Tp, const float *Mp, const float *Td,
+                        const float *Md, int B, float lr, float drop_weight, float *loss);
 void ispm_csr_out_free(ispm_csr_out *m);
 
 #ifdef __cplusplus

@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <thread>
 #include <vector>
 #include "convert.h"
 #include "internal.h"
@@ -657,6 +658,57 @@ int ispm_plan_estimate(int vt, int et, const ispm_csr *A, unsigned *S, unsigned 
   L.V = v != 0;
   *launch = default_launch(L);
   return ISPM_OK;
+}
+
+int ispm_plan_time_formats(int vt, int et, const ispm_csr *A, int count, const unsigned *S,
+                           const unsigned *hblock, const int *V, const int *D, double *seconds)
+{
+  int st = check_csr(A);
+  if (st) return st;
+  if (count < 0 || (count && (!S || !hblock || !V || !D || !seconds)))
+    return fail_plain(ISPM_ERR_INVALID, "plan_time_formats: bad argument");
+  if ((st = ensure_init())) return st;
+  if (et < 0) et = A->elm_type;
+  const size_t nvec = std::max<size_t>((size_t)A->n_rows + 512, (size_t)A->n_cols + 512);
+  void *x = nullptr, *y = nullptr, *x2 = nullptr, *y2 = nullptr;
+  if ((st = ispm_malloc(&x, tsize(vt) * nvec)) || (st = ispm_malloc(&y, tsize(vt) * nvec)) ||
+      (st = ispm_malloc(&x2, tsize(vt) * nvec)) || (st = ispm_malloc(&y2, tsize(vt) * nvec))) {
+    ispm_free(x); ispm_free(y); ispm_free(x2); ispm_free(y2);
+    return st;
+  }
+  /* layouts are converted on host threads, a batch at a time, then uploaded and timed */
+  const int nt = (int)std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+  for (int c0 = 0; c0 < count && !st; c0 += nt) {
+    const int nb = std::min(nt, count - c0);
+    std::vector<conv::layout> Ls(nb);
+    std::vector<int> ok(nb, 0);
+    std::vector<std::thread> th;
+    for (int k = 0; k < nb; k++)
+      th.emplace_back([&, k] { ok[k] = !make_layout(A, et, S[c0 + k], hblock[c0 + k], V[c0 + k], D[c0 + k], Ls[k]); });
+    for (auto &t : th) t.join();
+    for (int k = 0; k < nb && !st; k++) {
+      const int c = c0 + k;
+      seconds[c] = INFINITY;
+      if (!ok[k]) continue;   /* format not buildable */
+      const conv::layout &L = Ls[k];
+      std::vector<unsigned char> el(L.n_elms * tsize(et) + 1);
+      fill_values(L, A, et, el.data());
+      ispm_slell_desc d;
+      if ((st = upload_layout(L, el.data(), et, &d))) break;
+      diag_pattern pat;
+      unsigned *uni = nullptr;
+      bool has_pat = false;
+      if (!(st = upload_pattern(L, pat, uni, has_pat))) {
+        const ispm_launch lp = default_launch(L);
+        double sec;
+        if (!time_candidate(vt, d, lp, x, y, x2, y2, sec, has_pat ? &pat : nullptr, uni)) seconds[c] = sec;
+      }
+      ispm_slell_device_free(&d);
+      if (uni) ispm_free(uni);
+    }
+  }
+  ispm_free(x); ispm_free(y); ispm_free(x2); ispm_free(y2);
+  return st;
 }
 
 int ispm_plan_tune(int vt, int et, const ispm_csr *A, unsigned *S, unsigned *hblock, int *V,

@@ -17,7 +17,8 @@ struct spmv_launch_params
   bool use_tex_sliceptr;   /* accepted, no effect: x is read through the cache hierarchy */
   bool use_tex_x;
   int mode;                /* 0: workgroup per slice (reference mapping); 1: packed rows for
-                              slice heights below 64 (see ispm_launch) */
+                              slice heights below 64; 2: streamed elements for plain
+                              slices up to 256 rows (see ispm_launch) */
 
   bool valid() const { return SPMV_PARAMS_VALID_P(N, S, H); }
 };

@@ -136,6 +136,15 @@ _SIGS = {
                                     C.c_longlong, C.c_double, C.POINTER(CsrOut), C.POINTER(CsrOut)]),
     "ispm_csr_out_free": (None, [C.POINTER(CsrOut)]),
     "ispm_csr_multiply": (C.c_int, [C.POINTER(Csr), C.POINTER(Csr), C.c_int, C.POINTER(CsrOut)]),
+    "ispm_plan_time_formats": (C.c_int, [C.c_int, C.c_int, C.POINTER(Csr), C.c_int, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ispm_mlp_param_count": (C.c_int, [C.c_int]),
+    "ispm_mlp_forward": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "ispm_mlp_gradient": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_int, C.c_float, C.c_void_p, C.c_void_p]),
+    "ispm_mlp_train_step": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_float,
+                                      C.c_float, C.c_void_p]),
     "ispm_mm_read": (C.c_int, [C.c_char_p, C.POINTER(CsrOut)]),
     "ispm_mm_write": (C.c_int, [C.c_char_p, C.POINTER(Csr)]),
     "ispm_csr_fingerprint": (C.c_int, [C.POINTER(Csr), C.POINTER(C.c_ulonglong)]),

@@ -129,6 +129,48 @@ def stencil27_contrast(m: int = 192, block: int = 8, contrast: float = 1e4, seed
     return A
 
 
+def stencil(nx: int, ny: int, nz: int, points: int = 7, anisotropy: float = 1.0,
+            contrast: float = 1.0, block: int = 4, seed: int = 0, shift: float = 1e-3) -> CSR:
+    """SPD structured-grid operator with a 7-, 19- or 27-point stencil: a weighted graph
+    Laplacian, edge weight = offset class factor (face 1, edge 1/2, corner 1/4) times z
+    anisotropy^|dz| times the harmonic mean of the two cells' coefficients, the coefficients
+    log-uniform in [1, contrast] on random block^3 blocks; diagonal = sum of the row's edge
+    weights (Dirichlet boundary: edges leaving the grid count) plus shift."""
+    n = nx * ny * nz
+    offs = []
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                k = abs(dx) + abs(dy) + abs(dz)
+                if k == 0 or (points == 7 and k > 1) or (points == 19 and k > 2):
+                    continue
+                offs.append((dx, dy, dz))
+    offs.append((0, 0, 0))
+    bx, by, bz = -(-nx // block), -(-ny // block), -(-nz // block)
+    kb = np.exp(np.log(max(contrast, 1.0)) * uniform01(seed, bx * by * bz))
+    idx = np.arange(n, dtype=np.int64)
+    x, y, z = idx % nx, (idx // nx) % ny, idx // (nx * ny)
+    kappa = kb[(x // block) + bx * ((y // block) + by * (z // block))]
+    del x, y, z
+    diag = np.full(n, float(shift))
+    cls = {1: 1.0, 2: 0.5, 3: 0.25}
+
+    def val(mask, idx_, o):
+        if o == (0, 0, 0):
+            return np.zeros(idx_.size)
+        lin = o[2] * nx * ny + o[1] * nx + o[0]
+        nbr = np.clip(idx_ + lin, 0, n - 1)
+        kn = np.where(mask, kappa[nbr], kappa)
+        w = cls[abs(o[0]) + abs(o[1]) + abs(o[2])] * anisotropy ** abs(o[2]) * 2 * kappa * kn / (kappa + kn)
+        diag[:] += w
+        return np.where(mask, -w, 0.0)
+
+    A = _stencil_csr(nx, ny, nz, offs, val)
+    rows = np.repeat(np.arange(A.n_rows), np.diff(A.row_ptr.astype(np.int64)))
+    A.vals[A.cols == rows] = diag
+    return A
+
+
 def fv7_lognormal(m: int = 128, sigma: float = 1.5, seed: int = 2001, permute_seed: int = 2002,
                   rcm: bool = True) -> CSR:
     import scipy.sparse as sp

@@ -253,7 +253,9 @@ def test_full_size_27pt_192(gpu, vt, et):
     A = _big()
     p = I.Plan(A, vt, elm_type=et)
     inf = p.info()
-    assert inf.D == 1 and inf.S == 64
+    # the guess picks 64-row diagonal slices; the shared 27-diagonal pattern moves the
+    # default plan to 256-row slices
+    assert inf.D == 1 and inf.S == 256 and inf.pattern_planes == 27
     n = p.n_rows_padded
     dt = np.float64 if vt == I.F64 else np.float32
     ones = I.DeviceArray.from_numpy(np.ones(n, dt))
