@@ -206,7 +206,7 @@ static int make_layout(const ispm_csr *A, int et, unsigned S, unsigned W, int V,
  * bit per slice that has exactly that list and no ELL planes. */
 bool build_pattern(const conv::layout &L, diag_pattern &pat, std::vector<unsigned> &mask)
 {
-  if (!L.D || L.V || L.W != 1 || L.S < 64 || L.S > 256 || !L.n_slices) return false;
+  if (!L.D || L.V || L.W != 1 || L.S < 64 || L.S > 512 || !L.n_slices) return false;
   const unsigned S = L.S, ns = L.n_slices;
   auto rel_of = [&](unsigned s, std::vector<long long> &r) {
     const unsigned *rec = &L.sp[3 * (size_t)s];

@@ -288,7 +288,7 @@ template<typename T, typename E> struct rows_load<T, E, 4> {
   }
 };
 
-template<typename T, typename E, unsigned LOG2S, int H>
+template<typename T, typename E, unsigned LOG2S, int H, unsigned NPM>
 __global__ void __launch_bounds__((1u << LOG2S) / H)
 slell_diagpat_kernel(spmv_args a, diag_pattern pat, const unsigned *__restrict__ uni)
 {
@@ -316,10 +316,10 @@ slell_diagpat_kernel(spmv_args a, diag_pattern pat, const unsigned *__restrict__
   for (int h = 0; h < H; h++) acc[h] = 0;
   if ((uni[slice >> 5] >> (slice & 31u)) & 1u) {
     const unsigned np = pat.n;
-    T ev[kPatPlanes][H];
+    T ev[NPM][H];   /* NPM >= the pattern's plane count: registers sized to it */
     const E *ep = el + e0 + r0;
 #pragma unroll
-    for (unsigned j = 0; j < kPatPlanes; j++)
+    for (unsigned j = 0; j < NPM; j++)
       if (j < np) {
         rows_load<T, E, H>::get(ep, j << LOG2S, ev[j]);
       }
@@ -355,7 +355,7 @@ slell_diagpat_kernel(spmv_args a, diag_pattern pat, const unsigned *__restrict__
     }
     __syncthreads();
 #pragma unroll
-    for (unsigned j = 0; j < kPatPlanes; j++)
+    for (unsigned j = 0; j < NPM; j++)
       if (j < np)
 #pragma unroll
         for (int h = 0; h < H; h++) acc[h] += ev[j][h] * win[pat.poff[j] + r0 + h];
@@ -394,6 +394,27 @@ slell_diagpat_kernel(spmv_args a, diag_pattern pat, const unsigned *__restrict__
   }
 }
 
+/* the (slice height, rows per thread, plane bound) instance of the pattern kernel */
+template<typename T, typename E, unsigned NPM>
+int diagpat_launch(const ispm_slell_desc *A, unsigned H, dim3 g, dim3 b, size_t lds, const spmv_args &a,
+                   const diag_pattern &pat, const unsigned *uni)
+{
+  hipStream_t st = stream();
+  switch (A->slice_height * 4 + H) {
+    case 64 * 4 + 1: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 6, 1, NPM>), g, b, lds, st, a, pat, uni); break;
+    case 128 * 4 + 1: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 7, 1, NPM>), g, b, lds, st, a, pat, uni); break;
+    case 256 * 4 + 1: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 8, 1, NPM>), g, b, lds, st, a, pat, uni); break;
+    case 128 * 4 + 2: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 7, 2, NPM>), g, b, lds, st, a, pat, uni); break;
+    case 256 * 4 + 2: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 8, 2, NPM>), g, b, lds, st, a, pat, uni); break;
+    case 256 * 4 + 4: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 8, 4, NPM>), g, b, lds, st, a, pat, uni); break;
+    case 512 * 4 + 1: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 9, 1, NPM>), g, b, lds, st, a, pat, uni); break;
+    case 512 * 4 + 2: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 9, 2, NPM>), g, b, lds, st, a, pat, uni); break;
+    default: return fail(ISPM_ERR_UNSUPPORTED, "spmv: pattern kernel for slice height %u", A->slice_height);
+  }
+  ISPM_HIP(hipGetLastError());
+  return ISPM_OK;
+}
+
 template<typename T, typename E>
 int launch_diagpat(const ispm_slell_desc *A, const diag_pattern &pat, const unsigned *uni,
                    const void *x, void *y, const unsigned *skip, const void *dotv = nullptr,
@@ -429,18 +450,9 @@ int launch_diagpat(const ispm_slell_desc *A, const diag_pattern &pat, const unsi
   const size_t lds = (size_t)pat.lds_len * sizeof(T);
   if (lds > 65536) return fail(ISPM_ERR_UNSUPPORTED, "spmv: pattern windows need %zu bytes of LDS", lds);
   const dim3 g(a.per_xcd * 8), b(A->slice_height / H);
-  hipStream_t st = stream();
-  switch (A->slice_height * 4 + H) {
-    case 64 * 4 + 1: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 6, 1>), g, b, lds, st, a, pat, uni); break;
-    case 128 * 4 + 1: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 7, 1>), g, b, lds, st, a, pat, uni); break;
-    case 256 * 4 + 1: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 8, 1>), g, b, lds, st, a, pat, uni); break;
-    case 128 * 4 + 2: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 7, 2>), g, b, lds, st, a, pat, uni); break;
-    case 256 * 4 + 2: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 8, 2>), g, b, lds, st, a, pat, uni); break;
-    case 256 * 4 + 4: hipLaunchKernelGGL((slell_diagpat_kernel<T, E, 8, 4>), g, b, lds, st, a, pat, uni); break;
-    default: return fail(ISPM_ERR_UNSUPPORTED, "spmv: pattern kernel for slice height %u", A->slice_height);
-  }
-  ISPM_HIP(hipGetLastError());
-  return ISPM_OK;
+  if (pat.n <= 8) return diagpat_launch<T, E, 8>(A, H, g, b, lds, a, pat, uni);
+  if (pat.n <= 27) return diagpat_launch<T, E, 27>(A, H, g, b, lds, a, pat, uni);
+  return diagpat_launch<T, E, kPatPlanes>(A, H, g, b, lds, a, pat, uni);
 }
 
 /* pattern path: a plan's diagonal fixed-height slices, one row per thread, single vector */
@@ -449,7 +461,7 @@ inline bool use_diagpat(const ispm_launch *p, const ispm_slell_desc *A, int vt, 
 {
   return pat && uni && pat->n && p->mode == 0 && !A->var_height && A->diags && A->hblock == 1 &&
          (H == 1 || H == 2) && p->N * H == A->slice_height && A->slice_height >= 64 &&
-         A->slice_height <= 256 && (size_t)pat->lds_len * type_size(vt) <= 65536;
+         A->slice_height <= 512 && (size_t)pat->lds_len * type_size(vt) <= 65536;
 }
 
 inline int launch_diagpat_types(const ispm_slell_desc *A, int vt, const diag_pattern &pat,

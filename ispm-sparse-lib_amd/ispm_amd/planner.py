@@ -224,7 +224,7 @@ def load_corpus(paths):
     return (np.concatenate(X)[order], np.concatenate(T)[order], np.concatenate(DT)[order], idx[order])
 
 
-def targets(T, DT):
+def targets(T, DT, tau: float = 8.0):
     """(Tp, Mp, Td, Md) training targets from the timing tables."""
     N = T.shape[0]
     Tp = np.zeros((N, N_CLASSES), np.float32)
@@ -233,7 +233,7 @@ def targets(T, DT):
     Md = np.zeros(N, np.float32)
     for i in range(N):
         if np.isfinite(T[i]).any():
-            Tp[i] = plan_target(T[i])
+            Tp[i] = plan_target(T[i], tau)
             Mp[i] = 1
         ok = np.isfinite(DT[i])
         if ok.any():
@@ -250,7 +250,8 @@ def _dp(a: DeviceArray, off_elems: int = 0):
 class Model:
     """Parameters (flat FP32, the layout of ispm_mlp_*) and the feature normalisation."""
 
-    def __init__(self, params=None, mean=None, std=None, seed=0):
+    def __init__(self, params=None, mean=None, std=None, seed=0, confidence=0.3):
+        self.confidence = float(confidence)
         self.n_params = lib().ispm_mlp_param_count(N_CLASSES)
         if params is None:
             rng = np.random.default_rng(seed)
@@ -318,8 +319,15 @@ class Model:
         self.params = p.numpy()
         return hist
 
+    def calibrate(self, X, quantile=0.25):
+        """Confidence threshold: the given quantile of the top-class probability over X (the
+        share of matrices below it goes to the exhaustive fallback)."""
+        p, _ = self.predict(X)
+        self.confidence = float(np.quantile(p.max(1), quantile))
+        return self.confidence
+
     def save(self, path):
-        np.savez(path, params=self.params, mean=self.mean, std=self.std,
+        np.savez(path, params=self.params, mean=self.mean, std=self.std, confidence=self.confidence,
                  classes=np.array(CLASSES, np.int32), droptol_grid=np.array(DROPTOL_GRID))
 
     @classmethod
@@ -327,7 +335,8 @@ class Model:
         z = np.load(path, allow_pickle=False)
         if not np.array_equal(z["classes"], np.array(CLASSES, np.int32)):
             raise ValueError("the model was trained on another class list")
-        return cls(z["params"], z["mean"], z["std"])
+        conf = float(z["confidence"]) if "confidence" in z.files else 0.3
+        return cls(z["params"], z["mean"], z["std"], confidence=conf)
 
 
 def load_default():
@@ -335,14 +344,15 @@ def load_default():
 
 
 # ---------------------------------------------------------------------- decisions --
-def choose_plan(A: CSR, vt=F64, et=F64, model=None, confidence=0.3):
+def choose_plan(A: CSR, vt=F64, et=F64, model=None, confidence=None):
     """(S, V, D, hblock, source): the model's class when its probability reaches
-    `confidence`, else the fastest class of the exhaustive timing table."""
+    `confidence` (default: the model's calibrated threshold), else the fastest class of the
+    exhaustive timing table."""
     model = model or load_default()
     if model is not None:
         p, _ = model.predict(features(A))
         c = int(np.argmax(p[0]))
-        if p[0, c] >= confidence:
+        if p[0, c] >= (model.confidence if confidence is None else confidence):
             return (*CLASSES[c], "model")
     t = time_formats(A, vt, et)
     return (*CLASSES[int(np.argmin(t))], "exhaustive")
@@ -365,13 +375,14 @@ def best_fixed_class(T) -> int:
     return int(np.argmin(np.exp(np.log(T / T.min(1, keepdims=True)).mean(0))))
 
 
-def evaluate(model, X, T, DT, confidence=0.3, fixed_class=None):
+def evaluate(model, X, T, DT, confidence=None, fixed_class=None):
     """Held-out quality: for the plan head, the time of the chosen class (model when
     confident, else exhaustive) over the exhaustive best, and of the model's top class alone
     (and of one fixed class, when given); for the droptol head, set-up + solve time at the grid
     point nearest the prediction over that at the default 0.125 and over the best grid point."""
     p, lg = model.predict(X)
-    res = {"n": int(X.shape[0])}
+    confidence = model.confidence if confidence is None else confidence
+    res = {"n": int(X.shape[0]), "confidence_threshold": confidence}
     if fixed_class is not None:
         r = T[:, fixed_class] / T.min(1)
         res["plan_fixed_class"] = CLASSES[fixed_class]
@@ -413,6 +424,8 @@ def main(argv=None):
     t.add_argument("--corpus", nargs="+", required=True)
     t.add_argument("--holdout", type=int, default=60)
     t.add_argument("--epochs", type=int, default=400)
+    t.add_argument("--tau", type=float, default=8.0)
+    t.add_argument("--lr", type=float, default=3e-3)
     t.add_argument("--out", default=MODEL_PATH)
     e = sub.add_parser("eval")
     e.add_argument("--corpus", nargs="+", required=True)
@@ -430,9 +443,10 @@ def main(argv=None):
     X, T, DT, idx = load_corpus(args.corpus)
     ntr = X.shape[0] - args.holdout
     if args.cmd == "train":
-        Tp, Mp, Td, Md = targets(T, DT)
+        Tp, Mp, Td, Md = targets(T, DT, args.tau)
         model = Model(seed=0)
-        model.fit(X[:ntr], Tp[:ntr], Mp[:ntr], Td[:ntr], Md[:ntr], epochs=args.epochs, log=log)
+        model.fit(X[:ntr], Tp[:ntr], Mp[:ntr], Td[:ntr], Md[:ntr], epochs=args.epochs, lr=args.lr, log=log)
+        model.calibrate(X[:ntr])
         model.save(args.out)
         fixed = best_fixed_class(T[:ntr])
         log("train", evaluate(model, X[:ntr], T[:ntr], DT[:ntr], fixed_class=fixed))

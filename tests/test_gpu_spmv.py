@@ -126,7 +126,7 @@ def stencil_then_irregular():
     return I.CSR(A.n_rows + B.n_rows, A.n_cols + B.n_cols, rp, cols, vals)
 
 
-@pytest.mark.parametrize("S,H", [(64, 1), (128, 1), (256, 1), (128, 2), (256, 2)])
+@pytest.mark.parametrize("S,H", [(64, 1), (128, 1), (256, 1), (128, 2), (256, 2), (512, 1), (512, 2)])
 @pytest.mark.parametrize("vt,et", TYPES)
 def test_spmv_diag_pattern(mats, S, H, vt, et):
     """Plans whose diagonal slices mostly share one pattern: those slices stage the x windows

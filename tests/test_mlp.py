@@ -193,3 +193,23 @@ def test_gpu_committed_model_predicts(gpu):
     A = G.stencil(20, 20, 20, 19, anisotropy=10, contrast=100, seed=11)
     p, lg = model.predict(PL.features(A))
     assert p.shape == (1, C_) and np.isfinite(p).all() and np.isfinite(lg).all()
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(PL.MODEL_PATH), reason="no trained model in the package")
+def test_gpu_committed_model_beats_fixed_format_on_holdout(gpu):
+    """What the trained network gains, on the held-out end of the committed corpus (matrices it
+    was not trained on): the SpMV time of its top format over the exhaustive best, against the
+    single format that is best on average over the training part (the method without the
+    network picks one format for all matrices; the exhaustive search pays for 39 conversions
+    and timings per matrix)."""
+    data = os.path.join(os.path.dirname(PL.MODEL_PATH), "planner_data")
+    paths = sorted(os.path.join(data, f) for f in os.listdir(data) if f.endswith(".npz"))
+    X, T, DT, idx = PL.load_corpus(paths)
+    hold = 50
+    model = PL.load_default()
+    res = PL.evaluate(model, X[-hold:], T[-hold:], DT[-hold:],
+                      fixed_class=PL.best_fixed_class(T[:-hold]))
+    assert res["plan_top_ratio_gmean"] < res["plan_fixed_ratio_gmean"]
+    assert res["plan_top_ratio_gmean"] <= 1.06
+    assert res["plan_choice_ratio_gmean"] <= res["plan_top_ratio_gmean"] + 1e-12
