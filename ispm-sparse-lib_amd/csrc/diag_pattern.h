@@ -23,6 +23,12 @@ struct diag_pattern {
   unsigned rbase[kPatPlanes];    /* LDS index of run r's window */
   unsigned rlen[kPatPlanes];     /* window length (S + span of the run) */
   unsigned lds_len;              /* sum of the window lengths */
+  /* Set by the SpMV launcher when it restates the windows as their aligned covers (the
+   * enclosing runs of whole 16-byte vectors of x): the covers span x[first row + cover_lo,
+   * first row + cover_hi), and vec_runs is nonzero when each cover is at most one vector per
+   * thread and all runs fit the kernel's registers. */
+  long long cover_lo, cover_hi;
+  unsigned vec_runs;
 };
 
 }  // namespace ispm

@@ -258,15 +258,52 @@ __global__ void __launch_bounds__(1024) slell_kernel(spmv_args a)
 }
 
 
-/* Diagonal slices with one row per thread (fixed S, N == S, H == 1, one vector) of a plan
- * whose slices mostly share one diagonal pattern (diag_pattern.h).  A slice marked in `uni`
- * has no ELL planes and exactly the pattern's diagonals: its elements for all planes are
- * requested at once, the pattern's x windows are loaded into LDS with coalesced loads
- * (addresses known from the slice index alone), and every plane reads its x from LDS.  Other
- * slices take the generic path, reading positions and x directly.  Both paths add the ELL
+/* Diagonal slices with H rows per thread (fixed S, one vector) of a plan whose slices mostly
+ * share one diagonal pattern (diag_pattern.h).  A slice marked in `uni` has no ELL planes
+ * and exactly the pattern's diagonals: the loads of its x windows (16-byte vectors of their
+ * aligned covers, addresses known from the slice index alone) are issued first, then the
+ * loads of all its element planes; the windows go to LDS as soon as they arrive, and every
+ * plane reads its x from LDS.  Other slices take the generic path, reading positions and x
+ * directly.  Both paths add the ELL
  * planes, then the diagonal planes in storage order, as slell_kernel does, so the results
  * are identical to it. */
 template<typename T> using vecx = T __attribute__((ext_vector_type(16 / sizeof(T))));
+
+/* x window runs a pattern kernel instance keeps in registers (bound by its plane count) */
+constexpr unsigned pattern_runs_in_regs(unsigned NPM) { return NPM <= 8 ? 8u : NPM <= 27 ? 9u : 16u; }
+
+/* The pattern restated for x loads of VX elements: each run's window [rrel, rrel + rlen)
+ * widened to its VX-aligned cover, covers packed at VX-aligned LDS offsets, poff moved with
+ * them; NT threads per workgroup and NR runs in registers decide vec_runs.  Slice first rows
+ * are multiples of the slice height, so the covers are aligned in every slice. */
+inline void cover_pattern(const diag_pattern &p, unsigned VX, unsigned NT, unsigned NR, diag_pattern &c)
+{
+  c = p;
+  unsigned used = 0, maxnv = 0;
+  long long lo = 0, hi = 0;
+  for (unsigned r = 0; r < p.nrun; r++) {
+    const long long rel = p.rrel[r];
+    const long long a = (rel >= 0 ? rel / VX : -((-rel + VX - 1) / VX)) * VX;
+    const unsigned len = (unsigned)(((rel - a) + p.rlen[r] + VX - 1) / VX * VX);
+    c.rrel[r] = (int)a;
+    c.rlen[r] = len;
+    c.rbase[r] = used;
+    used += len;
+    lo = r == 0 || a < lo ? a : lo;
+    hi = r == 0 || a + len > hi ? a + len : hi;
+    maxnv = len / VX > maxnv ? len / VX : maxnv;
+  }
+  for (unsigned j = 0; j < p.n; j++)
+    for (unsigned r = 0; r < p.nrun; r++)
+      if (p.poff[j] >= p.rbase[r] && p.poff[j] < p.rbase[r] + p.rlen[r]) {
+        c.poff[j] = c.rbase[r] + (p.poff[j] - p.rbase[r]) + (unsigned)(p.rrel[r] - c.rrel[r]);
+        break;
+      }
+  c.lds_len = used;
+  c.cover_lo = lo;
+  c.cover_hi = hi;
+  c.vec_runs = p.nrun <= NR && maxnv <= NT ? 1u : 0u;
+}
 
 /* H consecutive elements (aligned to H) of one plane, converted to T */
 template<typename T, typename E, int H> struct rows_load;
@@ -294,7 +331,7 @@ slell_diagpat_kernel(spmv_args a, diag_pattern pat, const unsigned *__restrict__
 {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *win = reinterpret_cast<T *>(lds_raw);
-  constexpr unsigned S = 1u << LOG2S, NT = S / H, VX = 16 / sizeof(T);
+  constexpr unsigned S = 1u << LOG2S, NT = S / H, VX = 16 / sizeof(T), NR = pattern_runs_in_regs(NPM);
   const unsigned t = threadIdx.x, r0 = t * H;   /* this thread's first row in the slice */
   /* one slice per workgroup */
   const unsigned slice = (blockIdx.x & 7u) * a.per_xcd + (blockIdx.x >> 3);
@@ -315,40 +352,36 @@ slell_diagpat_kernel(spmv_args a, diag_pattern pat, const unsigned *__restrict__
 #pragma unroll
   for (int h = 0; h < H; h++) acc[h] = 0;
   if ((uni[slice >> 5] >> (slice & 31u)) & 1u) {
-    const unsigned np = pat.n;
-    T ev[NPM][H];   /* NPM >= the pattern's plane count: registers sized to it */
-    const E *ep = el + e0 + r0;
+    const unsigned np = pat.n, nrun = pat.nrun;
+    /* x windows, staged with the rest of their 16-byte aligned covers (rrel, rlen, rbase and
+     * poff describe the covers).  When every cover lies inside x and needs at most one vector
+     * per thread, the window loads are issued first, before the element loads, so staging
+     * them waits for them alone; otherwise a bounds-checked loop stages them after. */
+    const bool fast = xvec && pat.vec_runs && (long long)row0 + pat.cover_lo >= 0 &&
+                      (long long)row0 + pat.cover_hi <= (long long)ncols;
+    vecx<T> w[NR];
+    if (fast) {
+      /* runs past the pattern's count load run 0 again: a fixed number of loads */
+#pragma unroll
+      for (unsigned r = 0; r < NR; r++) {
+        const unsigned rr = r < nrun ? r : 0u;
+        const unsigned nv = pat.rlen[rr] / VX, v = t < nv ? t : nv - 1;
+        w[r] = *reinterpret_cast<const vecx<T> *>(xv + (row0 + (unsigned)pat.rrel[rr] + v * VX));
+      }
+    }
+    /* every plane requested at once; planes past the pattern's count read one address that
+     * is the same in all lanes (a single request per wave), keeping the number of loads in
+     * flight fixed so each wait above and below is for exactly the loads it needs */
+    T ev[NPM][H];
 #pragma unroll
     for (unsigned j = 0; j < NPM; j++)
-      if (j < np) {
-        rows_load<T, E, H>::get(ep, j << LOG2S, ev[j]);
-      }
-    if (xvec) {
-      /* 16-byte loads from the aligned cover of each window; entries outside the window
-       * are not stored (they belong to neighbouring windows) */
-      for (unsigned r = 0; r < pat.nrun; r++) {
-        const unsigned x0 = row0 + (unsigned)pat.rrel[r], L = pat.rlen[r], b = pat.rbase[r];
-        const unsigned xa = x0 & ~(VX - 1), nvec = (x0 - xa + L + VX - 1) / VX;
-        for (unsigned v = t; v < nvec; v += NT) {
-          const unsigned xi = xa + v * VX;
-          T w[VX];
-          if (xi < ncols && xi + VX <= ncols) {
-            const vecx<T> q = *reinterpret_cast<const vecx<T> *>(xv + xi);
+      rows_load<T, E, H>::get(el + e0 + (j < np ? r0 : 0u), j < np ? (j << LOG2S) : 0u, ev[j]);
+    if (fast) {
 #pragma unroll
-            for (unsigned k = 0; k < VX; k++) w[k] = q[k];
-          } else {
-#pragma unroll
-            for (unsigned k = 0; k < VX; k++) w[k] = xload(xv, xi + k, ncols);
-          }
-#pragma unroll
-          for (unsigned k = 0; k < VX; k++) {
-            const unsigned pos = xi + k - x0;   /* modulo 2^32, like the indices */
-            if (pos < L) win[b + pos] = w[k];
-          }
-        }
-      }
+      for (unsigned r = 0; r < NR; r++)
+        if (r < nrun && t < pat.rlen[r] / VX) *reinterpret_cast<vecx<T> *>(win + pat.rbase[r] + t * VX) = w[r];
     } else {
-      for (unsigned r = 0; r < pat.nrun; r++) {
+      for (unsigned r = 0; r < nrun; r++) {
         const unsigned x0 = row0 + (unsigned)pat.rrel[r], L = pat.rlen[r], b = pat.rbase[r];
         for (unsigned i = t; i < L; i += NT) win[b + i] = xload(xv, x0 + i, ncols);
       }
@@ -447,12 +480,15 @@ int launch_diagpat(const ispm_slell_desc *A, const diag_pattern &pat, const unsi
     a.dparts = dparts;
     *dgrid = a.per_xcd * 8;
   }
-  const size_t lds = (size_t)pat.lds_len * sizeof(T);
+  const unsigned NPM = pat.n <= 8 ? 8u : pat.n <= 27 ? 27u : (unsigned)kPatPlanes;
+  diag_pattern c;
+  cover_pattern(pat, 16 / sizeof(T), A->slice_height / H, pattern_runs_in_regs(NPM), c);
+  const size_t lds = (size_t)c.lds_len * sizeof(T);
   if (lds > 65536) return fail(ISPM_ERR_UNSUPPORTED, "spmv: pattern windows need %zu bytes of LDS", lds);
   const dim3 g(a.per_xcd * 8), b(A->slice_height / H);
-  if (pat.n <= 8) return diagpat_launch<T, E, 8>(A, H, g, b, lds, a, pat, uni);
-  if (pat.n <= 27) return diagpat_launch<T, E, 27>(A, H, g, b, lds, a, pat, uni);
-  return diagpat_launch<T, E, kPatPlanes>(A, H, g, b, lds, a, pat, uni);
+  if (NPM == 8) return diagpat_launch<T, E, 8>(A, H, g, b, lds, a, c, uni);
+  if (NPM == 27) return diagpat_launch<T, E, 27>(A, H, g, b, lds, a, c, uni);
+  return diagpat_launch<T, E, kPatPlanes>(A, H, g, b, lds, a, c, uni);
 }
 
 /* pattern path: a plan's diagonal fixed-height slices, one row per thread, single vector */
@@ -461,7 +497,8 @@ inline bool use_diagpat(const ispm_launch *p, const ispm_slell_desc *A, int vt, 
 {
   return pat && uni && pat->n && p->mode == 0 && !A->var_height && A->diags && A->hblock == 1 &&
          (H == 1 || H == 2) && p->N * H == A->slice_height && A->slice_height >= 64 &&
-         A->slice_height <= 512 && (size_t)pat->lds_len * type_size(vt) <= 65536;
+         A->slice_height <= 512 &&
+         ((size_t)pat->lds_len + 2 * pat->nrun * (16 / type_size(vt))) * type_size(vt) <= 65536;
 }
 
 inline int launch_diagpat_types(const ispm_slell_desc *A, int vt, const diag_pattern &pat,
