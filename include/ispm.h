@@ -263,14 +263,18 @@ void ispm_plan_destroy(ispm_plan *p);
  *   method ISPM_SOLVER_PPCG: Ghysels-Vanroose pipelined PCG with the reference's recurrences
  *     (util/cuda/sblas.cu:156-217); M = Ainv2*Ainv1 applied as two SpMVs.
  *   method ISPM_SOLVER_PCG : Hestenes-Stiefel PCG.
+ *   method ISPM_SOLVER_PCG1: single-reduction PCG (Chronopoulos-Gear): z = M r, w = A z,
+ *     both dot products (r,z), (z,w) and the convergence sum in one finishing step, then one
+ *     fused update of p, s = A p, x and r; the reference's scalar formulas.
  *   precision ISPM_PREC_FP64 : every vector and scalar in FP64.
  *   precision ISPM_PREC_MIXED: FP32 vectors, FP32/BF16 matrices, FP64 dot products and
  *     scalars; x accumulated in FP64 and the residual replaced by the FP64 true residual
- *     b - A x whenever the FP32 recursive residual has dropped by reliable_delta (PCG only).
+ *     b - A x whenever the FP32 recursive residual has dropped by reliable_delta (PCG and
+ *     PCG1).
  * Convergence is always decided on the FP64 true residual ||b - A x||_2 / ||b||_2 <= tol.
  * Iterations run in hipGraph-captured batches of check_every iterations; a device flag
  * turns the rest of a batch into no-ops once the recursive residual meets the target. */
-enum ispm_solver_method { ISPM_SOLVER_PPCG = 0, ISPM_SOLVER_PCG = 1 };
+enum ispm_solver_method { ISPM_SOLVER_PPCG = 0, ISPM_SOLVER_PCG = 1, ISPM_SOLVER_PCG1 = 2 };
 enum ispm_solver_precision { ISPM_PREC_FP64 = 0, ISPM_PREC_MIXED = 1 };
 
 typedef struct ispm_solver_opts {

@@ -1,6 +1,6 @@
 /* solver.hip -- device-resident FSAI-preconditioned CG on the library stream.
  *
- * Three variants share one driver:
+ * Four variants share one driver:
  *  - FP64 pipelined PCG (Ghysels-Vanroose): per iteration t = Ainv1 w, m = Ainv2 t, n = A m
  *    and ONE fused kernel that evaluates the reference's scalar update
  *    (beta = g/g_old, alpha = g/(delta - beta g/alpha_old), util/cuda/sblas.cu:156-165) in every
@@ -8,12 +8,16 @@
  *    (r,r) for the convergence test.
  *  - FP64 PCG (Hestenes-Stiefel): z = M r, rho = (r,z), p = z + beta p, q = A p, alpha,
  *    x += alpha p, r -= alpha q.
- *  - mixed PCG: the same loop with FP32 vectors, FP32 A and FP32/BF16 factors; x is
- *    accumulated in FP64 and the residual is replaced by the FP64 true residual
- *    (b - A x with the FP32-stored values read into FP64 accumulators) whenever the FP32
- *    recursive residual has dropped by reliable_delta ("reliable updates").
- * All scalars live in device memory in FP64; every reduction is single pass (last block
- * finishes) and deterministic.  Iterations are launched in hipGraph-captured batches; once
+ *  - single-reduction PCG (Chronopoulos-Gear): z = M r with (r,z), w = A z with (z,w), one
+ *    finishing kernel (alpha, beta by the reference's formulas; the convergence test on the
+ *    (r,r) of the previous update) and one fused update p = z + beta p, s = w + beta s,
+ *    x += alpha p, r -= alpha s: four launches per iteration instead of seven.
+ *  - mixed precision (PCG or single-reduction PCG): FP32 vectors, FP32 A and FP32/BF16
+ *    factors; x is accumulated in FP64 and the residual is replaced by the FP64 true
+ *    residual (b - A x with the FP32-stored values read into FP64 accumulators) whenever the
+ *    FP32 recursive residual has dropped by reliable_delta ("reliable updates").
+ * All scalars live in device memory in FP64; every reduction is two-phase (block partials,
+ * then a one-block finishing kernel) and deterministic.  Iterations are launched in hipGraph-captured batches; once
  * the recursive residual reaches its target a device flag turns the rest of the batch into
  * no-ops, and the host then decides on the FP64 true residual.
  */
@@ -38,6 +42,7 @@ struct scalars {
   double rr, l1, tol2, r64;
   double rscale;
   unsigned done, iters;
+  unsigned rr_valid;   /* single-reduction PCG: the (r,r) partials hold a residual to test */
 };
 
 enum { OP_PQ, OP_RZ, OP_GAMMA, OP_DELTA, OP_R64, OP_XR, OP_PPCG };
@@ -134,6 +139,88 @@ k_xr(T *__restrict__ x, T *__restrict__ r, const T *__restrict__ p, const T *__r
     T ri = r[i] - a * q[i];
     r[i] = ri;
     v[0] += (double)ri * (double)ri;
+  }
+  store_partials<1>(v, parts, lds);
+}
+
+/* Single-reduction PCG (Chronopoulos-Gear), the one finishing kernel of an iteration: sums
+ * gamma = (r, z) (partials of the preconditioner product), delta = (z, w) (partials of
+ * w = A z) and the (r, r) partials k_cg_update left one iteration earlier.  When that
+ * residual meets the target the solve is done and nothing more of the batch runs; otherwise
+ * beta = gamma / gamma_old and alpha = gamma / (delta - beta gamma / alpha_old), the scalar
+ * update of util/cuda/sblas.cu:156-165 of the reference. */
+__global__ void __launch_bounds__(kFinThreads)
+k_fin_cg(const double *pg, unsigned Gg, const double *pd, unsigned Gd, const double *pr, unsigned Gr,
+         scalars *s)
+{
+  __shared__ double lds[3 * kFinThreads / 64];
+  if (s->done) return;
+  double out[3] = {0.0, 0.0, 0.0};
+  const unsigned t = threadIdx.x;
+  /* eight loads of each sum in flight per step; a fixed order keeps the sums reproducible */
+  constexpr unsigned C = 8 * kFinThreads;
+  const unsigned Gm = Gg > Gd ? (Gg > Gr ? Gg : Gr) : (Gd > Gr ? Gd : Gr);
+  for (unsigned i0 = 0; i0 < Gm; i0 += C) {
+    double a[8], b[8], c[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const unsigned i = i0 + t + k * kFinThreads;
+      a[k] = i < Gg ? pg[i] : 0.0;
+      b[k] = i < Gd ? pd[i] : 0.0;
+      c[k] = i < Gr ? pr[i] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) { out[0] += a[k]; out[1] += b[k]; out[2] += c[k]; }
+  }
+  dev::block_sum<kFinThreads, 3>(out, lds);
+  if (t) return;
+  if (s->rr_valid) {
+    s->rr = out[2];
+    if (out[2] <= s->tol2) { s->done = 1; return; }
+  }
+  const double g = out[0], dl = out[1];
+  const double bd = g / s->gamma_old;
+  const double ad = g / (dl - bd * g / s->alpha_old);
+  s->gamma = g;
+  s->delta = dl;
+  s->gamma_old = g;
+  s->alpha_old = ad;
+  s->alpha = ad;
+  s->beta = bd;
+  s->rr_valid = 1;
+  s->iters++;
+}
+
+/* p = z + beta p; s = w + beta s; x += alpha p; r -= alpha s; (r, r) partials */
+template<typename T>
+__global__ void __launch_bounds__(256)
+k_cg_update(const T *__restrict__ z, const T *__restrict__ w, T *__restrict__ p, T *__restrict__ sv,
+            T *__restrict__ x, T *__restrict__ r, int n, const scalars *s, double *parts)
+{
+  __shared__ double lds[4];
+  if (s->done) return;
+  constexpr int L = vtraits<T>::L;
+  const T a = (T)s->alpha, b = (T)s->beta;
+  double v[1] = {0.0};
+  auto one = [&](T zi, T wi, T &pi, T &si, T &xi, T &ri) {
+    pi = zi + b * pi;
+    si = wi + b * si;
+    xi = xi + a * pi;
+    ri = ri - a * si;
+    v[0] += (double)ri * (double)ri;
+  };
+  const int gt = blockIdx.x * 256 + threadIdx.x, gs = gridDim.x * 256, nv = n / L;
+  for (int i = gt; i < nv; i += gs) {
+    pack16<T> zz = ld16(z, i), ww = ld16(w, i), pp = ld16(p, i), ss = ld16(sv, i), xx = ld16(x, i),
+              rr = ld16(r, i);
+#pragma unroll
+    for (int l = 0; l < L; l++) one(zz.v[l], ww.v[l], pp.v[l], ss.v[l], xx.v[l], rr.v[l]);
+    st16(p, i, pp); st16(sv, i, ss); st16(x, i, xx); st16(r, i, rr);
+  }
+  for (int i = nv * L + gt; i < n; i += gs) {
+    T pi = p[i], si = sv[i], xi = x[i], ri = r[i];
+    one(z[i], w[i], pi, si, xi, ri);
+    p[i] = pi; sv[i] = si; x[i] = xi; r[i] = ri;
   }
   store_partials<1>(v, parts, lds);
 }
@@ -267,7 +354,7 @@ struct ispm_solver {
   int n = 0;
   int T = ISPM_F64;
   size_t ts = 8;
-  bool mixed = false, pipelined = false;
+  bool mixed = false, pipelined = false, single = false;
   void *r = nullptr, *u = nullptr, *w = nullptr, *m = nullptr, *nv = nullptr, *z = nullptr,
        *q = nullptr, *sv = nullptr, *p = nullptr, *t = nullptr, *xs = nullptr;
   double *r64 = nullptr, *tmp64 = nullptr;
@@ -341,14 +428,15 @@ int apply_M(ispm_solver *S, const void *rin, void *zout, const unsigned *skip)
  * else a separate dot kernel.  Returns the number of partials in G. */
 template<typename T>
 int spmv_dot(ispm_solver *S, ispm_plan *P, const void *x, void *y, const void *d,
-             const unsigned *skip, unsigned &G)
+             const unsigned *skip, unsigned &G, double *parts = nullptr)
 {
   G = 0;
+  if (!parts) parts = S->parts;
   int st = spmv_dot_internal(&P->launch, &P->dev, S->T, x, y, skip, P->has_pat ? &P->pat : nullptr,
-                             P->dev_uni, d, S->parts, &G);
+                             P->dev_uni, d, parts, &G);
   if (st || G) return st;
   hipLaunchKernelGGL((k_dot<T, OP_PQ>), dim3(S->grid), dim3(256), 0, stream(), (const T *)d, (const T *)y,
-                     S->n, S->s, S->parts, skip ? 1 : 0);
+                     S->n, S->s, parts, skip ? 1 : 0);
   ISPM_HIP(hipGetLastError());
   G = (unsigned)S->grid;
   return ISPM_OK;
@@ -371,6 +459,45 @@ int iterate(ispm_solver *S)
                        (double *)S->xs, (double *)S->r, (double *)S->u, (double *)S->w, S->n, S->s,
                        S->parts);
     finish<OP_PPCG>(S, 1);
+    ISPM_HIP(hipGetLastError());
+    return ISPM_OK;
+  }
+  if (S->single) {
+    /* z = M r with gamma = (r, z); w = A z with delta = (z, w); one finishing kernel; the
+     * fused update with (r, r) for the next iteration's test */
+    double *pg = S->parts, *pd = S->parts + kMaxDotParts, *pr = S->parts + 2 * kMaxDotParts;
+    unsigned Gg = 0, Gd = 0;
+    if (S->M1) {
+      ispm_plan *last = S->M2 ? S->M2 : S->M1;
+      const void *in = S->r;
+      if (S->M2) {
+        if ((st = spmv(S->M1, S->T, S->r, S->t, sk))) return st;
+        in = S->t;
+      }
+      st = S->T == ISPM_F64 ? spmv_dot<double>(S, last, in, S->z, S->r, sk, Gg, pg)
+                            : spmv_dot<float>(S, last, in, S->z, S->r, sk, Gg, pg);
+    } else {
+      /* no preconditioner: z = r, gamma = (r, r) */
+      if ((st = apply_M(S, S->r, S->z, sk))) return st;
+      if (S->T == ISPM_F64)
+        hipLaunchKernelGGL((k_dot<double, OP_PQ>), g, b, 0, sm, (const double *)S->r, (const double *)S->z, S->n, S->s, pg, 1);
+      else
+        hipLaunchKernelGGL((k_dot<float, OP_PQ>), g, b, 0, sm, (const float *)S->r, (const float *)S->z, S->n, S->s, pg, 1);
+      ISPM_HIP(hipGetLastError());
+      Gg = (unsigned)S->grid;
+    }
+    if (st) return st;
+    st = S->T == ISPM_F64 ? spmv_dot<double>(S, S->A, S->z, S->q, S->z, sk, Gd, pd)
+                          : spmv_dot<float>(S, S->A, S->z, S->q, S->z, sk, Gd, pd);
+    if (st) return st;
+    hipLaunchKernelGGL(k_fin_cg, dim3(1), dim3(kFinThreads), 0, sm, (const double *)pg, Gg,
+                       (const double *)pd, Gd, (const double *)pr, (unsigned)S->grid, S->s);
+    if (S->T == ISPM_F64)
+      hipLaunchKernelGGL(k_cg_update<double>, g, b, 0, sm, (const double *)S->z, (const double *)S->q,
+                         (double *)S->p, (double *)S->sv, (double *)S->xs, (double *)S->r, S->n, S->s, pr);
+    else
+      hipLaunchKernelGGL(k_cg_update<float>, g, b, 0, sm, (const float *)S->z, (const float *)S->q,
+                         (float *)S->p, (float *)S->sv, (float *)S->xs, (float *)S->r, S->n, S->s, pr);
     ISPM_HIP(hipGetLastError());
     return ISPM_OK;
   }
@@ -480,7 +607,10 @@ int ispm_solver_create(ispm_solver **out, ispm_plan *A, ispm_plan *M1, ispm_plan
   if (!M1 && M2) return fail(ISPM_ERR_INVALID, "solver_create: second factor without the first");
   if (!A->has_dev) return fail(ISPM_ERR_INVALID, "solver_create: A plan has no device matrix");
   const bool mixed = o->precision == ISPM_PREC_MIXED;
-  if (mixed && o->method != ISPM_SOLVER_PCG) return fail(ISPM_ERR_UNSUPPORTED, "solver_create: mixed precision uses method PCG");
+  if (mixed && o->method != ISPM_SOLVER_PCG && o->method != ISPM_SOLVER_PCG1)
+    return fail(ISPM_ERR_UNSUPPORTED, "solver_create: mixed precision uses method PCG or PCG1");
+  if (o->method != ISPM_SOLVER_PPCG && o->method != ISPM_SOLVER_PCG && o->method != ISPM_SOLVER_PCG1)
+    return fail(ISPM_ERR_INVALID, "solver_create: unknown method %d", o->method);
   if (!mixed && A->E != ISPM_F64) return fail(ISPM_ERR_INVALID, "solver_create: FP64 solve needs an FP64 A plan");
   if (mixed && A->E != ISPM_F32) return fail(ISPM_ERR_INVALID, "solver_create: mixed solve needs an FP32 A plan");
   const int T = mixed ? ISPM_F32 : ISPM_F64;
@@ -501,6 +631,7 @@ int ispm_solver_create(ispm_solver **out, ispm_plan *A, ispm_plan *M1, ispm_plan
     if (ng[0] == '1') S->o.use_graph = 0;
   S->mixed = mixed;
   S->pipelined = o->method == ISPM_SOLVER_PPCG;
+  S->single = o->method == ISPM_SOLVER_PCG1;
   S->T = T;
   S->ts = mixed ? 4 : 8;
   S->n = A->L.n_rows;
@@ -517,8 +648,9 @@ int ispm_solver_create(ispm_solver **out, ispm_plan *A, ispm_plan *M1, ispm_plan
     void **pv[] = {&S->u, &S->w, &S->m, &S->nv, &S->sv};
     for (void **v : pv) if ((st = alloc(v, vb))) { ispm_solver_destroy(S); return st; }
   }
+  if (S->single && (st = alloc(&S->sv, vb))) { ispm_solver_destroy(S); return st; }
   if ((st = alloc((void **)&S->r64, 8 * (size_t)S->n + 64)) || (st = alloc((void **)&S->tmp64, 8 * (size_t)S->n + 64)) ||
-      (st = alloc((void **)&S->s, sizeof(scalars))) || (st = alloc((void **)&S->parts, sizeof(double) * std::max<size_t>(kMaxBlocks * 4, kMaxDotParts)))) {
+      (st = alloc((void **)&S->s, sizeof(scalars))) || (st = alloc((void **)&S->parts, sizeof(double) * (2 * (size_t)kMaxDotParts + 4 * (size_t)kMaxBlocks)))) {
     ispm_solver_destroy(S);
     return st;
   }
@@ -563,9 +695,13 @@ int ispm_solver_solve(ispm_solver *S, const double *b, double *x, ispm_solve_sta
       ISPM_HIP(hipGetLastError());
       if ((e = pull(S))) return e;
       h.rho = INFINITY;
+      h.gamma_old = INFINITY;
+      h.alpha_old = 1.0;
+      h.rr_valid = 0;
       h.done = 0;
       h.tol2 = tol2;
       if ((e = zero(S->xs, vb)) || (e = zero(S->p, vb))) return e;
+      if (S->single && (e = zero(S->sv, vb))) return e;
       return push(S);
     }
     /* FP64: r = r64 */
@@ -587,7 +723,11 @@ int ispm_solver_solve(ispm_solver *S, const double *b, double *x, ispm_solve_sta
       return push(S);
     }
     h.rho = INFINITY;
+    h.gamma_old = INFINITY;
+    h.alpha_old = 1.0;
+    h.rr_valid = 0;
     if ((e = zero(S->p, vb))) return e;
+    if (S->single && (e = zero(S->sv, vb))) return e;
     return push(S);
   };
 
@@ -642,10 +782,14 @@ int ispm_solver_solve(ispm_solver *S, const double *b, double *x, ispm_solve_sta
       const double c = rmax_old / rmax;
       hipLaunchKernelGGL(k_cast_scaled<float>, g, bl, 0, sm, (float *)S->r, (const double *)S->r64, n, S->s);
       hipLaunchKernelGGL(k_scale<float>, g, bl, 0, sm, (float *)S->p, n, c);
+      /* single-reduction PCG: s = A p is rescaled with p */
+      if (S->single) hipLaunchKernelGGL(k_scale<float>, g, bl, 0, sm, (float *)S->sv, n, c);
       ISPM_HIP(hipGetLastError());
       if ((st = pull(S))) break;
       rec_target = std::max(S->o.reliable_delta, margin * tol * bnorm / rmax);
       h.rho = h.rho * c * c;
+      h.gamma_old = h.gamma_old * c * c;
+      h.rr_valid = 0;
       h.done = 0;
       h.tol2 = rec_target * rec_target;
       if ((st = push(S))) break;

@@ -8,7 +8,7 @@ import numpy as np
 
 from . import (BF16, CSR, F32, F64, PLAN_ESTIMATE, DeviceArray, Plan, check, csr_multiply, lib)
 
-PPCG, PCG = 0, 1
+PPCG, PCG, PCG1 = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
 
 

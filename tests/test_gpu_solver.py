@@ -1,4 +1,4 @@
-"""GPU FSAI-PCG solves (FP64 pipelined, FP64 PCG, mixed precision) checked with the CPU
+"""GPU FSAI-PCG solves (FP64 pipelined, FP64 PCG and single-reduction PCG, mixed precision) checked with the CPU
 oracle: every variant must reach the requested FP64 relative residual ||b - A x|| / ||b||
 computed by the oracle's FP64 CSR product."""
 import numpy as np
@@ -11,7 +11,7 @@ from ispm_amd import solver as S
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = [("fp64", S.PPCG), ("fp64", S.PCG), ("mixed", S.PCG)]
+VARIANTS = [("fp64", S.PPCG), ("fp64", S.PCG), ("fp64", S.PCG1), ("mixed", S.PCG), ("mixed", S.PCG1)]
 
 
 def oracle_relres(A, b, x):
@@ -64,21 +64,38 @@ def test_pipelined_l1_matches_oracle_semantics(gpu):
     assert res.l1_resnorm <= np.sqrt(A.n_rows) * 1e-10 * np.linalg.norm(b) * 2
 
 
-def test_graph_and_eager_agree(gpu):
+@pytest.mark.parametrize("method", [S.PCG, S.PCG1])
+def test_graph_and_eager_agree(gpu, method):
     A = G.poisson7(16)
     b = O.csr_spmv(A, G.uniform(6, A.n_rows))
     M = I.fastainv_sym(A, I.F64, droptol=0.05)
-    r1 = S.solve(A, b, M=M, tol=1e-10, precision="fp64", method=S.PCG, use_graph=True)
-    r2 = S.solve(A, b, M=M, tol=1e-10, precision="fp64", method=S.PCG, use_graph=False)
+    r1 = S.solve(A, b, M=M, tol=1e-10, precision="fp64", method=method, use_graph=True)
+    r2 = S.solve(A, b, M=M, tol=1e-10, precision="fp64", method=method, use_graph=False)
     assert r1.iterations == r2.iterations
     np.testing.assert_array_equal(r1.x, r2.x)
 
 
-def test_repeated_solves_reuse_graph(gpu):
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_single_reduction_matches_pcg(gpu, prec):
+    """Single-reduction PCG is Hestenes-Stiefel PCG rearranged: in exact arithmetic the same
+    iterates, so on a well-conditioned system the iteration counts agree closely and both
+    reach the FP64 target."""
+    A = G.stencil27_contrast(24)
+    b = O.csr_spmv(A, G.uniform(9, A.n_rows))
+    M = I.fastainv_sym(A, I.F64, droptol=0.05)
+    r_hs = S.solve(A, b, M=M, tol=1e-10, precision=prec, method=S.PCG)
+    r_cg = S.solve(A, b, M=M, tol=1e-10, precision=prec, method=S.PCG1)
+    assert r_hs.converged and r_cg.converged
+    assert oracle_relres(A, b, r_cg.x) <= 1e-10
+    assert abs(r_cg.iterations - r_hs.iterations) <= max(3, r_hs.iterations // 20)
+
+
+@pytest.mark.parametrize("method", [S.PCG, S.PCG1])
+def test_repeated_solves_reuse_graph(gpu, method):
     """Graph replays across solves and eager work in between give identical results."""
     A = G.poisson7(16)
     M = I.fastainv_sym(A, I.F64, droptol=0.05)
-    sol = S.Solver(A, M=M, precision="mixed", tol=1e-10)
+    sol = S.Solver(A, M=M, precision="mixed", tol=1e-10, method=method)
     n = sol.n
     outs = []
     for k in range(3):
@@ -96,5 +113,6 @@ def test_repeated_solves_reuse_graph(gpu):
 def test_maxit_reports_not_converged(gpu):
     A = G.poisson7(16)
     b = O.csr_spmv(A, G.uniform(8, A.n_rows))
-    res = S.solve(A, b, M=None, tol=1e-12, precision="fp64", method=S.PCG, maxit=5, check_every=4)
-    assert not res.converged and res.iterations >= 5
+    for method in (S.PCG, S.PCG1):
+        res = S.solve(A, b, M=None, tol=1e-12, precision="fp64", method=method, maxit=5, check_every=4)
+        assert not res.converged and res.iterations >= 5
