@@ -123,7 +123,9 @@ def solve_leg(I, S, O, A, b, factors, precision, method, steps, warmup, tol):
     pre = solver.M_plans or ()
     return {"precision": precision,
             "method": {0: "pipelined-pcg", 1: "pcg", 2: "single-reduction-pcg"}[method],
-            "preconditioner": ("product Ainv2*Ainv1" if len(pre) == 1 else "factors Ainv1, Ainv2"),
+            "preconditioner": ("product Ainv2*Ainv1" if len(pre) == 1 else "factors Ainv1, Ainv2")
+                              + (", split-diagonal plans" if pre and pre[0].flags & I.SPMV_PLAN_SPLIT_DIAG
+                                 else ", SlELL plans"),
             "preconditioner_nnz_per_row": sum(p.info().device.n_nz for p in pre) / A.n_rows,
             "seconds": float(np.mean(times)), "seconds_min": float(np.min(times)),
             "iterations": stats.iterations, "replacements": stats.replacements,

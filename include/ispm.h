@@ -199,7 +199,14 @@ void ispm_slell_device_free(ispm_slell_desc *d);
 /* ------------------------------------------------------------- SpMV plan --------------- */
 
 enum ispm_plan_kind { ISPM_PLAN_DEFAULT = 0, ISPM_PLAN_ESTIMATE = 1, ISPM_PLAN_EXHAUSTIVE = 2 };
-enum ispm_plan_flags { ISPM_SPMV_NO_TEXTURE_MEMORY = 1, ISPM_SPMV_PLAN_NO_DEVICE = 2 };
+/* ISPM_SPMV_PLAN_SPLIT_DIAG (no reference counterpart): store a square matrix as its main
+ * diagonal (a dense vector) plus the rows that have off-diagonal entries (a compressed
+ * SlELL layout of those rows and a row map, streamed mapping); y = diag .* x, then those
+ * rows add their sums.  For approximate-inverse factors and their products, whose rows are
+ * mostly the diagonal alone.  Execution and the solver only (no SpMM, no coefficient
+ * refresh); plan info reports the remainder's layout with the full padded row count. */
+enum ispm_plan_flags { ISPM_SPMV_NO_TEXTURE_MEMORY = 1, ISPM_SPMV_PLAN_NO_DEVICE = 2,
+                       ISPM_SPMV_PLAN_SPLIT_DIAG = 4 };
 
 typedef struct ispm_plan ispm_plan;
 

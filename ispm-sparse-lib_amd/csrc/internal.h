@@ -43,11 +43,37 @@ int num_cus();
 
 size_t type_size(int t);
 
+/* Facts about a layout that plans compute once on the host and launches may use. */
+struct layout_hints {
+  /* streamed mapping: the largest element range (from its first element rounded down to a
+   * multiple of 8) of any group of 256*RPT consecutive rows, RPT = 1, 2, 4; 0 = unknown */
+  unsigned stream_span[3];
+};
+
 /* SpMV with the checks of ispm_spmv, an optional device skip flag (nothing is done while
- * *skip != 0) and a plan's shared diagonal pattern with its slice mask (both may be null). */
+ * *skip != 0), a plan's shared diagonal pattern with its slice mask (both may be null) and
+ * its layout facts (may be null). */
 int spmv_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, const void *x,
                   void *y, const unsigned *skip, const diag_pattern *pat = nullptr,
-                  const unsigned *uni = nullptr);
+                  const unsigned *uni = nullptr, const layout_hints *hints = nullptr);
+
+/* Split-diagonal form of a square matrix: y = diag .* x over all n rows, then the rows with
+ * off-diagonal entries (a SlELL layout R of m rows, streamed mapping) add their sums at
+ * y[rmap[i]]. */
+struct split_diag {
+  const void *diag;        /* n entries of elm_type (padded rows 0) */
+  const unsigned *rmap;    /* R row -> matrix row, ~0u for R's padding rows */
+  unsigned n;              /* rows (a multiple of 512) */
+  unsigned m;              /* rows of R with entries */
+  int elm_type;
+};
+
+/* SpMV of a split-diagonal matrix (R given by its launch and descriptor; may be empty), with
+ * the fused dot product as spmv_dot_internal (dparts/dgrid may be null). */
+int spmv_split(const split_diag &sd, const ispm_launch *p, const ispm_slell_desc *R, int vec_type,
+               const void *x, void *y, const unsigned *skip, const void *dotv = nullptr,
+               double *dparts = nullptr, unsigned *dgrid = nullptr,
+               const layout_hints *hints = nullptr);
 
 /* Partial sums a fused SpMV + dot may store (one per workgroup). */
 constexpr unsigned kMaxDotParts = 32768;
@@ -57,7 +83,8 @@ constexpr unsigned kMaxDotParts = 32768;
  * otherwise *dgrid is 0 and the caller computes the dot product itself. */
 int spmv_dot_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, const void *x,
                       void *y, const unsigned *skip, const diag_pattern *pat, const unsigned *uni,
-                      const void *dotv, double *dparts, unsigned *dgrid);
+                      const void *dotv, double *dparts, unsigned *dgrid,
+                      const layout_hints *hints = nullptr);
 
 }  // namespace ispm
 

@@ -274,8 +274,28 @@ static int upload_pattern(const conv::layout &L, diag_pattern &pat, unsigned *&d
   return ISPM_OK;
 }
 
+/* Streamed-mapping facts of a plain fixed-height layout: for groups of 256*RPT rows, the
+ * largest element range from the group's first element rounded down to a multiple of 8 (the
+ * kernel's LDS product buffer holds that range). */
+void compute_hints(const conv::layout &L, layout_hints &h)
+{
+  std::memset(&h, 0, sizeof h);
+  if (L.V || L.W != 1 || L.D || L.S == 0 || L.S > 256 || !L.n_slices || L.sp.size() < L.n_slices + 1)
+    return;
+  for (unsigned k = 0; k < 3; k++) {
+    const unsigned spg = (256u << k) / L.S;
+    unsigned span = 0;
+    for (unsigned s0 = 0; s0 < L.n_slices; s0 += spg) {
+      const unsigned s1 = std::min(L.n_slices, s0 + spg);
+      span = std::max(span, L.sp[s1] - (L.sp[s0] & ~7u));
+    }
+    h.stream_span[k] = span;
+  }
+}
+
 static int finish_plan(ispm_plan *p, const ispm_csr *A, unsigned flags)
 {
+  compute_hints(p->L, p->hints);
   p->elms.assign(p->L.n_elms * tsize(p->E) + 1, 0);
   if (fill_values(p->L, A, p->E, p->elms.data())) return fail_plain(ISPM_ERR_INVALID, "value map incomplete");
   p->stored = 0;
@@ -300,18 +320,19 @@ static int finish_plan(ispm_plan *p, const ispm_csr *A, unsigned flags)
 
 static int time_candidate(int vt, const ispm_slell_desc &d, const ispm_launch &lp, void *x,
                           void *y, void *x2, void *y2, double &sec,
-                          const diag_pattern *pat = nullptr, const unsigned *uni = nullptr)
+                          const diag_pattern *pat = nullptr, const unsigned *uni = nullptr,
+                          const layout_hints *hints = nullptr)
 {
   void *e0 = nullptr, *e1 = nullptr;
   int st;
   /* warm-up; also validity */
-  if ((st = spmv_internal(&lp, &d, vt, x, y, nullptr, pat, uni))) return st;
+  if ((st = spmv_internal(&lp, &d, vt, x, y, nullptr, pat, uni, hints))) return st;
   if ((st = ispm_event_create(&e0)) || (st = ispm_event_create(&e1))) return st;
   const int reps = 5;
   std::vector<float> t;
   for (int r = 0; r < reps; r++) {
     ispm_event_record(e0);
-    spmv_internal(&lp, &d, vt, (r & 1) ? x2 : x, (r & 1) ? y2 : y, nullptr, pat, uni);
+    spmv_internal(&lp, &d, vt, (r & 1) ? x2 : x, (r & 1) ? y2 : y, nullptr, pat, uni, hints);
     ispm_event_record(e1);
     ispm_event_synchronize(e1);
     float ms = 0;
@@ -394,12 +415,14 @@ int tune_exhaustive(const ispm_csr *A, int vt, int et, unsigned &bS, unsigned &b
           bool has_pat = false;
           if ((st = upload_pattern(L, pat, uni, has_pat))) { ispm_slell_device_free(&d); return st; }
           const diag_pattern *pp = has_pat ? &pat : nullptr;
+          layout_hints hints;
+          compute_hints(L, hints);
           /* packed (S < 64) and streamed (plain ELL planes) mappings */
           for (int mode = 1; mode <= 2; mode++) {
             if (V || (mode == 1 && S >= 64) || (mode == 2 && (W != 1 || D || S > 256))) continue;
             ispm_launch lp{256, S, 1, 0, 0, mode};
             double sec;
-            if (!time_candidate(vt, d, lp, x, y, x2, y2, sec) && sec < best) {
+            if (!time_candidate(vt, d, lp, x, y, x2, y2, sec, nullptr, nullptr, &hints) && sec < best) {
               best = sec;
               bS = S; bW = W; bV = V; bD = D;
               bl = lp;
@@ -410,7 +433,7 @@ int tune_exhaustive(const ispm_csr *A, int vt, int et, unsigned &bS, unsigned &b
               ispm_launch lp{N, V ? N : S, H, 0, 0, 0};
               if (!(lp.N >= lp.S / lp.H && lp.S >= lp.H)) continue;
               double sec;
-              if (time_candidate(vt, d, lp, x, y, x2, y2, sec, pp, uni)) continue;
+              if (time_candidate(vt, d, lp, x, y, x2, y2, sec, pp, uni, &hints)) continue;
               if (sec < best) {
                 best = sec;
                 bS = S; bW = W; bV = V; bD = D;
@@ -552,6 +575,83 @@ static void prefer_pattern_layout(const ispm_csr *A, int vt, int et, ispm_plan *
 }
 }  // namespace ispm
 
+namespace ispm {
+/* Split-diagonal plan: the main diagonal as a dense vector and the rows with off-diagonal
+ * entries as a compressed CSR converted to the plain fixed-height layout (S <= 32, streamed
+ * mapping) with the fewest stored bytes. */
+static int create_split(const ispm_csr *A, int et, unsigned flags, ispm_plan *p)
+{
+  if (A->n_rows != A->n_cols) return fail_plain(ISPM_ERR_INVALID, "split-diagonal plan: matrix must be square");
+  if (flags & ISPM_SPMV_PLAN_NO_DEVICE) return fail_plain(ISPM_ERR_INVALID, "split-diagonal plan needs the device");
+  const int n = A->n_rows;
+  const size_t n_pad = ((size_t)n + 511) / 512 * 512;
+  std::vector<double> dg(n_pad, 0.0);
+  std::vector<int> rp{0}, rc;
+  std::vector<double> rv;
+  std::vector<unsigned> rmap;
+  for (int i = 0; i < n; i++) {
+    const int k0 = A->row_ptr[i], k1 = A->row_ptr[i + 1];
+    for (int k = k0; k < k1; k++) {
+      const double v = csr_value(A, k);
+      if (A->cols[k] == i) dg[i] += v;
+      else { rc.push_back(A->cols[k]); rv.push_back(v); }
+    }
+    if ((int)rc.size() != rp.back()) { rp.push_back((int)rc.size()); rmap.push_back((unsigned)i); }
+  }
+  const unsigned m = (unsigned)rmap.size();
+  p->split = true;
+  p->split_n = (unsigned)n_pad;
+  p->split_m = m;
+  int st;
+  if (m) {
+    const ispm_csr R{(int)m, n, (int)rc.size(), rp.data(), rc.data(), rv.data(), ISPM_F64};
+    /* plain layout with the fewest stored bytes: 4 per slice record + (4 + E) per slot */
+    unsigned bS = 1;
+    size_t best = SIZE_MAX;
+    for (unsigned S = 1; S <= 32; S <<= 1) {
+      size_t slots = 0, slices = 0;
+      for (unsigned r0 = 0; r0 < m; r0 += S, slices++) {
+        int w = 0;
+        for (unsigned r = r0; r < std::min(m, r0 + S); r++) w = std::max(w, rp[r + 1] - rp[r]);
+        slots += (size_t)w * S;
+      }
+      const size_t b = 4 * slices + (4 + tsize(et)) * slots;
+      if (b < best) { best = b; bS = S; }
+    }
+    if ((st = make_layout(&R, et, bS, 1, 0, 0, p->L))) return st;
+    p->launch = default_launch(p->L);
+    if ((st = finish_plan(p, &R, flags))) return st;
+  } else {
+    std::memset(&p->dev, 0, sizeof p->dev);
+    p->dev.n_cols = n;
+    p->dev.elm_type = et;
+    p->L.n_cols = n;
+    p->L.S = 1;
+    p->launch = default_launch(p->L);
+    p->has_dev = true;
+    p->stored = 0;
+  }
+  /* the diagonal in E and the row map (padding rows of R map nowhere) */
+  std::vector<unsigned char> dh(n_pad * tsize(et));
+  for (size_t i = 0; i < n_pad; i++) store_value(dh.data(), et, i, dg[i]);
+  if ((st = ispm_malloc(&p->dev_diag, dh.size()))) return st;
+  if ((st = ispm_memcpy(p->dev_diag, dh.data(), dh.size(), ISPM_H2D))) return st;
+  const size_t m_pad = m ? (size_t)p->L.n_rows : 0;
+  if (m_pad) {
+    std::vector<unsigned> mh(m_pad, ~0u);
+    std::copy(rmap.begin(), rmap.end(), mh.begin());
+    if ((st = ispm_malloc((void **)&p->dev_rmap, 4 * m_pad))) return st;
+    if ((st = ispm_memcpy(p->dev_rmap, mh.data(), 4 * m_pad, ISPM_H2D))) return st;
+  }
+  /* bytes one product moves: diagonal, remainder layout, row map, x and y once */
+  const size_t rem = m ? 4 * p->dev.slice_ptr_len + 4 * p->dev.cols_len + tsize(et) * p->dev.elms_len : 0;
+  p->stored = n_pad * tsize(et) + rem + 4 * m_pad + tsize(p->T) * ((size_t)n + n_pad);
+  p->csr = csr_bytes(A, p->E, p->T);
+  p->dev.n_rows = (int)n_pad;   /* reported as the matrix's padded row count */
+  return ISPM_OK;
+}
+}  // namespace ispm
+
 int ispm_plan_create(ispm_plan **out, int vt, int et, const ispm_csr *A, int kind, unsigned flags)
 {
   if (!out) return fail_plain(ISPM_ERR_INVALID, "plan_create: null output");
@@ -564,6 +664,12 @@ int ispm_plan_create(ispm_plan **out, int vt, int et, const ispm_csr *A, int kin
   ispm_plan *p = new ispm_plan;
   p->T = vt;
   p->E = et;
+  if (flags & ISPM_SPMV_PLAN_SPLIT_DIAG) {
+    st = create_split(A, et, flags, p);
+    if (st) { ispm_plan_destroy(p); return st; }
+    *out = p;
+    return ISPM_OK;
+  }
   if (kind == ISPM_PLAN_EXHAUSTIVE) {
     if (flags & ISPM_SPMV_PLAN_NO_DEVICE) { delete p; return fail_plain(ISPM_ERR_INVALID, "exhaustive plan needs the device"); }
     unsigned S = 0, W = 1;
@@ -700,8 +806,10 @@ int ispm_plan_time_formats(int vt, int et, const ispm_csr *A, int count, const u
       bool has_pat = false;
       if (!(st = upload_pattern(L, pat, uni, has_pat))) {
         const ispm_launch lp = default_launch(L);
+        layout_hints hints;
+        compute_hints(L, hints);
         double sec;
-        if (!time_candidate(vt, d, lp, x, y, x2, y2, sec, has_pat ? &pat : nullptr, uni)) seconds[c] = sec;
+        if (!time_candidate(vt, d, lp, x, y, x2, y2, sec, has_pat ? &pat : nullptr, uni, &hints)) seconds[c] = sec;
       }
       ispm_slell_device_free(&d);
       if (uni) ispm_free(uni);
@@ -751,20 +859,26 @@ int ispm_plan_execute(ispm_plan *p, const void *x, void *y)
   if (!p->has_dev) return fail_plain(ISPM_ERR_INVALID, "plan_execute: plan has no device matrix");
   int st = ensure_init();
   if (st) return st;
+  if (p->split) {
+    const split_diag sd{p->dev_diag, p->dev_rmap, p->split_n, p->split_m, p->E};
+    return spmv_split(sd, &p->launch, &p->dev, p->T, x, y, nullptr, nullptr, nullptr, nullptr, &p->hints);
+  }
   return spmv_internal(&p->launch, &p->dev, p->T, x, y, nullptr, p->has_pat ? &p->pat : nullptr,
-                       p->dev_uni);
+                       p->dev_uni, &p->hints);
 }
 
 int ispm_plan_execute_multi(ispm_plan *p, int k, const void *X, size_t ldx, void *Y, size_t ldy)
 {
   if (!p) return fail_plain(ISPM_ERR_INVALID, "plan_execute_multi: null plan");
   if (!p->has_dev) return fail_plain(ISPM_ERR_INVALID, "plan_execute_multi: plan has no device matrix");
+  if (p->split) return fail_plain(ISPM_ERR_UNSUPPORTED, "plan_execute_multi: split-diagonal plans multiply one vector");
   return ispm_spmm(&p->launch, &p->dev, p->T, k, X, ldx, Y, ldy);
 }
 
 int ispm_plan_update_coefs(ispm_plan *p, const ispm_csr *A)
 {
   if (!p) return fail_plain(ISPM_ERR_INVALID, "plan_update_coefs: null plan");
+  if (p->split) return fail_plain(ISPM_ERR_UNSUPPORTED, "plan_update_coefs: split-diagonal plans are rebuilt");
   int st = check_csr(A);
   if (st) return st;
   if ((long long)A->row_ptr[A->n_rows] != (long long)p->L.map.size())
@@ -779,6 +893,8 @@ void ispm_plan_destroy(ispm_plan *p)
 {
   if (!p) return;
   if (p->has_dev) ispm_slell_device_free(&p->dev);
+  if (p->dev_diag) ispm_free(p->dev_diag);
+  if (p->dev_rmap) ispm_free(p->dev_rmap);
   if (p->dev_map) ispm_free(p->dev_map);
   if (p->dev_uni) ispm_free(p->dev_uni);
   delete p;

@@ -6,6 +6,7 @@
 #include "convert.h"
 #include "diag_pattern.h"
 #include "ispm.h"
+#include "internal.h"
 
 struct ispm_plan {
   int T = ISPM_F64, E = ISPM_F64;         /* vector and element types */
@@ -21,7 +22,16 @@ struct ispm_plan {
   ispm::diag_pattern pat{};
   unsigned *dev_uni = nullptr;            /* device bit mask of the slices that have it */
   unsigned pat_slices = 0;
+  /* split-diagonal plan (ISPM_SPMV_PLAN_SPLIT_DIAG): L / dev / launch describe the remainder */
+  bool split = false;
+  void *dev_diag = nullptr;               /* split_n entries of E */
+  unsigned *dev_rmap = nullptr;           /* remainder row -> matrix row (~0u: padding) */
+  unsigned split_n = 0, split_m = 0;      /* padded rows of the matrix; remainder rows */
+  ispm::layout_hints hints{};             /* launch facts of the layout (streamed mapping) */
 };
+
+/* rows (padded) of the matrix a plan multiplies */
+inline unsigned plan_rows(const ispm_plan *p) { return p->split ? p->split_n : (unsigned)p->L.n_rows; }
 
 namespace ispm {
 size_t tsize(int t);
@@ -30,6 +40,7 @@ int fill_values(const conv::layout &L, const ispm_csr *A, int et, void *elms);
 int upload_layout(const conv::layout &L, const void *elms, int et, ispm_slell_desc *d);
 ispm_launch default_launch(const conv::layout &L);
 bool build_pattern(const conv::layout &L, diag_pattern &pat, std::vector<unsigned> &mask);
+void compute_hints(const conv::layout &L, layout_hints &h);
 }
 
 #endif

@@ -374,7 +374,12 @@ int alloc(void **p, size_t bytes) { return ispm_malloc(p, bytes); }
 
 int spmv(ispm_plan *P, int vt, const void *x, void *y, const unsigned *skip)
 {
-  return spmv_internal(&P->launch, &P->dev, vt, x, y, skip, P->has_pat ? &P->pat : nullptr, P->dev_uni);
+  if (P->split) {
+    const split_diag sd{P->dev_diag, P->dev_rmap, P->split_n, P->split_m, P->E};
+    return spmv_split(sd, &P->launch, &P->dev, vt, x, y, skip, nullptr, nullptr, nullptr, &P->hints);
+  }
+  return spmv_internal(&P->launch, &P->dev, vt, x, y, skip, P->has_pat ? &P->pat : nullptr, P->dev_uni,
+                       &P->hints);
 }
 
 /* G partials of the reduction before (the solver's grid when 0) */
@@ -432,8 +437,14 @@ int spmv_dot(ispm_solver *S, ispm_plan *P, const void *x, void *y, const void *d
 {
   G = 0;
   if (!parts) parts = S->parts;
-  int st = spmv_dot_internal(&P->launch, &P->dev, S->T, x, y, skip, P->has_pat ? &P->pat : nullptr,
-                             P->dev_uni, d, parts, &G);
+  int st;
+  if (P->split) {
+    const split_diag sd{P->dev_diag, P->dev_rmap, P->split_n, P->split_m, P->E};
+    st = spmv_split(sd, &P->launch, &P->dev, S->T, x, y, skip, d, parts, &G, &P->hints);
+  } else {
+    st = spmv_dot_internal(&P->launch, &P->dev, S->T, x, y, skip, P->has_pat ? &P->pat : nullptr,
+                           P->dev_uni, d, parts, &G, &P->hints);
+  }
   if (st || G) return st;
   hipLaunchKernelGGL((k_dot<T, OP_PQ>), dim3(S->grid), dim3(256), 0, stream(), (const T *)d, (const T *)y,
                      S->n, S->s, parts, skip ? 1 : 0);
@@ -618,7 +629,7 @@ int ispm_solver_create(ispm_solver **out, ispm_plan *A, ispm_plan *M1, ispm_plan
     if (!M) continue;
     if (!M->has_dev || M->T != T)
       return fail(ISPM_ERR_INVALID, "solver_create: preconditioner plans must be on the device with vector type %s", mixed ? "F32" : "F64");
-    if (M->L.n_rows != A->L.n_rows) return fail(ISPM_ERR_INVALID, "solver_create: preconditioner size differs from A");
+    if (plan_rows(M) != plan_rows(A)) return fail(ISPM_ERR_INVALID, "solver_create: preconditioner size differs from A");
   }
   ispm_solver *S = new ispm_solver;
   S->A = A; S->M1 = M1; S->M2 = M2;
@@ -634,7 +645,7 @@ int ispm_solver_create(ispm_solver **out, ispm_plan *A, ispm_plan *M1, ispm_plan
   S->single = o->method == ISPM_SOLVER_PCG1;
   S->T = T;
   S->ts = mixed ? 4 : 8;
-  S->n = A->L.n_rows;
+  S->n = (int)plan_rows(A);
   const size_t vb = S->ts * (size_t)S->n + 64;
   /* one 16-byte vector access per thread where possible (the vector kernels are latency
    * bound with fewer, longer grid-stride loops) */
@@ -809,7 +820,7 @@ int ispm_solver_solve(ispm_solver *S, const double *b, double *x, ispm_solve_sta
 int ispm_solver_set_residual_plan(ispm_solver *S, ispm_plan *A64)
 {
   if (!S) return fail(ISPM_ERR_INVALID, "solver_set_residual_plan: null solver");
-  if (A64 && (!A64->has_dev || A64->E != ISPM_F64 || A64->L.n_rows != S->n))
+  if (A64 && (!A64->has_dev || A64->E != ISPM_F64 || (int)plan_rows(A64) != S->n))
     return fail(ISPM_ERR_INVALID, "solver_set_residual_plan: need an FP64 device plan of the same size");
   S->Ares = A64;
   return ISPM_OK;

@@ -55,6 +55,9 @@ struct spmv_args {
   const unsigned *skip;  /* if non-null and *skip != 0 the launch does nothing (solver done flag) */
   const void *dotv = nullptr;   /* fused dot: workgroup w stores sum_rows dotv[row]*y[row] */
   double *dparts = nullptr;     /* at dparts[w] (kernels that support it) */
+  /* streamed mapping only: layout row i is matrix row rmap[i] (~0u: none) and its sum is
+   * added to y there (split-diagonal plans: the rows with off-diagonal entries) */
+  const unsigned *rmap = nullptr;
 };
 
 template<typename T> __device__ __forceinline__ T xload(const T *x, unsigned i, unsigned n)
@@ -602,19 +605,22 @@ __global__ void __launch_bounds__(256) slell_packed_kernel(spmv_args a)
  * planes.  A 256-thread workgroup takes 256*RPT consecutive rows, whose elements form one
  * contiguous range of the element and column arrays.  Phase 1 reads that range with
  * consecutive lanes on consecutive elements (every fetched line used in full by one
- * instruction), eight elements per thread in flight, and leaves the products e*x[col] in LDS
- * in storage order; phase 2 gives each thread RPT rows and adds each row's products plane
- * by plane, in storage order.  A workgroup whose range exceeds the LDS capacity reads its
- * rows directly.  Suited to irregular, short rows (approximate-inverse factors): the work of
- * a workgroup is three dependent memory round trips (offsets, elements and columns, x), so
- * larger groups keep more bytes in flight per round trip. */
+ * instruction), up to eight elements per thread in flight, and leaves the products
+ * e*x[col] in LDS in storage order; phase 2 gives each thread RPT rows and adds each row's
+ * products plane by plane, in storage order.  The LDS product buffer holds cap elements:
+ * plans size it from their layout's largest group, so every group takes this path; a group
+ * whose range exceeds it (launches without layout facts) reads its rows directly, one
+ * dependent gather per element.  Suited to irregular, short rows (approximate-inverse
+ * factors): the work of a workgroup is three dependent memory round trips (offsets,
+ * elements and columns, x). */
 template<typename T, typename E, int RPT>
-__global__ void __launch_bounds__(256) slell_stream_kernel(spmv_args a)
+__global__ void __launch_bounds__(256) slell_stream_kernel(spmv_args a, unsigned cap)
 {
-  constexpr unsigned ROWS = 256u * RPT, CAP = (sizeof(T) == 4 ? 1024u : 2048u) * RPT, U = 4,
-                     VEC = dev::evec<E>::n;
-  __shared__ __align__(16) T prod[CAP];
-  __shared__ unsigned soff[ROWS + 1];
+  constexpr unsigned ROWS = 256u * RPT, U = 4, VEC = dev::evec<E>::n;
+  /* products (cap elements, sized by the launch), then the group's slice offsets */
+  extern __shared__ __align__(16) unsigned char stream_lds[];
+  T *prod = reinterpret_cast<T *>(stream_lds);
+  unsigned *soff = reinterpret_cast<unsigned *>(prod + cap);
   if (a.skip && *a.skip) return;
   const unsigned log2S = a.log2S, S = 1u << log2S;
   const unsigned spg = ROWS >> log2S;
@@ -623,6 +629,7 @@ __global__ void __launch_bounds__(256) slell_stream_kernel(spmv_args a)
   const unsigned *__restrict__ cl = a.cols;
   const T *__restrict__ xv = static_cast<const T *>(a.x);
   T *__restrict__ yv = static_cast<T *>(a.y);
+  const T *__restrict__ dv = static_cast<const T *>(a.dotv);
   const unsigned ncols = a.n_cols;
   const unsigned group = (blockIdx.x & 7u) * a.per_xcd + (blockIdx.x >> 3);
   const unsigned s0 = group * spg;
@@ -631,15 +638,32 @@ __global__ void __launch_bounds__(256) slell_stream_kernel(spmv_args a)
     return;
   }
   const unsigned ns = min(spg, a.n_slices - s0);
+  /* Destination rows (~0u: none), and what the epilogue reads there (the row map's entry,
+   * y to add to, the dot product's vector): requested first, as they do not depend on the
+   * layout's offsets, so their latency overlaps the element and x loads. */
+  unsigned dst[RPT];
+  T yold[RPT], dvv[RPT];
+#pragma unroll
+  for (int k = 0; k < RPT; k++) {
+    const unsigned row = t + k * 256u;
+    dst[k] = (row >> log2S) < ns ? (s0 << log2S) + row : ~0u;
+    if (a.rmap && dst[k] != ~0u) dst[k] = a.rmap[dst[k]];
+    yold[k] = 0;
+    dvv[k] = 0;
+    if (dst[k] != ~0u) {
+      if (a.rmap) yold[k] = yv[dst[k]];
+      if (a.dparts) dvv[k] = dv[dst[k]];
+    }
+  }
   /* the element range comes from two (scalar) loads, so the element loads need not wait for
    * the offsets table, which only phase 2 reads */
   const unsigned E0 = a.sp[s0], E1 = a.sp[s0 + ns], n = E1 - E0;
-  for (unsigned q = t; q <= ns; q += 256) soff[q] = a.sp[s0 + q];   /* ns + 1 <= ROWS + 1 */
+  for (unsigned q = t; q <= ns; q += 256) soff[q] = a.sp[s0 + q];   /* ns + 1 <= spg + 1 */
   /* products are stored at prod[e - base], base = E0 rounded down to VEC elements, so the
    * VEC-aligned body [A0, A1) of the range is loaded and stored as whole vectors; the head
    * [E0, min(A0, E1)) and tail [A1, E1) (fewer than VEC elements each) one by one */
   const unsigned base = E0 & ~(VEC - 1);
-  const bool fits = n + (E0 - base) <= CAP;
+  const bool fits = n + (E0 - base) <= cap;
   T acc[RPT];
 #pragma unroll
   for (int k = 0; k < RPT; k++) acc[k] = 0;
@@ -656,22 +680,34 @@ __global__ void __launch_bounds__(256) slell_stream_kernel(spmv_args a)
       prod[e - base] = elt<T, E>::get(el, e) * xload(xv, dev::col_load(cl + e), ncols);
     }
     const unsigned nv = (A1 - A0) / VEC;
-    for (unsigned v0 = t; v0 < nv; v0 += 256 * U) {
-      T ev[U][VEC];
+    /* rounds of 256 vectors this workgroup needs (uniform), up to U per step: the element
+     * and column loads of all rounds of a step, then all their x gathers, are issued before
+     * the first wait; a thread past the end of a round repeats its first vector (in range)
+     * and stores nothing for it */
+    const unsigned rounds = (nv + 255) / 256;
+    for (unsigned r0 = 0; r0 < rounds; r0 += U) {
+      const unsigned v0 = t + r0 * 256;
+      if (v0 >= nv) break;
+      T ev[U][VEC], xg[U][VEC];
       unsigned cv[U][VEC];
 #pragma unroll
       for (unsigned u = 0; u < U; u++)
-        if (v0 + u * 256 < nv) {
-          const unsigned e = A0 + (v0 + u * 256) * VEC;
-          dev::elt_vec<T>(el, e, ev[u]);
-          dev::col_vec(cl, e, cv[u]);
+        if (r0 + u < rounds) {
+          const unsigned vi = v0 + u * 256 < nv ? v0 + u * 256 : v0;
+          dev::elt_vec<T>(el, A0 + vi * VEC, ev[u]);
+          dev::col_vec(cl, A0 + vi * VEC, cv[u]);
         }
 #pragma unroll
       for (unsigned u = 0; u < U; u++)
-        if (v0 + u * 256 < nv) {
+        if (r0 + u < rounds)
+#pragma unroll
+          for (unsigned q = 0; q < VEC; q++) xg[u][q] = xload(xv, cv[u][q], ncols);
+#pragma unroll
+      for (unsigned u = 0; u < U; u++)
+        if (r0 + u < rounds && v0 + u * 256 < nv) {
           const unsigned o = A0 - base + (v0 + u * 256) * VEC;
 #pragma unroll
-          for (unsigned q = 0; q < VEC; q++) prod[o + q] = ev[u][q] * xload(xv, cv[u][q], ncols);
+          for (unsigned q = 0; q < VEC; q++) prod[o + q] = ev[u][q] * xg[u][q];
         }
     }
   }
@@ -699,29 +735,50 @@ __global__ void __launch_bounds__(256) slell_stream_kernel(spmv_args a)
     }
   }
 #pragma unroll
-  for (int k = 0; k < RPT; k++) {
-    const unsigned row = t + k * 256u;
-    if ((row >> log2S) < ns) yv[(s0 << log2S) + row] = acc[k];
-  }
+  for (int k = 0; k < RPT; k++)
+    if (dst[k] != ~0u) yv[dst[k]] = a.rmap ? yold[k] + acc[k] : acc[k];
   if (a.dparts) {
     __shared__ double red[4];
-    const T *__restrict__ dv = static_cast<const T *>(a.dotv);
     double d[1] = {0.0};
 #pragma unroll
-    for (int k = 0; k < RPT; k++) {
-      const unsigned row = t + k * 256u;
-      if ((row >> log2S) < ns) d[0] += (double)dv[(s0 << log2S) + row] * (double)acc[k];
-    }
+    for (int k = 0; k < RPT; k++)
+      if (dst[k] != ~0u) d[0] += (double)dvv[k] * (double)acc[k];
     dev::block_sum<256, 1>(d, red);
     if (t == 0) a.dparts[blockIdx.x] = d[0];
   }
 }
 
+/* LDS bytes of a streamed-mapping workgroup with a product buffer of cap elements */
+template<typename T> constexpr size_t stream_lds_bytes(unsigned cap, unsigned rpt, unsigned log2S)
+{
+  return ((size_t)cap * sizeof(T) + 4 * (size_t)(((256u * rpt) >> log2S) + 1) + 15) / 16 * 16;
+}
+
+/* Rows per thread (one for FP64 vectors, two for FP32; ISPM_STREAM_RPT overrides) and the
+ * product capacity: the layout's largest group span when known (64 KiB of LDS at most),
+ * else 8 (FP64) or 4 (FP32) products per row. */
+template<typename T>
+void stream_shape(const layout_hints *h, unsigned log2S, unsigned &rpt, unsigned &cap)
+{
+  rpt = sizeof(T) == 8 ? 1u : 2u;
+  if (const char *e = std::getenv("ISPM_STREAM_RPT")) {
+    const unsigned r = (unsigned)std::atoi(e);
+    if ((r == 1 || r == 2 || r == 4) && (256u * r) >= (1u << log2S)) rpt = r;
+  }
+  const unsigned k = rpt == 1 ? 0u : rpt == 2 ? 1u : 2u;
+  cap = h && h->stream_span[k] ? (h->stream_span[k] + 3) & ~3u : (sizeof(T) == 4 ? 1024u : 2048u) * rpt;
+  const size_t soff = 4 * (size_t)(((256u * rpt) >> log2S) + 1) + 16;
+  if ((size_t)cap * sizeof(T) + soff > 65536) cap = (unsigned)((65536 - soff) / sizeof(T)) & ~3u;
+}
+
 template<typename T, typename E>
 int launch_stream(const ispm_slell_desc *A, const void *x, void *y, const unsigned *skip,
-                  const void *dotv = nullptr, double *dparts = nullptr, unsigned *dgrid = nullptr)
+                  const void *dotv = nullptr, double *dparts = nullptr, unsigned *dgrid = nullptr,
+                  const unsigned *rmap = nullptr, unsigned max_parts = kMaxDotParts,
+                  const layout_hints *hints = nullptr)
 {
   spmv_args a;
+  a.rmap = rmap;
   a.skip = skip;
   a.sp = A->slice_ptr;
   a.cols = A->cols;
@@ -733,37 +790,100 @@ int launch_stream(const ispm_slell_desc *A, const void *x, void *y, const unsign
   a.log2S = (unsigned)(31 - __builtin_clz(A->slice_height));
   a.n_cols = (unsigned)A->n_cols;
   a.spb = 1;
-  /* rows per thread: LDS per workgroup grows with it and limits occupancy; measured best
-   * on approximate-inverse factors: one for FP64 vectors, two for FP32 */
-  unsigned rpt = sizeof(T) == 8 ? 1u : 2u;
-  if (const char *e = std::getenv("ISPM_STREAM_RPT")) rpt = (unsigned)std::atoi(e);
-  if (rpt != 1 && rpt != 2 && rpt != 4) rpt = sizeof(T) == 8 ? 1u : 2u;
+  unsigned rpt, cap;
+  stream_shape<T>(hints, a.log2S, rpt, cap);
+  const size_t lds = stream_lds_bytes<T>(cap, rpt, a.log2S);
   const unsigned spg = (256u * rpt) >> a.log2S;
   const unsigned groups = (A->n_slices + spg - 1) / spg;
   a.per_xcd = (groups + 7) / 8;
-  if (dparts && dgrid && a.per_xcd * 8 <= kMaxDotParts) {
+  if (dparts && dgrid && a.per_xcd * 8 <= max_parts) {
     a.dotv = dotv;
     a.dparts = dparts;
     *dgrid = a.per_xcd * 8;
   }
   const dim3 g(a.per_xcd * 8), b(256);
-  if (rpt == 1) hipLaunchKernelGGL((slell_stream_kernel<T, E, 1>), g, b, 0, stream(), a);
-  else if (rpt == 2) hipLaunchKernelGGL((slell_stream_kernel<T, E, 2>), g, b, 0, stream(), a);
-  else hipLaunchKernelGGL((slell_stream_kernel<T, E, 4>), g, b, 0, stream(), a);
+  if (rpt == 1) hipLaunchKernelGGL((slell_stream_kernel<T, E, 1>), g, b, lds, stream(), a, cap);
+  else if (rpt == 2) hipLaunchKernelGGL((slell_stream_kernel<T, E, 2>), g, b, lds, stream(), a, cap);
+  else hipLaunchKernelGGL((slell_stream_kernel<T, E, 4>), g, b, lds, stream(), a, cap);
   ISPM_HIP(hipGetLastError());
   return ISPM_OK;
 }
 
 inline int launch_stream_types(const ispm_slell_desc *A, int vt, const void *x, void *y,
                                const unsigned *skip, const void *dotv = nullptr,
-                               double *dparts = nullptr, unsigned *dgrid = nullptr)
+                               double *dparts = nullptr, unsigned *dgrid = nullptr,
+                               const unsigned *rmap = nullptr, unsigned max_parts = kMaxDotParts,
+                               const layout_hints *h = nullptr)
 {
   const int et = A->elm_type;
-  if (vt == ISPM_F64 && et == ISPM_F64) return launch_stream<double, double>(A, x, y, skip, dotv, dparts, dgrid);
-  if (vt == ISPM_F64 && et == ISPM_F32) return launch_stream<double, float>(A, x, y, skip, dotv, dparts, dgrid);
-  if (vt == ISPM_F32 && et == ISPM_F32) return launch_stream<float, float>(A, x, y, skip, dotv, dparts, dgrid);
-  if (vt == ISPM_F32 && et == ISPM_BF16) return launch_stream<float, bf16_t>(A, x, y, skip, dotv, dparts, dgrid);
+  if (vt == ISPM_F64 && et == ISPM_F64) return launch_stream<double, double>(A, x, y, skip, dotv, dparts, dgrid, rmap, max_parts, h);
+  if (vt == ISPM_F64 && et == ISPM_F32) return launch_stream<double, float>(A, x, y, skip, dotv, dparts, dgrid, rmap, max_parts, h);
+  if (vt == ISPM_F32 && et == ISPM_F32) return launch_stream<float, float>(A, x, y, skip, dotv, dparts, dgrid, rmap, max_parts, h);
+  if (vt == ISPM_F32 && et == ISPM_BF16) return launch_stream<float, bf16_t>(A, x, y, skip, dotv, dparts, dgrid, rmap, max_parts, h);
   return fail(ISPM_ERR_UNSUPPORTED, "spmv: unsupported (vector, element) types (%d, %d)", vt, et);
+}
+
+/* Diagonal part of a split-diagonal product: y = d .* x over n rows (n a multiple of the
+ * vector width), with the block partial of (dotv, y) at dparts[block] when asked.  Each
+ * thread takes VEC consecutive rows per step: one vector load of d (nontemporal, read
+ * once), vector loads of x (and dotv) and a vector store of y when VL (x, y, dotv aligned). */
+template<typename T, typename E, bool VL>
+__global__ void __launch_bounds__(256)
+split_diag_kernel(const E *__restrict__ d, const T *__restrict__ x, T *__restrict__ y, unsigned n,
+                  const unsigned *skip, const T *__restrict__ dv, double *dparts)
+{
+  __shared__ double red[4];
+  if (skip && *skip) return;
+  constexpr unsigned V = dev::evec<E>::n;
+  typedef T tv __attribute__((ext_vector_type(V)));
+  double acc[1] = {0.0};
+  const unsigned nv = n / V;
+  for (unsigned i = blockIdx.x * 256u + threadIdx.x; i < nv; i += gridDim.x * 256u) {
+    T dd[V], xx[V], yy[V];
+    dev::elt_vec<T>(d, i * V, dd);
+    if (VL) {
+      const tv xv = *reinterpret_cast<const tv *>(x + (size_t)i * V);
+#pragma unroll
+      for (unsigned q = 0; q < V; q++) xx[q] = xv[q];
+    } else {
+#pragma unroll
+      for (unsigned q = 0; q < V; q++) xx[q] = x[(size_t)i * V + q];
+    }
+    tv yv;
+#pragma unroll
+    for (unsigned q = 0; q < V; q++) { yy[q] = dd[q] * xx[q]; yv[q] = yy[q]; }
+    if (VL) *reinterpret_cast<tv *>(y + (size_t)i * V) = yv;
+    else {
+#pragma unroll
+      for (unsigned q = 0; q < V; q++) y[(size_t)i * V + q] = yy[q];
+    }
+    if (dparts) {
+#pragma unroll
+      for (unsigned q = 0; q < V; q++) acc[0] += (double)dv[(size_t)i * V + q] * (double)yy[q];
+    }
+  }
+  if (dparts) {
+    dev::block_sum<256, 1>(acc, red);
+    if (threadIdx.x == 0) dparts[blockIdx.x] = acc[0];
+  }
+}
+
+template<typename T, typename E>
+int launch_split_diag(const void *d, const void *x, void *y, unsigned n, const unsigned *skip,
+                      const void *dotv, double *dparts, unsigned grid)
+{
+  constexpr unsigned V = dev::evec<E>::n;
+  const uintptr_t al = reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(y) |
+                       (dparts ? reinterpret_cast<uintptr_t>(dotv) : 0);
+  const bool vl = al % (V * sizeof(T)) == 0;
+  if (vl)
+    hipLaunchKernelGGL((split_diag_kernel<T, E, true>), dim3(grid), dim3(256), 0, stream(), static_cast<const E *>(d),
+                       static_cast<const T *>(x), static_cast<T *>(y), n, skip, static_cast<const T *>(dotv), dparts);
+  else
+    hipLaunchKernelGGL((split_diag_kernel<T, E, false>), dim3(grid), dim3(256), 0, stream(), static_cast<const E *>(d),
+                       static_cast<const T *>(x), static_cast<T *>(y), n, skip, static_cast<const T *>(dotv), dparts);
+  ISPM_HIP(hipGetLastError());
+  return ISPM_OK;
 }
 
 template<typename T, typename E>

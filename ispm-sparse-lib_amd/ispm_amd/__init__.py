@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(PKG_DIR, "libispm0.so")
 F32, F64, BF16 = 0, 1, 2
 H2D, D2H, D2D, DEFAULT = 0, 1, 2, 3
 PLAN_DEFAULT, PLAN_ESTIMATE, PLAN_EXHAUSTIVE = 0, 1, 2
-SPMV_NO_TEXTURE_MEMORY, SPMV_PLAN_NO_DEVICE = 1, 2
+SPMV_NO_TEXTURE_MEMORY, SPMV_PLAN_NO_DEVICE, SPMV_PLAN_SPLIT_DIAG = 1, 2, 4
 
 _NP = {F32: np.float32, F64: np.float64, BF16: np.uint16}
 _TSIZE = {F32: 4, F64: 8, BF16: 2}
@@ -421,6 +421,7 @@ class Plan:
             check(lib().ispm_plan_create(C.byref(p), vec_type, elm_type, C.byref(A.c()), kind, flags))
         self.p = p
         self.vec_type = vec_type
+        self.flags = flags
         self.n_rows_padded = self.info().device.n_rows
 
     def info(self):

@@ -6,7 +6,8 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import (BF16, CSR, F32, F64, PLAN_ESTIMATE, DeviceArray, Plan, check, csr_multiply, lib)
+from . import (BF16, CSR, F32, F64, PLAN_ESTIMATE, SPMV_PLAN_SPLIT_DIAG, DeviceArray, Plan, check,
+               csr_multiply, lib)
 
 PPCG, PCG, PCG1 = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
@@ -87,11 +88,13 @@ class Solver:
     precision="fp64": A in FP64, preconditioner in FP64, FP64 vectors.  precision="mixed": A
     in FP32, preconditioner in BF16 (factor_type), FP32 vectors, FP64 scalars, reliable FP64
     residual replacement.  M: (Ainv1, Ainv2) from fastainv_sym, applied as Ainv2 (Ainv1 r) or
-    as their product (see preconditioner_matrices; fsai_product chooses), or one CSR matrix."""
+    as their product (see preconditioner_matrices; fsai_product chooses), or one CSR matrix;
+    each matrix as a SlELL plan or a split-diagonal plan (split_diag chooses; None: the
+    faster, timed at set-up)."""
 
     def __init__(self, A: CSR, M=None, precision="fp64", method=None, tol=1e-10, maxit=100000,
                  check_every=16, use_graph=True, reliable_delta=0.01, factor_type=None,
-                 A_plan=None, M_plans=None, fsai_product=None):
+                 A_plan=None, M_plans=None, fsai_product=None, split_diag=None):
         L = _bind()
         mixed = precision == "mixed"
         vt = F32 if mixed else F64
@@ -103,22 +106,30 @@ class Solver:
             self.M_plans = M_plans
         elif M is not None:
             ft = factor_type if factor_type is not None else (BF16 if mixed else F64)
-            # preconditioner rows are irregular: take the byte-minimal format
-            def plans(mats):
+            # preconditioner rows are irregular: take the byte-minimal format, and also the
+            # split-diagonal form (diagonal vector + the rows with off-diagonal entries)
+            def plans(mats, split=False):
+                if split:
+                    return tuple(Plan(m, vt, elm_type=ft, flags=SPMV_PLAN_SPLIT_DIAG) for m in mats)
                 return tuple(Plan(m, vt, elm_type=ft, kind=PLAN_ESTIMATE) for m in mats)
+            n = self.A_plan.n_rows_padded
             if fsai_product is None and not isinstance(M, CSR):
-                # two factors or their product: whichever applies faster here (the product
-                # has fewer nonzeros but its off-diagonal entries gather x from further away)
+                # two factors or their product, each as SlELL or split: whichever applies
+                # fastest here (the product has fewer nonzeros but its off-diagonal entries
+                # gather x from further away)
                 a1, a2 = M
                 P = csr_multiply(a2, a1, F64)
-                self.M_plans = plans((a1, a2))
-                if P.nnz < a1.nnz + a2.nnz:
-                    cand = plans((P,))
-                    n = self.A_plan.n_rows_padded
-                    if apply_seconds(cand, vt, n) < apply_seconds(self.M_plans, vt, n):
-                        self.M_plans = cand
+                forms = [(a1, a2)] + ([(P,)] if P.nnz < a1.nnz + a2.nnz else [])
             else:
-                self.M_plans = plans(preconditioner_matrices(M, fsai_product))
+                forms = [preconditioner_matrices(M, fsai_product)]
+            best = None
+            for mats in forms:
+                for split in ((False, True) if split_diag is None else (split_diag,)):
+                    cand = plans(mats, split)
+                    t = apply_seconds(cand, vt, n) if len(forms) > 1 or split_diag is None else 0.0
+                    if best is None or t < best[0]:
+                        best = (t, cand)
+            self.M_plans = best[1]
         self.n = self.A_plan.n_rows_padded
         o = SolverOpts(method, PREC_MIXED if mixed else PREC_FP64, tol, maxit, check_every,
                        int(use_graph), reliable_delta)

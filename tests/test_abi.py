@@ -36,3 +36,15 @@ def test_cpu_only_calls():
     A = I.CSR(2, 2, [0, 1, 2], [0, 1], [1.0, 2.0])
     hs = I.csr_to_slell(A, 32)
     assert hs.n_rows == 512 and hs.n_slices == 16
+
+
+def test_split_diagonal_plan_argument_checks():
+    """Split-diagonal plans need a square matrix and the device; both are checked before any
+    device work."""
+    import pytest
+    A = I.CSR(2, 3, [0, 1, 2], [0, 2], [1.0, 2.0])
+    with pytest.raises(I.IspmError):
+        I.Plan(A, I.F64, flags=I.SPMV_PLAN_SPLIT_DIAG)
+    B = I.CSR(2, 2, [0, 1, 2], [0, 1], [1.0, 2.0])
+    with pytest.raises(I.IspmError):
+        I.Plan(B, I.F64, flags=I.SPMV_PLAN_SPLIT_DIAG | I.SPMV_PLAN_NO_DEVICE)

@@ -44,6 +44,22 @@ def test_contrast_stencil(gpu, prec, method, product):
     assert res.converged and oracle_relres(A, b, res.x) <= 1e-10
 
 
+@pytest.mark.parametrize("product", [False, True], ids=["factors", "product"])
+@pytest.mark.parametrize("prec,method", VARIANTS)
+def test_split_diagonal_preconditioner(gpu, prec, method, product):
+    """Preconditioner plans in the split-diagonal form (the product's and the factors' rows are
+    mostly their diagonal alone): the fused dot products take both parts' partial sums."""
+    A = G.stencil27_contrast(32)
+    b = O.csr_spmv(A, G.uniform(2, A.n_rows))
+    M = I.fastainv_sym(A, I.F64, droptol=0.125)
+    ref = S.solve(A, b, M=M, tol=1e-10, precision=prec, method=method, fsai_product=product,
+                  split_diag=False)
+    res = S.solve(A, b, M=M, tol=1e-10, precision=prec, method=method, fsai_product=product,
+                  split_diag=True)
+    assert res.converged and oracle_relres(A, b, res.x) <= 1e-10
+    assert abs(res.iterations - ref.iterations) <= max(3, ref.iterations // 20)
+
+
 @pytest.mark.parametrize("prec,method", VARIANTS)
 def test_no_preconditioner_and_nonzero_guess(gpu, prec, method):
     A = G.random_sparse(3000, 41, max_len=12, spd=True)

@@ -228,6 +228,70 @@ def test_spmm_matches_spmv(mats):
                 np.testing.assert_array_equal(Y[j], yd.numpy())
 
 
+def mostly_diagonal(n=5000, seed=16):
+    """Rows holding only their diagonal entry except every 23rd row (1-40 off-diagonal entries
+    on both sides); row 7 has no diagonal entry and row 11 a zero one."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    for i in range(n):
+        c = {i} if i != 7 else set()
+        if i % 23 == 0:
+            c |= set(rng.choice(n, rng.integers(1, 41), replace=False).tolist()) - {i}
+        rows.append(sorted(c))
+    rp = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int64)
+    cols = np.concatenate([np.array(r, np.int32) for r in rows])
+    vals = rng.uniform(-1, 1, rp[-1])
+    vals[rp[11]:rp[12]][np.array(rows[11]) == 11] = 0.0
+    return I.CSR(n, n, rp, cols, vals)
+
+
+def run_plan(p, A, x, vt):
+    n = p.n_rows_padded
+    dt = np.float64 if vt == I.F64 else np.float32
+    xd = I.DeviceArray.from_numpy(np.concatenate([x, np.zeros(n - A.n_rows)]).astype(dt))
+    yd = I.DeviceArray(n, dt)
+    yd.upload(np.full(n, 7.0, dt))
+    p.execute(xd, yd)
+    return yd.numpy()
+
+
+@pytest.mark.parametrize("vt,et", TYPES)
+@pytest.mark.parametrize("name", ["mostly_diagonal", "diagonal", "poisson7", "irregular", "short_long"])
+def test_split_diagonal_plan(mats, name, vt, et):
+    """Split-diagonal plans (diagonal vector + compressed remainder rows with a row map) give
+    the CSR product: the remainder's rows add onto the diagonal part at their matrix rows,
+    padded rows stay zero."""
+    if name == "mostly_diagonal":
+        A = mostly_diagonal()
+    elif name == "diagonal":
+        A = I.CSR(3000, 3000, np.arange(3001), np.arange(3000, dtype=np.int32), G.uniform(17, 3000))
+    else:
+        A = mats[name]
+    p = I.Plan(A, vt, elm_type=et, flags=I.SPMV_PLAN_SPLIT_DIAG)
+    assert p.n_rows_padded % 512 == 0 and p.n_rows_padded >= A.n_rows
+    x = G.uniform(18, A.n_rows)
+    check(A, None, x, run_plan(p, A, x, vt), vt, et)
+
+
+def test_split_diagonal_integer_exact(mats):
+    A = mostly_diagonal()
+    Ai = I.CSR(A.n_rows, A.n_cols, A.row_ptr, A.cols, np.round(A.vals * 8))
+    x = np.round(G.uniform(19, A.n_rows) * 8)
+    p = I.Plan(Ai, I.F64, flags=I.SPMV_PLAN_SPLIT_DIAG)
+    check(Ai, None, x, run_plan(p, Ai, x, I.F64), I.F64, I.F64, integer=True)
+
+
+def test_split_diagonal_refuses_spmm_and_refresh(mats):
+    A = mostly_diagonal()
+    p = I.Plan(A, I.F64, flags=I.SPMV_PLAN_SPLIT_DIAG)
+    with pytest.raises(I.IspmError):
+        p.update_coefs(A)
+    n = p.n_rows_padded
+    X = I.DeviceArray(2 * n)
+    with pytest.raises(I.IspmError):
+        p.execute_multi(2, X, n, X, n)
+
+
 def test_plan_default_and_update_coefs(mats):
     A = mats["poisson7"]
     p = I.Plan(A, I.F64)
