@@ -57,12 +57,13 @@ int spmv_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, 
                   void *y, const unsigned *skip, const diag_pattern *pat = nullptr,
                   const unsigned *uni = nullptr, const layout_hints *hints = nullptr);
 
-/* Split-diagonal form of a square matrix: y = diag .* x over all n rows, then the rows with
- * off-diagonal entries (a SlELL layout R of m rows, streamed mapping) add their sums at
- * y[rmap[i]]. */
+/* Split-diagonal form of a square matrix: the rows with off-diagonal entries form a SlELL
+ * layout R of m rows (streamed mapping) whose row i is matrix row rmap[i]; y = diag .* x on
+ * every other row, y = diag .* x + (R x) on those (one launch, see spmv_kernel.h). */
 struct split_diag {
   const void *diag;        /* n entries of elm_type (padded rows 0) */
   const unsigned *rmap;    /* R row -> matrix row, ~0u for R's padding rows */
+  const unsigned *rmask;   /* n/32 words: bit r set when matrix row r is a row of R */
   unsigned n;              /* rows (a multiple of 512) */
   unsigned m;              /* rows of R with entries */
   int elm_type;

@@ -638,14 +638,17 @@ static int create_split(const ispm_csr *A, int et, unsigned flags, ispm_plan *p)
   if ((st = ispm_memcpy(p->dev_diag, dh.data(), dh.size(), ISPM_H2D))) return st;
   const size_t m_pad = m ? (size_t)p->L.n_rows : 0;
   if (m_pad) {
-    std::vector<unsigned> mh(m_pad, ~0u);
+    std::vector<unsigned> mh(m_pad, ~0u), bits(n_pad / 32, 0u);
     std::copy(rmap.begin(), rmap.end(), mh.begin());
+    for (unsigned r : rmap) bits[r >> 5] |= 1u << (r & 31);
     if ((st = ispm_malloc((void **)&p->dev_rmap, 4 * m_pad))) return st;
     if ((st = ispm_memcpy(p->dev_rmap, mh.data(), 4 * m_pad, ISPM_H2D))) return st;
+    if ((st = ispm_malloc((void **)&p->dev_rmask, 4 * bits.size()))) return st;
+    if ((st = ispm_memcpy(p->dev_rmask, bits.data(), 4 * bits.size(), ISPM_H2D))) return st;
   }
-  /* bytes one product moves: diagonal, remainder layout, row map, x and y once */
+  /* bytes one product moves: diagonal, remainder layout, row map and mask, x and y once */
   const size_t rem = m ? 4 * p->dev.slice_ptr_len + 4 * p->dev.cols_len + tsize(et) * p->dev.elms_len : 0;
-  p->stored = n_pad * tsize(et) + rem + 4 * m_pad + tsize(p->T) * ((size_t)n + n_pad);
+  p->stored = n_pad * tsize(et) + rem + 4 * m_pad + (m_pad ? n_pad / 8 : 0) + tsize(p->T) * ((size_t)n + n_pad);
   p->csr = csr_bytes(A, p->E, p->T);
   p->dev.n_rows = (int)n_pad;   /* reported as the matrix's padded row count */
   return ISPM_OK;
@@ -860,7 +863,7 @@ int ispm_plan_execute(ispm_plan *p, const void *x, void *y)
   int st = ensure_init();
   if (st) return st;
   if (p->split) {
-    const split_diag sd{p->dev_diag, p->dev_rmap, p->split_n, p->split_m, p->E};
+    const split_diag sd{p->dev_diag, p->dev_rmap, p->dev_rmask, p->split_n, p->split_m, p->E};
     return spmv_split(sd, &p->launch, &p->dev, p->T, x, y, nullptr, nullptr, nullptr, nullptr, &p->hints);
   }
   return spmv_internal(&p->launch, &p->dev, p->T, x, y, nullptr, p->has_pat ? &p->pat : nullptr,
@@ -895,6 +898,7 @@ void ispm_plan_destroy(ispm_plan *p)
   if (p->has_dev) ispm_slell_device_free(&p->dev);
   if (p->dev_diag) ispm_free(p->dev_diag);
   if (p->dev_rmap) ispm_free(p->dev_rmap);
+  if (p->dev_rmask) ispm_free(p->dev_rmask);
   if (p->dev_map) ispm_free(p->dev_map);
   if (p->dev_uni) ispm_free(p->dev_uni);
   delete p;

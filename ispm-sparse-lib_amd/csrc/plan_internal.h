@@ -26,6 +26,7 @@ struct ispm_plan {
   bool split = false;
   void *dev_diag = nullptr;               /* split_n entries of E */
   unsigned *dev_rmap = nullptr;           /* remainder row -> matrix row (~0u: padding) */
+  unsigned *dev_rmask = nullptr;          /* bit per matrix row: a remainder row */
   unsigned split_n = 0, split_m = 0;      /* padded rows of the matrix; remainder rows */
   ispm::layout_hints hints{};             /* launch facts of the layout (streamed mapping) */
 };

@@ -375,7 +375,7 @@ int alloc(void **p, size_t bytes) { return ispm_malloc(p, bytes); }
 int spmv(ispm_plan *P, int vt, const void *x, void *y, const unsigned *skip)
 {
   if (P->split) {
-    const split_diag sd{P->dev_diag, P->dev_rmap, P->split_n, P->split_m, P->E};
+    const split_diag sd{P->dev_diag, P->dev_rmap, P->dev_rmask, P->split_n, P->split_m, P->E};
     return spmv_split(sd, &P->launch, &P->dev, vt, x, y, skip, nullptr, nullptr, nullptr, &P->hints);
   }
   return spmv_internal(&P->launch, &P->dev, vt, x, y, skip, P->has_pat ? &P->pat : nullptr, P->dev_uni,
@@ -439,7 +439,7 @@ int spmv_dot(ispm_solver *S, ispm_plan *P, const void *x, void *y, const void *d
   if (!parts) parts = S->parts;
   int st;
   if (P->split) {
-    const split_diag sd{P->dev_diag, P->dev_rmap, P->split_n, P->split_m, P->E};
+    const split_diag sd{P->dev_diag, P->dev_rmap, P->dev_rmask, P->split_n, P->split_m, P->E};
     st = spmv_split(sd, &P->launch, &P->dev, S->T, x, y, skip, d, parts, &G, &P->hints);
   } else {
     st = spmv_dot_internal(&P->launch, &P->dev, S->T, x, y, skip, P->has_pat ? &P->pat : nullptr,

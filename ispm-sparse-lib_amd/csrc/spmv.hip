@@ -20,7 +20,7 @@ int spmv_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, 
   if (A->n_slices == 0) return ISPM_OK;
   if (spmvk::use_diagpat(p, A, vec_type, H, pat, uni)) return spmvk::launch_diagpat_types(A, vec_type, *pat, uni, x, y, skip);
   if (spmvk::use_stream(p, A))
-    return spmvk::launch_stream_types(A, vec_type, x, y, skip, nullptr, nullptr, nullptr, nullptr, kMaxDotParts, hints);
+    return spmvk::launch_stream_types(A, vec_type, x, y, skip, nullptr, nullptr, nullptr, hints);
   if (spmvk::use_packed(p, A)) return spmvk::launch_packed_types(p, A, vec_type, x, y, skip);
   return spmvk::launch_types<1>(p, A, vec_type, x, 0, y, 0, H, skip);
 }
@@ -38,7 +38,7 @@ int spmv_dot_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_ty
   if (spmvk::use_diagpat(p, A, vec_type, H, pat, uni))
     return spmvk::launch_diagpat_types(A, vec_type, *pat, uni, x, y, skip, dotv, dparts, dgrid);
   if (spmvk::use_stream(p, A))
-    return spmvk::launch_stream_types(A, vec_type, x, y, skip, dotv, dparts, dgrid, nullptr, kMaxDotParts, hints);
+    return spmvk::launch_stream_types(A, vec_type, x, y, skip, dotv, dparts, dgrid, hints);
   return spmv_internal(p, A, vec_type, x, y, skip, pat, uni, hints);
 }
 
@@ -51,32 +51,18 @@ int spmv_split(const split_diag &sd, const ispm_launch *p, const ispm_slell_desc
   const int et = sd.elm_type;
   const unsigned V = et == ISPM_F64 ? 2u : 4u;
   if (sd.n % V) return fail(ISPM_ERR_INVALID, "spmv: split-diagonal rows not a multiple of %u", V);
-  /* one vector of rows per thread and step, at most 2048 workgroups (grid-stride) */
-  const unsigned grid = std::max(1u, std::min(2048u, (sd.n / V + 255) / 256));
-  const bool fuse = dotv && dparts && dgrid;
-  double *dp = fuse ? dparts : nullptr;
-  int st;
-  if (vec_type == ISPM_F64 && et == ISPM_F64) st = spmvk::launch_split_diag<double, double>(sd.diag, x, y, sd.n, skip, dotv, dp, grid);
-  else if (vec_type == ISPM_F64 && et == ISPM_F32) st = spmvk::launch_split_diag<double, float>(sd.diag, x, y, sd.n, skip, dotv, dp, grid);
-  else if (vec_type == ISPM_F32 && et == ISPM_F32) st = spmvk::launch_split_diag<float, float>(sd.diag, x, y, sd.n, skip, dotv, dp, grid);
-  else if (vec_type == ISPM_F32 && et == ISPM_BF16) st = spmvk::launch_split_diag<float, spmvk::bf16_t>(sd.diag, x, y, sd.n, skip, dotv, dp, grid);
-  else return fail(ISPM_ERR_UNSUPPORTED, "spmv: unsupported (vector, element) types (%d, %d)", vec_type, et);
-  if (st) return st;
-  if (sd.m == 0 || !R || R->n_slices == 0) {
-    if (fuse) *dgrid = grid;
-    return ISPM_OK;
-  }
-  if (!spmvk::use_stream(p, R) || R->elm_type != et)
+  const bool rem = sd.m > 0 && R && R->n_slices > 0;
+  if (rem && (!spmvk::use_stream(p, R) || R->elm_type != et || !sd.rmap || !sd.rmask))
     return fail(ISPM_ERR_INVALID, "spmv: split-diagonal remainder needs the streamed mapping");
-  unsigned G2 = 0;
-  const bool fuse2 = fuse && grid < kMaxDotParts;
-  st = spmvk::launch_stream_types(R, vec_type, x, y, skip, fuse2 ? dotv : nullptr,
-                                  fuse2 ? dparts + grid : nullptr, fuse2 ? &G2 : nullptr, sd.rmap,
-                                  kMaxDotParts - grid, hints);
-  if (st) return st;
-  /* both parts' partials, or none (the caller then takes the dot product itself) */
-  if (fuse && G2) *dgrid = grid + G2;
-  return ISPM_OK;
+  if (vec_type == ISPM_F64 && et == ISPM_F64)
+    return spmvk::launch_split<double, double>(R, sd.diag, sd.rmap, sd.rmask, sd.n, rem, x, y, skip, dotv, dparts, dgrid, hints);
+  if (vec_type == ISPM_F64 && et == ISPM_F32)
+    return spmvk::launch_split<double, float>(R, sd.diag, sd.rmap, sd.rmask, sd.n, rem, x, y, skip, dotv, dparts, dgrid, hints);
+  if (vec_type == ISPM_F32 && et == ISPM_F32)
+    return spmvk::launch_split<float, float>(R, sd.diag, sd.rmap, sd.rmask, sd.n, rem, x, y, skip, dotv, dparts, dgrid, hints);
+  if (vec_type == ISPM_F32 && et == ISPM_BF16)
+    return spmvk::launch_split<float, spmvk::bf16_t>(R, sd.diag, sd.rmap, sd.rmask, sd.n, rem, x, y, skip, dotv, dparts, dgrid, hints);
+  return fail(ISPM_ERR_UNSUPPORTED, "spmv: unsupported (vector, element) types (%d, %d)", vec_type, et);
 }
 
 /* SpMM for the block solver: same checks as ispm_spmm */

@@ -55,9 +55,11 @@ struct spmv_args {
   const unsigned *skip;  /* if non-null and *skip != 0 the launch does nothing (solver done flag) */
   const void *dotv = nullptr;   /* fused dot: workgroup w stores sum_rows dotv[row]*y[row] */
   double *dparts = nullptr;     /* at dparts[w] (kernels that support it) */
-  /* streamed mapping only: layout row i is matrix row rmap[i] (~0u: none) and its sum is
-   * added to y there (split-diagonal plans: the rows with off-diagonal entries) */
+  /* streamed mapping only (split-diagonal plans, whose layout holds the rows with
+   * off-diagonal entries): layout row i is matrix row r = rmap[i] (~0u: none), and
+   * y[r] = rdiag[r] * x[r] + the row's sum */
   const unsigned *rmap = nullptr;
+  const void *rdiag = nullptr;
 };
 
 template<typename T> __device__ __forceinline__ T xload(const T *x, unsigned i, unsigned n)
@@ -614,7 +616,7 @@ __global__ void __launch_bounds__(256) slell_packed_kernel(spmv_args a)
  * factors): the work of a workgroup is three dependent memory round trips (offsets,
  * elements and columns, x). */
 template<typename T, typename E, int RPT>
-__global__ void __launch_bounds__(256) slell_stream_kernel(spmv_args a, unsigned cap)
+__device__ __forceinline__ void stream_body(const spmv_args &a, unsigned cap, unsigned bid)
 {
   constexpr unsigned ROWS = 256u * RPT, U = 4, VEC = dev::evec<E>::n;
   /* products (cap elements, sized by the launch), then the group's slice offsets */
@@ -631,27 +633,28 @@ __global__ void __launch_bounds__(256) slell_stream_kernel(spmv_args a, unsigned
   T *__restrict__ yv = static_cast<T *>(a.y);
   const T *__restrict__ dv = static_cast<const T *>(a.dotv);
   const unsigned ncols = a.n_cols;
-  const unsigned group = (blockIdx.x & 7u) * a.per_xcd + (blockIdx.x >> 3);
+  const unsigned group = (bid & 7u) * a.per_xcd + (bid >> 3);
   const unsigned s0 = group * spg;
   if (s0 >= a.n_slices) {   /* uniform over the workgroup */
-    if (a.dparts && t == 0) a.dparts[blockIdx.x] = 0.0;
+    if (a.dparts && t == 0) a.dparts[bid] = 0.0;
     return;
   }
   const unsigned ns = min(spg, a.n_slices - s0);
   /* Destination rows (~0u: none), and what the epilogue reads there (the row map's entry,
-   * y to add to, the dot product's vector): requested first, as they do not depend on the
-   * layout's offsets, so their latency overlaps the element and x loads. */
+   * the diagonal term of a split-diagonal row, the dot product's vector): requested first,
+   * as they do not depend on the layout's offsets, so their latency overlaps the element
+   * and x loads. */
   unsigned dst[RPT];
-  T yold[RPT], dvv[RPT];
+  T dterm[RPT], dvv[RPT];
 #pragma unroll
   for (int k = 0; k < RPT; k++) {
     const unsigned row = t + k * 256u;
     dst[k] = (row >> log2S) < ns ? (s0 << log2S) + row : ~0u;
     if (a.rmap && dst[k] != ~0u) dst[k] = a.rmap[dst[k]];
-    yold[k] = 0;
+    dterm[k] = 0;
     dvv[k] = 0;
     if (dst[k] != ~0u) {
-      if (a.rmap) yold[k] = yv[dst[k]];
+      if (a.rmap) dterm[k] = elt<T, E>::get(static_cast<const E *>(a.rdiag), dst[k]) * xv[dst[k]];
       if (a.dparts) dvv[k] = dv[dst[k]];
     }
   }
@@ -736,7 +739,10 @@ __global__ void __launch_bounds__(256) slell_stream_kernel(spmv_args a, unsigned
   }
 #pragma unroll
   for (int k = 0; k < RPT; k++)
-    if (dst[k] != ~0u) yv[dst[k]] = a.rmap ? yold[k] + acc[k] : acc[k];
+    if (a.rmap) acc[k] = dterm[k] + acc[k];
+#pragma unroll
+  for (int k = 0; k < RPT; k++)
+    if (dst[k] != ~0u) yv[dst[k]] = acc[k];
   if (a.dparts) {
     __shared__ double red[4];
     double d[1] = {0.0};
@@ -744,8 +750,14 @@ __global__ void __launch_bounds__(256) slell_stream_kernel(spmv_args a, unsigned
     for (int k = 0; k < RPT; k++)
       if (dst[k] != ~0u) d[0] += (double)dvv[k] * (double)acc[k];
     dev::block_sum<256, 1>(d, red);
-    if (t == 0) a.dparts[blockIdx.x] = d[0];
+    if (t == 0) a.dparts[bid] = d[0];
   }
+}
+
+template<typename T, typename E, int RPT>
+__global__ void __launch_bounds__(256) slell_stream_kernel(spmv_args a, unsigned cap)
+{
+  stream_body<T, E, RPT>(a, cap, blockIdx.x);
 }
 
 /* LDS bytes of a streamed-mapping workgroup with a product buffer of cap elements */
@@ -774,11 +786,9 @@ void stream_shape(const layout_hints *h, unsigned log2S, unsigned &rpt, unsigned
 template<typename T, typename E>
 int launch_stream(const ispm_slell_desc *A, const void *x, void *y, const unsigned *skip,
                   const void *dotv = nullptr, double *dparts = nullptr, unsigned *dgrid = nullptr,
-                  const unsigned *rmap = nullptr, unsigned max_parts = kMaxDotParts,
                   const layout_hints *hints = nullptr)
 {
   spmv_args a;
-  a.rmap = rmap;
   a.skip = skip;
   a.sp = A->slice_ptr;
   a.cols = A->cols;
@@ -796,7 +806,7 @@ int launch_stream(const ispm_slell_desc *A, const void *x, void *y, const unsign
   const unsigned spg = (256u * rpt) >> a.log2S;
   const unsigned groups = (A->n_slices + spg - 1) / spg;
   a.per_xcd = (groups + 7) / 8;
-  if (dparts && dgrid && a.per_xcd * 8 <= max_parts) {
+  if (dparts && dgrid && a.per_xcd * 8 <= kMaxDotParts) {
     a.dotv = dotv;
     a.dparts = dparts;
     *dgrid = a.per_xcd * 8;
@@ -812,35 +822,50 @@ int launch_stream(const ispm_slell_desc *A, const void *x, void *y, const unsign
 inline int launch_stream_types(const ispm_slell_desc *A, int vt, const void *x, void *y,
                                const unsigned *skip, const void *dotv = nullptr,
                                double *dparts = nullptr, unsigned *dgrid = nullptr,
-                               const unsigned *rmap = nullptr, unsigned max_parts = kMaxDotParts,
                                const layout_hints *h = nullptr)
 {
   const int et = A->elm_type;
-  if (vt == ISPM_F64 && et == ISPM_F64) return launch_stream<double, double>(A, x, y, skip, dotv, dparts, dgrid, rmap, max_parts, h);
-  if (vt == ISPM_F64 && et == ISPM_F32) return launch_stream<double, float>(A, x, y, skip, dotv, dparts, dgrid, rmap, max_parts, h);
-  if (vt == ISPM_F32 && et == ISPM_F32) return launch_stream<float, float>(A, x, y, skip, dotv, dparts, dgrid, rmap, max_parts, h);
-  if (vt == ISPM_F32 && et == ISPM_BF16) return launch_stream<float, bf16_t>(A, x, y, skip, dotv, dparts, dgrid, rmap, max_parts, h);
+  if (vt == ISPM_F64 && et == ISPM_F64) return launch_stream<double, double>(A, x, y, skip, dotv, dparts, dgrid, h);
+  if (vt == ISPM_F64 && et == ISPM_F32) return launch_stream<double, float>(A, x, y, skip, dotv, dparts, dgrid, h);
+  if (vt == ISPM_F32 && et == ISPM_F32) return launch_stream<float, float>(A, x, y, skip, dotv, dparts, dgrid, h);
+  if (vt == ISPM_F32 && et == ISPM_BF16) return launch_stream<float, bf16_t>(A, x, y, skip, dotv, dparts, dgrid, h);
   return fail(ISPM_ERR_UNSUPPORTED, "spmv: unsupported (vector, element) types (%d, %d)", vt, et);
 }
 
-/* Diagonal part of a split-diagonal product: y = d .* x over n rows (n a multiple of the
- * vector width), with the block partial of (dotv, y) at dparts[block] when asked.  Each
- * thread takes VEC consecutive rows per step: one vector load of d (nontemporal, read
- * once), vector loads of x (and dotv) and a vector store of y when VL (x, y, dotv aligned). */
+/* Split-diagonal product in one launch: workgroups [0, G2) are the streamed mapping over the
+ * layout of the rows with off-diagonal entries (each such row's value is its diagonal term
+ * plus its sum, written once), workgroups [G2, G2 + G1) the diagonal part y = d .* x of
+ * every other row (a bit per row in rmask marks the rows the first group writes).  The two
+ * roles touch disjoint rows, so they run side by side: the latency-bound remainder overlaps
+ * the bandwidth-bound diagonal stream.  Each diagonal workgroup thread takes VEC rows per
+ * step: a vector load of d (nontemporal, read once), vector loads of x (and the dot
+ * product's vector) and a vector store of y when VL (x, y, dotv aligned); fused dot
+ * partials go to dparts[block] for both roles. */
+struct split_args {
+  const void *d;
+  const unsigned *rmask;
+  unsigned n;        /* rows, a multiple of VEC */
+  unsigned G2;       /* remainder workgroups (a multiple of 8, or 0) */
+  double *dparts;    /* fused dot: G2 + G1 partials, or null */
+};
+
 template<typename T, typename E, bool VL>
-__global__ void __launch_bounds__(256)
-split_diag_kernel(const E *__restrict__ d, const T *__restrict__ x, T *__restrict__ y, unsigned n,
-                  const unsigned *skip, const T *__restrict__ dv, double *dparts)
+__device__ __forceinline__ void diag_body(const spmv_args &a, const split_args &sa, unsigned bid, unsigned nb)
 {
   __shared__ double red[4];
-  if (skip && *skip) return;
+  if (a.skip && *a.skip) return;
   constexpr unsigned V = dev::evec<E>::n;
   typedef T tv __attribute__((ext_vector_type(V)));
+  const E *__restrict__ d = static_cast<const E *>(sa.d);
+  const T *__restrict__ x = static_cast<const T *>(a.x);
+  const T *__restrict__ dv = static_cast<const T *>(a.dotv);
+  T *__restrict__ y = static_cast<T *>(a.y);
   double acc[1] = {0.0};
-  const unsigned nv = n / V;
-  for (unsigned i = blockIdx.x * 256u + threadIdx.x; i < nv; i += gridDim.x * 256u) {
+  const unsigned nv = sa.n / V;
+  for (unsigned i = bid * 256u + threadIdx.x; i < nv; i += nb * 256u) {
     T dd[V], xx[V], yy[V];
     dev::elt_vec<T>(d, i * V, dd);
+    const unsigned bits = sa.rmask ? (sa.rmask[(i * V) >> 5] >> ((i * V) & 31u)) & ((1u << V) - 1u) : 0u;
     if (VL) {
       const tv xv = *reinterpret_cast<const tv *>(x + (size_t)i * V);
 #pragma unroll
@@ -852,36 +877,84 @@ split_diag_kernel(const E *__restrict__ d, const T *__restrict__ x, T *__restric
     tv yv;
 #pragma unroll
     for (unsigned q = 0; q < V; q++) { yy[q] = dd[q] * xx[q]; yv[q] = yy[q]; }
-    if (VL) *reinterpret_cast<tv *>(y + (size_t)i * V) = yv;
+    if (VL && !bits) *reinterpret_cast<tv *>(y + (size_t)i * V) = yv;
     else {
 #pragma unroll
-      for (unsigned q = 0; q < V; q++) y[(size_t)i * V + q] = yy[q];
+      for (unsigned q = 0; q < V; q++)
+        if (!((bits >> q) & 1u)) y[(size_t)i * V + q] = yy[q];
     }
-    if (dparts) {
+    if (sa.dparts) {
 #pragma unroll
-      for (unsigned q = 0; q < V; q++) acc[0] += (double)dv[(size_t)i * V + q] * (double)yy[q];
+      for (unsigned q = 0; q < V; q++)
+        if (!((bits >> q) & 1u)) acc[0] += (double)dv[(size_t)i * V + q] * (double)yy[q];
     }
   }
-  if (dparts) {
+  if (sa.dparts) {
     dev::block_sum<256, 1>(acc, red);
-    if (threadIdx.x == 0) dparts[blockIdx.x] = acc[0];
+    if (threadIdx.x == 0) sa.dparts[sa.G2 + bid] = acc[0];
   }
 }
 
+template<typename T, typename E, int RPT, bool VL>
+__global__ void __launch_bounds__(256) split_kernel(spmv_args a, unsigned cap, split_args sa)
+{
+  if (blockIdx.x < sa.G2) stream_body<T, E, RPT>(a, cap, blockIdx.x);
+  else diag_body<T, E, VL>(a, sa, blockIdx.x - sa.G2, gridDim.x - sa.G2);
+}
+
 template<typename T, typename E>
-int launch_split_diag(const void *d, const void *x, void *y, unsigned n, const unsigned *skip,
-                      const void *dotv, double *dparts, unsigned grid)
+int launch_split(const ispm_slell_desc *R, const void *diag, const unsigned *rmap,
+                 const unsigned *rmask, unsigned n, bool has_rem, const void *x, void *y,
+                 const unsigned *skip, const void *dotv, double *dparts, unsigned *dgrid,
+                 const layout_hints *hints)
 {
   constexpr unsigned V = dev::evec<E>::n;
+  spmv_args a;
+  a.skip = skip;
+  a.x = x;
+  a.y = y;
+  a.ldx = a.ldy = 0;
+  a.spb = 1;
+  a.rmap = rmap;
+  a.rdiag = diag;
+  unsigned rpt = 1, cap = 0, G2 = 0;
+  size_t lds = 0;
+  if (has_rem) {
+    a.sp = R->slice_ptr;
+    a.cols = R->cols;
+    a.elms = R->elms;
+    a.n_slices = R->n_slices;
+    a.log2S = (unsigned)(31 - __builtin_clz(R->slice_height));
+    a.n_cols = (unsigned)R->n_cols;
+    stream_shape<T>(hints, a.log2S, rpt, cap);
+    lds = stream_lds_bytes<T>(cap, rpt, a.log2S);
+    const unsigned spg = (256u * rpt) >> a.log2S;
+    const unsigned groups = (R->n_slices + spg - 1) / spg;
+    a.per_xcd = (groups + 7) / 8;
+    G2 = a.per_xcd * 8;
+  } else {
+    a.n_slices = 0;
+    a.per_xcd = 0;
+  }
+  /* one vector of rows per thread and step, at most 2048 diagonal workgroups (grid-stride) */
+  const unsigned G1 = std::max(1u, std::min(2048u, (n / V + 255) / 256));
+  split_args sa{diag, rmask, n, G2, nullptr};
+  if (dotv && dparts && dgrid && G1 + G2 <= kMaxDotParts) {
+    a.dotv = dotv;
+    a.dparts = dparts;
+    sa.dparts = dparts;
+    *dgrid = G1 + G2;
+  }
   const uintptr_t al = reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(y) |
-                       (dparts ? reinterpret_cast<uintptr_t>(dotv) : 0);
+                       (sa.dparts ? reinterpret_cast<uintptr_t>(dotv) : 0);
   const bool vl = al % (V * sizeof(T)) == 0;
-  if (vl)
-    hipLaunchKernelGGL((split_diag_kernel<T, E, true>), dim3(grid), dim3(256), 0, stream(), static_cast<const E *>(d),
-                       static_cast<const T *>(x), static_cast<T *>(y), n, skip, static_cast<const T *>(dotv), dparts);
-  else
-    hipLaunchKernelGGL((split_diag_kernel<T, E, false>), dim3(grid), dim3(256), 0, stream(), static_cast<const E *>(d),
-                       static_cast<const T *>(x), static_cast<T *>(y), n, skip, static_cast<const T *>(dotv), dparts);
+  const dim3 g(G1 + G2), b(256);
+  hipStream_t st = stream();
+#define ISPM_SPLIT_LAUNCH(R_, VL_) hipLaunchKernelGGL((split_kernel<T, E, R_, VL_>), g, b, lds, st, a, cap, sa)
+  if (rpt == 1) { if (vl) ISPM_SPLIT_LAUNCH(1, true); else ISPM_SPLIT_LAUNCH(1, false); }
+  else if (rpt == 2) { if (vl) ISPM_SPLIT_LAUNCH(2, true); else ISPM_SPLIT_LAUNCH(2, false); }
+  else { if (vl) ISPM_SPLIT_LAUNCH(4, true); else ISPM_SPLIT_LAUNCH(4, false); }
+#undef ISPM_SPLIT_LAUNCH
   ISPM_HIP(hipGetLastError());
   return ISPM_OK;
 }
