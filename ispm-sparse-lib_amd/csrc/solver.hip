@@ -17,9 +17,10 @@
  *    residual (b - A x with the FP32-stored values read into FP64 accumulators) whenever the
  *    FP32 recursive residual has dropped by reliable_delta ("reliable updates").
  * All scalars live in device memory in FP64; every reduction is two-phase (block partials,
- * then a one-block finishing kernel) and deterministic.  Iterations are launched in hipGraph-captured batches; once
- * the recursive residual reaches its target a device flag turns the rest of the batch into
- * no-ops, and the host then decides on the FP64 true residual.
+ * then a one-block finishing kernel) and deterministic.  Iterations are launched in
+ * hipGraph-captured batches; once the recursive residual reaches its target a device flag
+ * turns the rest of the batch into no-ops, and the host then decides on the FP64 true
+ * residual.
  */
 #include <hip/hip_runtime.h>
 #include <cmath>

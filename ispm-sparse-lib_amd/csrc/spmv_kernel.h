@@ -458,10 +458,11 @@ int launch_diagpat(const ispm_slell_desc *A, const diag_pattern &pat, const unsi
                    const void *x, void *y, const unsigned *skip, const void *dotv = nullptr,
                    double *dparts = nullptr, unsigned *dgrid = nullptr)
 {
-  /* Rows per thread is the kernel's choice (each row's sum is the same either way): two for
-   * FP32 vectors (half the load instructions; 205 -> 184 us on a 27-point 192^3 stencil),
-   * one for FP64, where two rows per thread run no faster. */
-  unsigned H = (sizeof(T) == 4 && A->slice_height >= 128) ? 2u : 1u;
+  /* Rows per thread is the kernel's choice (each row's sum is the same either way): one,
+   * measured fastest on a 27-point 192^3 stencil with 256-row slices (FP32: 141-146 us with
+   * one row per thread, 151 us with two; FP64: 292 us with one, 350 us with two, whose LDS
+   * windows then halve the waves resident per CU). */
+  unsigned H = 1;
   if (const char *e = std::getenv("ISPM_DIAGPAT_H")) H = (unsigned)std::atoi(e);
   if ((H != 1 && H != 2 && H != 4) || A->slice_height / H < 64 ||
       (reinterpret_cast<uintptr_t>(y) % (H * sizeof(T))) != 0 ||
