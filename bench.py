@@ -155,6 +155,114 @@ def cpu_solve_baseline(O, A, b, factors, iters_needed, seconds=15.0):
             "seconds_per_iteration": per_iter}
 
 
+def batching_legs(I, G, O, S):
+    """The other workload configurations, timed on the device with inputs resident: the 128^3
+    finite-volume operator (FP64 SpMV and FSAI-PCG to 1e-8), its 3 and 8 right-hand sides as
+    one block PCG against as many single-RHS solves with the same preconditioner, and 512
+    uneven random-geometric-graph systems in one persistent launch against one launch per
+    system."""
+    import ctypes as C
+    from ispm_amd import batch as B
+    out = {}
+    t = time.perf_counter()
+    A = G.fv7_lognormal(128)
+    log(f"generated 128^3 finite-volume: n={A.n_rows} nnz={A.nnz} in {time.perf_counter() - t:.1f}s")
+    out["cfg2_spmv_fp64"] = spmv_leg(I, G, A, I.F64, I.F64, 20, 3)
+    M = I.fastainv_sym(A, I.F64, max_elts=64 * A.n_rows)
+    b = O.csr_spmv(A, G.uniform(1, A.n_rows))
+    sol, _ = solve_leg(I, S, O, A, b, M, "fp64", S.PCG, 3, 1, 1e-8)
+    out["cfg2_pcg_fp64"] = sol
+    log(f"cfg2 pcg: {sol['seconds'] * 1e3:.1f} ms, {sol['iterations']} it")
+    # cfg4a: block PCG (SpMM over k columns, the two factors as SlELL plans) vs k single solves
+    Ap = I.Plan(A, I.F64, elm_type=I.F64)
+    Mp = (I.Plan(M[0], I.F64, elm_type=I.F64, kind=I.PLAN_ESTIMATE),
+          I.Plan(M[1], I.F64, elm_type=I.F64, kind=I.PLAN_ESTIMATE))
+    n = Ap.n_rows_padded
+    single = S.Solver(A, M_plans=Mp, A_plan=Ap, precision="fp64", method=S.PCG, tol=1e-8)
+    for k in (3, 8):
+        Bm = np.zeros((k, n))
+        for j in range(k):
+            Bm[j, :A.n_rows] = O.csr_spmv(A, G.uniform(300 + j, A.n_rows))
+        opts = S.SolverOpts(S.PCG, S.PREC_FP64, 1e-8, 100000, 16, 0, 0.1)
+        bs = C.c_void_p()
+        I.check(I.lib().ispm_block_solver_create(C.byref(bs), Ap.p, Mp[0].p, Mp[1].p, k, C.byref(opts)))
+        bd = I.DeviceArray.from_numpy(Bm.ravel())
+        xd = I.DeviceArray(k * n)
+        stats = (S.SolveStats * k)()
+        ts = []
+        for rep in range(3):
+            xd.zero()
+            I.synchronize()
+            t0 = time.perf_counter()
+            I.check(I.lib().ispm_block_solver_solve(bs, bd.vp, n, xd.vp, n, stats))
+            I.synchronize()
+            ts.append(time.perf_counter() - t0)
+        X = xd.numpy().reshape(k, n)[:, :A.n_rows]
+        worst = max(float(np.linalg.norm(Bm[j, :A.n_rows] - O.csr_spmv(A, X[j])) / np.linalg.norm(Bm[j, :A.n_rows]))
+                    for j in range(k))
+        I.lib().ispm_block_solver_destroy(bs)
+        t1 = []
+        for j in range(k):
+            bj = I.DeviceArray.from_numpy(Bm[j].copy())
+            xj = I.DeviceArray(n)
+            single.solve_device(bj, xj)   # warm-up (graph capture)
+            xj.zero()
+            I.synchronize()
+            t0 = time.perf_counter()
+            single.solve_device(bj, xj)
+            I.synchronize()
+            t1.append(time.perf_counter() - t0)
+        out[f"cfg4a_block_k{k}"] = {"seconds": float(np.min(ts)), "iterations": [st.iterations for st in stats],
+                                    "oracle_relres_max": worst, "single_rhs_seconds_sum": float(np.sum(t1)),
+                                    "speedup_vs_single": float(np.sum(t1) / np.min(ts))}
+        log(f"cfg4a k={k}: block {np.min(ts) * 1e3:.1f} ms vs {np.sum(t1) * 1e3:.1f} ms single solves")
+    del single, Ap, Mp, A, M
+    # cfg4b: 512 uneven systems in one persistent launch vs one launch per system
+    count = 512
+    sizes = np.exp(np.log(1000) + (np.log(64000) - np.log(1000)) * G.uniform01(4001, count)).astype(int)
+    ks = 10 + np.minimum(16, (G.uniform01(4002, count) * 17).astype(int))
+    t = time.perf_counter()
+    systems = [G.rgg_laplacian(int(m), int(kk), seed=5000 + i, dtype=np.float32)
+               for i, (m, kk) in enumerate(zip(sizes, ks))]
+    rhs = [O.csr_spmv(Ai, G.uniform(6000 + i, Ai.n_rows)).astype(np.float32) for i, Ai in enumerate(systems)]
+    log(f"generated 512 systems ({sum(a.n_rows for a in systems)} rows) in {time.perf_counter() - t:.1f}s")
+    bt = B.Batch(systems)
+    bd = I.DeviceArray.from_numpy(bt.pack(rhs))
+    xd = I.DeviceArray(bt.rows_padded, np.float32)
+    ts = []
+    for rep in range(3):
+        xd.zero()
+        I.synchronize()
+        t0 = time.perf_counter()
+        it, rel = bt.solve_device(bd, xd, 1e-6, 20000, want_stats=False)
+        I.synchronize()
+        ts.append(time.perf_counter() - t0)
+    it, rel = bt.solve_device(bd, xd, 1e-6, 20000)
+    xs = bt.unpack(xd.numpy())
+    worst = 0.0
+    for Ai, bi, xi in zip(systems, rhs, xs):
+        A64 = I.CSR(Ai.n_rows, Ai.n_cols, Ai.row_ptr, Ai.cols, Ai.vals.astype(np.float64))
+        r = bi.astype(np.float64) - O.csr_spmv(A64, xi.astype(np.float64))
+        worst = max(worst, float(np.linalg.norm(r) / np.linalg.norm(bi)))
+    one = []
+    for i in range(count):
+        b1 = B.Batch([systems[i]])
+        bd1 = I.DeviceArray.from_numpy(b1.pack([rhs[i]]))
+        xd1 = I.DeviceArray(b1.rows_padded, np.float32)
+        I.synchronize()
+        t0 = time.perf_counter()
+        b1.solve_device(bd1, xd1, 1e-6, 20000, want_stats=False)
+        I.synchronize()
+        one.append(time.perf_counter() - t0)
+    out["cfg4b_batch_512"] = {"seconds": float(np.min(ts)), "rows": int(sum(a.n_rows for a in systems)),
+                              "nnz": int(sum(a.nnz for a in systems)), "iterations_max": int(it.max()),
+                              "iterations_mean": float(it.mean()), "oracle_relres_max": worst,
+                              "one_launch_per_system_seconds": float(np.sum(one)),
+                              "speedup_vs_one_per_system": float(np.sum(one) / np.min(ts))}
+    log(f"cfg4b: {np.min(ts) * 1e3:.1f} ms batched vs {np.sum(one) * 1e3:.1f} ms one launch per system")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +271,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--droptol", type=float, default=0.125)   # fastainv_sym default (fastainv.h:14)
     ap.add_argument("--spmv-only", action="store_true")
+    ap.add_argument("--batching", action="store_true",
+                    help="time the 128^3 finite-volume, block-RHS and 512-system batched configurations "
+                         "instead (one JSON line of its own)")
     ap.add_argument("--calibrate", action="store_true",
                     help="first copy 1 GiB device-to-device 3 times (a plain copy of known size, "
                          "the reference for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -184,6 +295,10 @@ def main():
         I.synchronize()
         del src, dst
         log("calibration: 3 device-to-device copies of 1 GiB")
+    if args.batching:
+        from ispm_amd import solver as S
+        print(json.dumps({"metric": "batching configurations", "legs": batching_legs(I, G, O, S)}))
+        return
     t = time.perf_counter()
     A = G.stencil27_contrast(192)
     log(f"generated 27-point 192^3: n={A.n_rows} nnz={A.nnz} in {time.perf_counter() - t:.1f}s")

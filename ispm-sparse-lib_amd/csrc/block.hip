@@ -17,7 +17,7 @@
 namespace ispm {
 
 int spmm_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, int k,
-                  const void *X, size_t ldx, void *Y, size_t ldy);
+                  const void *X, size_t ldx, void *Y, size_t ldy, const layout_hints *hints);
 
 namespace {
 
@@ -147,7 +147,7 @@ namespace {
 
 int mspmm(ispm_plan *P, int k, const double *X, size_t ldx, double *Y, size_t ldy)
 {
-  return spmm_internal(&P->launch, &P->dev, ISPM_F64, k, X, ldx, Y, ldy);
+  return spmm_internal(&P->launch, &P->dev, ISPM_F64, k, X, ldx, Y, ldy, &P->hints);
 }
 
 template<int OP> void mfinish(ispm_block_solver *S)

@@ -4,7 +4,7 @@
 namespace ispm {
 /* defined in spmm.hip */
 int spmm_dispatch(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, int k,
-                  const void *X, size_t ldx, void *Y, size_t ldy);
+                  const void *X, size_t ldx, void *Y, size_t ldy, const layout_hints *hints);
 }
 
 namespace ispm {
@@ -65,11 +65,19 @@ int spmv_split(const split_diag &sd, const ispm_launch *p, const ispm_slell_desc
   return fail(ISPM_ERR_UNSUPPORTED, "spmv: unsupported (vector, element) types (%d, %d)", vec_type, et);
 }
 
-/* SpMM for the block solver: same checks as ispm_spmm */
+/* SpMM for the block solver: the checks of ispm_spmm, with the plan's layout facts */
 int spmm_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, int k,
-                  const void *X, size_t ldx, void *Y, size_t ldy)
+                  const void *X, size_t ldx, void *Y, size_t ldy, const layout_hints *hints)
 {
-  return ispm_spmm(p, A, vec_type, k, X, ldx, Y, ldy);
+  int st = ensure_init();
+  if (st) return st;
+  unsigned H;
+  st = spmvk::check_args(p, A, vec_type, X, Y, H);
+  if (st) return st;
+  if (k < 1 || k > 8) return fail(ISPM_ERR_UNSUPPORTED, "spmm: k=%d outside 1..8", k);
+  if (A->n_slices == 0) return ISPM_OK;
+  if (k == 1) return spmv_internal(p, A, vec_type, X, Y, nullptr, nullptr, nullptr, hints);
+  return spmm_dispatch(p, A, vec_type, k, X, ldx, Y, ldy, hints);
 }
 }  // namespace ispm
 
@@ -100,6 +108,6 @@ extern "C" int ispm_spmm(const ispm_launch *p, const ispm_slell_desc *A, int vec
   if (st) return st;
   if (k < 1 || k > 8) return fail(ISPM_ERR_UNSUPPORTED, "spmm: k=%d outside 1..8", k);
   if (A->n_slices == 0) return ISPM_OK;
-  if (k == 1) return launch_types<1>(p, A, vec_type, X, 0, Y, 0, H);
-  return spmm_dispatch(p, A, vec_type, k, X, ldx, Y, ldy);
+  if (k == 1) return spmv_internal(p, A, vec_type, X, Y, nullptr);
+  return spmm_dispatch(p, A, vec_type, k, X, ldx, Y, ldy, nullptr);
 }

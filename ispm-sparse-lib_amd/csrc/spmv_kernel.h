@@ -732,8 +732,12 @@ __device__ __forceinline__ void stream_body(const spmv_args &a, unsigned cap, un
       if (i < ns) {
         const unsigned o = soff[i], nk = (soff[i + 1] - o) >> log2S;
         for (unsigned q = 0; q < nk; q++) {
+          /* product rounded, then added, as through LDS: the result does not depend on
+           * whether the group fits */
+#pragma clang fp contract(off)
           const unsigned e = o + (q << log2S) + r;
-          acc[k] += elt<T, E>::get(el, e) * xload(xv, dev::col_load(cl + e), ncols);
+          const T pr = elt<T, E>::get(el, e) * xload(xv, dev::col_load(cl + e), ncols);
+          acc[k] += pr;
         }
       }
     }
@@ -818,6 +822,122 @@ int launch_stream(const ispm_slell_desc *A, const void *x, void *y, const unsign
   else hipLaunchKernelGGL((slell_stream_kernel<T, E, 4>), g, b, lds, stream(), a, cap);
   ISPM_HIP(hipGetLastError());
   return ISPM_OK;
+}
+
+/* Streamed mapping for k = 2..8 right-hand sides (SpMM of the block solver): a 256-thread
+ * workgroup takes 256 consecutive rows; phase 1 copies their contiguous element range (values
+ * converted to T, and column indices) into LDS with coalesced vector loads, phase 2 gives each
+ * thread one row and adds, plane by plane in storage order, value times x[col] for every
+ * column of X: one pass over the matrix serves all k vectors.  Groups whose range exceeds
+ * cap elements read their rows directly. */
+template<typename T, typename E, int K>
+__global__ void __launch_bounds__(256) slell_stream_spmm_kernel(spmv_args a, unsigned cap)
+{
+  constexpr unsigned VEC = dev::evec<E>::n;
+  extern __shared__ __align__(16) unsigned char spmm_lds[];
+  T *lval = reinterpret_cast<T *>(spmm_lds);                   /* cap values */
+  unsigned *lcol = reinterpret_cast<unsigned *>(lval + cap);   /* cap columns */
+  unsigned *soff = lcol + cap;                                  /* spg + 1 offsets */
+  const unsigned log2S = a.log2S, S = 1u << log2S, spg = 256u >> log2S, t = threadIdx.x;
+  const E *__restrict__ el = static_cast<const E *>(a.elms);
+  const unsigned *__restrict__ cl = a.cols;
+  const T *__restrict__ xv = static_cast<const T *>(a.x);
+  T *__restrict__ yv = static_cast<T *>(a.y);
+  const unsigned ncols = a.n_cols;
+  const unsigned group = (blockIdx.x & 7u) * a.per_xcd + (blockIdx.x >> 3);
+  const unsigned s0 = group * spg;
+  if (s0 >= a.n_slices) return;   /* uniform over the workgroup */
+  const unsigned ns = min(spg, a.n_slices - s0);
+  const unsigned E0 = a.sp[s0], E1 = a.sp[s0 + ns], n = E1 - E0;
+  for (unsigned q = t; q <= ns; q += 256) soff[q] = a.sp[s0 + q];
+  const unsigned base = E0 & ~(VEC - 1);
+  const bool fits = n + (E0 - base) <= cap;
+  if (fits) {
+    const unsigned A0 = (E0 + VEC - 1) & ~(VEC - 1);
+    const unsigned A1 = max(A0, E1 & ~(VEC - 1));
+    const unsigned hend = min(A0, E1);
+    if (E0 + t < hend) {
+      lval[E0 + t - base] = elt<T, E>::get(el, E0 + t);
+      lcol[E0 + t - base] = dev::col_load(cl + E0 + t);
+    }
+    if (A1 + t < E1 && A1 >= hend) {
+      lval[A1 + t - base] = elt<T, E>::get(el, A1 + t);
+      lcol[A1 + t - base] = dev::col_load(cl + A1 + t);
+    }
+    for (unsigned v = t; v < (A1 - A0) / VEC; v += 256) {
+      T ev[VEC];
+      unsigned cv[VEC];
+      dev::elt_vec<T>(el, A0 + v * VEC, ev);
+      dev::col_vec(cl, A0 + v * VEC, cv);
+      const unsigned o = A0 - base + v * VEC;
+#pragma unroll
+      for (unsigned q = 0; q < VEC; q++) { lval[o + q] = ev[q]; lcol[o + q] = cv[q]; }
+    }
+  }
+  __syncthreads();
+  const unsigned i = t >> log2S, r = t & (S - 1);
+  if (i >= ns) return;
+  T acc[K];
+#pragma unroll
+  for (int j = 0; j < K; j++) acc[j] = 0;
+  const unsigned nk = (soff[i + 1] - soff[i]) >> log2S;
+  for (unsigned q = 0; q < nk; q++) {
+    /* product rounded, then added: the streamed SpMV's arithmetic, so column j of the result
+     * is bit-identical to the single-vector product */
+#pragma clang fp contract(off)
+    const unsigned e = soff[i] + (q << log2S) + r;
+    const T v = fits ? lval[e - base] : elt<T, E>::get(el, e);
+    const unsigned c = fits ? lcol[e - base] : dev::col_load(cl + e);
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      const T pr = v * xload(xv + j * a.ldx, c, ncols);
+      acc[j] += pr;
+    }
+  }
+  const size_t row = ((size_t)s0 << log2S) + t;
+#pragma unroll
+  for (int j = 0; j < K; j++) yv[row + j * a.ldy] = acc[j];
+}
+
+template<typename T, typename E, int K>
+int launch_stream_spmm(const ispm_slell_desc *A, const void *X, size_t ldx, void *Y, size_t ldy,
+                       const layout_hints *hints)
+{
+  spmv_args a;
+  a.sp = A->slice_ptr;
+  a.cols = A->cols;
+  a.elms = A->elms;
+  a.x = X;
+  a.y = Y;
+  a.ldx = ldx;
+  a.ldy = ldy;
+  a.n_slices = A->n_slices;
+  a.log2S = (unsigned)(31 - __builtin_clz(A->slice_height));
+  a.n_cols = (unsigned)A->n_cols;
+  a.spb = 1;
+  a.skip = nullptr;
+  unsigned cap = hints && hints->stream_span[0] ? (hints->stream_span[0] + 3) & ~3u : 2048u;
+  const size_t soffb = 4 * (size_t)((256u >> a.log2S) + 1) + 16;
+  if ((size_t)cap * (sizeof(T) + 4) + soffb > 65536) cap = (unsigned)((65536 - soffb) / (sizeof(T) + 4)) & ~3u;
+  const size_t lds = ((size_t)cap * (sizeof(T) + 4) + soffb + 15) / 16 * 16;
+  const unsigned spg = 256u >> a.log2S;
+  const unsigned groups = (A->n_slices + spg - 1) / spg;
+  a.per_xcd = (groups + 7) / 8;
+  hipLaunchKernelGGL((slell_stream_spmm_kernel<T, E, K>), dim3(a.per_xcd * 8), dim3(256), lds, stream(), a, cap);
+  ISPM_HIP(hipGetLastError());
+  return ISPM_OK;
+}
+
+template<int K>
+int launch_stream_spmm_types(const ispm_slell_desc *A, int vt, const void *X, size_t ldx, void *Y,
+                             size_t ldy, const layout_hints *hints)
+{
+  const int et = A->elm_type;
+  if (vt == ISPM_F64 && et == ISPM_F64) return launch_stream_spmm<double, double, K>(A, X, ldx, Y, ldy, hints);
+  if (vt == ISPM_F64 && et == ISPM_F32) return launch_stream_spmm<double, float, K>(A, X, ldx, Y, ldy, hints);
+  if (vt == ISPM_F32 && et == ISPM_F32) return launch_stream_spmm<float, float, K>(A, X, ldx, Y, ldy, hints);
+  if (vt == ISPM_F32 && et == ISPM_BF16) return launch_stream_spmm<float, bf16_t, K>(A, X, ldx, Y, ldy, hints);
+  return fail(ISPM_ERR_UNSUPPORTED, "spmm: unsupported (vector, element) types (%d, %d)", vt, et);
 }
 
 inline int launch_stream_types(const ispm_slell_desc *A, int vt, const void *x, void *y,

@@ -399,6 +399,11 @@ class DeviceSlell:
         lp = Launch(N, S, H, 0, 0, mode)
         return lib().ispm_spmv(C.byref(lp), C.byref(self.desc), vec_type, x.vp, y.vp)
 
+    def spmm(self, k, X: DeviceArray, ldx, Y: DeviceArray, ldy, vec_type, N=64, S=None, H=1, mode=0):
+        S = self.desc.slice_height if S is None else S
+        lp = Launch(N, S, H, 0, 0, mode)
+        return lib().ispm_spmm(C.byref(lp), C.byref(self.desc), vec_type, k, X.vp, ldx, Y.vp, ldy)
+
     def __del__(self):
         try:
             lib().ispm_slell_device_free(C.byref(self.desc))

@@ -228,6 +228,39 @@ def test_spmm_matches_spmv(mats):
                 np.testing.assert_array_equal(Y[j], yd.numpy())
 
 
+@pytest.mark.parametrize("name", ["irregular", "skewed", "short_long"])
+@pytest.mark.parametrize("S", [1, 4, 32])
+@pytest.mark.parametrize("vt,et", TYPES)
+def test_spmm_streamed_matches_spmv(mats, name, S, vt, et):
+    """Multi-vector products of plain small-slice layouts in the streamed mapping (the
+    block solver's preconditioner factors): every column bit-identical to the single-vector
+    streamed product, with and without the plan's layout facts (short_long has groups
+    beyond the default LDS capacity)."""
+    A = mats[name]
+    dt = np.float64 if vt == I.F64 else np.float32
+    hs = I.csr_to_slell(A, S, 1, False, False, out_type=et)
+    dm = I.DeviceSlell(hs)
+    p = I.Plan(A, vt, elm_type=et, explicit=dict(S=S), launch=I.Launch(256, S, 1, 0, 0, 2))
+    n = hs.n_rows
+    for k in (2, 5, 8):
+        X = np.zeros((k, n), dt)
+        X[:, :A.n_cols] = G.uniform(400 + k, k * A.n_cols).reshape(k, A.n_cols)
+        Xd = I.DeviceArray.from_numpy(X.ravel())
+        for how in ("desc", "plan"):
+            Yd = I.DeviceArray(k * n, dt)
+            if how == "desc":
+                assert dm.spmm(k, Xd, n, Yd, n, vt, N=256, mode=2) == 0, I.lib().ispm_last_error()
+            else:
+                p.execute_multi(k, Xd, n, Yd, n)
+            Y = Yd.numpy().reshape(k, n)
+            for j in range(k):
+                xd = I.DeviceArray.from_numpy(X[j])
+                yd = I.DeviceArray(n, dt)
+                assert dm.spmv(xd, yd, vt, N=256, mode=2) == 0
+                np.testing.assert_array_equal(Y[j], yd.numpy())
+            check(A, hs, X[0, :A.n_cols], Y[0], vt, et)
+
+
 def mostly_diagonal(n=5000, seed=16):
     """Rows holding only their diagonal entry except every 23rd row (1-40 off-diagonal entries
     on both sides); row 7 has no diagonal entry and row 11 a zero one."""
