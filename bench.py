@@ -45,8 +45,8 @@ def time_spmv(I, plan, x, y, steps, warmup):
     return float(np.mean(ts)), float(np.median(ts))
 
 
-def spmv_leg(I, G, A, vt, et, steps, warmup):
-    plan = I.Plan(A, vt, elm_type=et)
+def spmv_leg(I, G, A, vt, et, steps, warmup, flags=0):
+    plan = I.Plan(A, vt, elm_type=et, flags=flags)
     inf = plan.info()
     n = plan.n_rows_padded
     dt = np.float64 if vt == I.F64 else np.float32
@@ -303,9 +303,11 @@ def main():
     A = G.stencil27_contrast(192)
     log(f"generated 27-point 192^3: n={A.n_rows} nnz={A.nnz} in {time.perf_counter() - t:.1f}s")
     legs = {}
-    for name, vt, et in (("fp64", I.F64, I.F64), ("fp32", I.F32, I.F32)):
+    for name, vt, et, fl in (("fp64", I.F64, I.F64, 0), ("fp32", I.F32, I.F32, 0),
+                             ("fp64_sym", I.F64, I.F64, I.SPMV_PLAN_SYMMETRIC),
+                             ("fp32_sym", I.F32, I.F32, I.SPMV_PLAN_SYMMETRIC)):
         t = time.perf_counter()
-        legs[name] = spmv_leg(I, G, A, vt, et, max(args.steps, 10), args.warmup)
+        legs[name] = spmv_leg(I, G, A, vt, et, max(args.steps, 10), args.warmup, fl)
         log(f"spmv {name}: {legs[name]['GBps']:.0f} GB/s ({legs[name]['avg_s'] * 1e6:.1f} us) "
             f"[{time.perf_counter() - t:.1f}s]")
     if args.spmv_only:

@@ -205,8 +205,14 @@ enum ispm_plan_kind { ISPM_PLAN_DEFAULT = 0, ISPM_PLAN_ESTIMATE = 1, ISPM_PLAN_E
  * rows add their sums.  For approximate-inverse factors and their products, whose rows are
  * mostly the diagonal alone.  Execution and the solver only (no SpMM, no coefficient
  * refresh); plan info reports the remainder's layout with the full padded row count. */
+/* ISPM_SPMV_PLAN_SYMMETRIC (no reference counterpart): for an exactly symmetric square
+ * matrix whose entries lie on at most 16 upper diagonals (stencils): store only the upper
+ * diagonals as dense planes of 256-row slices; each lower entry is read from its mirror, which
+ * an earlier slice streamed moments before, so HBM carries about half the matrix.  Plan
+ * creation fails with ISPM_ERR_INVALID when the matrix is not symmetric or not on few
+ * diagonals.  Execution and the solvers only (no SpMM, no coefficient refresh). */
 enum ispm_plan_flags { ISPM_SPMV_NO_TEXTURE_MEMORY = 1, ISPM_SPMV_PLAN_NO_DEVICE = 2,
-                       ISPM_SPMV_PLAN_SPLIT_DIAG = 4 };
+                       ISPM_SPMV_PLAN_SPLIT_DIAG = 4, ISPM_SPMV_PLAN_SYMMETRIC = 8 };
 
 typedef struct ispm_plan ispm_plan;
 

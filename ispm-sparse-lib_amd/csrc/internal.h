@@ -76,6 +76,24 @@ int spmv_split(const split_diag &sd, const ispm_launch *p, const ispm_slell_desc
                double *dparts = nullptr, unsigned *dgrid = nullptr,
                const layout_hints *hints = nullptr);
 
+/* Symmetric diagonal storage of a symmetric matrix whose entries lie on at most 16 upper
+ * diagonals (ISPM_SPMV_PLAN_SYMMETRIC): plane k of 256-row slice s holds A[row, row + d[k]]
+ * at elms[(s K + k) 256 + r], d[0] = 0 < d[1] < ...; the lower entries are read from the
+ * mirror rows.  pat lists the 2K-1 column offsets -d[K-1] .. -d[1], d[0] .. d[K-1] in
+ * that order for the x windows. */
+struct sym_desc {
+  const void *elms;
+  int elm_type;
+  unsigned n_slices, n_cols, K;
+  int d[16];
+  diag_pattern pat;
+};
+
+/* y = A x for symmetric diagonal storage, with the fused dot product as spmv_dot_internal
+ * (dotv, dparts, dgrid may be null) */
+int spmv_sym(const sym_desc &sd, int vec_type, const void *x, void *y, const unsigned *skip,
+             const void *dotv = nullptr, double *dparts = nullptr, unsigned *dgrid = nullptr);
+
 /* Partial sums a fused SpMV + dot may store (one per workgroup). */
 constexpr unsigned kMaxDotParts = 32768;
 

@@ -655,6 +655,126 @@ static int create_split(const ispm_csr *A, int et, unsigned flags, ispm_plan *p)
 }
 }  // namespace ispm
 
+namespace ispm {
+/* Symmetric diagonal plan: the upper diagonals of an exactly symmetric matrix as dense planes
+ * of 256-row slices (see sym_desc); the x-window pattern lists all 2K-1 column offsets. */
+static int create_sym(const ispm_csr *A, int et, unsigned flags, ispm_plan *p)
+{
+  if (A->n_rows != A->n_cols) return fail_plain(ISPM_ERR_INVALID, "symmetric plan: matrix must be square");
+  if (flags & ISPM_SPMV_PLAN_NO_DEVICE) return fail_plain(ISPM_ERR_INVALID, "symmetric plan needs the device");
+  const int n = A->n_rows;
+  const size_t n_pad = ((size_t)n + 511) / 512 * 512, ns = n_pad / 256;
+  /* the upper diagonal offsets (0 always present) */
+  std::vector<long long> d{0};
+  for (int i = 0; i < n; i++)
+    for (int k = A->row_ptr[i]; k < A->row_ptr[i + 1]; k++) {
+      const long long off = std::llabs((long long)A->cols[k] - i);
+      if (std::find(d.begin(), d.end(), off) == d.end()) {
+        if (d.size() == 16) return fail_plain(ISPM_ERR_INVALID, "symmetric plan: more than 16 upper diagonals");
+        d.push_back(off);
+      }
+    }
+  std::sort(d.begin(), d.end());
+  const unsigned K = (unsigned)d.size();
+  auto idx = [&](long long off) { return (unsigned)(std::lower_bound(d.begin(), d.end(), off) - d.begin()); };
+  /* upper entries into the planes, then every lower entry must equal its mirror */
+  std::vector<double> v((size_t)K * n_pad, 0.0);
+  std::vector<unsigned char> has((size_t)K * n_pad, 0);
+  long long nup = 0, nlo = 0;
+  for (int i = 0; i < n; i++)
+    for (int k = A->row_ptr[i]; k < A->row_ptr[i + 1]; k++) {
+      const int c = A->cols[k];
+      if (c < i) continue;
+      const size_t at = (size_t)idx(c - i) * n_pad + i;
+      if (has[at]) {
+        char m[96];
+        std::snprintf(m, sizeof m, "symmetric plan: duplicate entry (%d, %d)", i, c);
+        return fail_plain(ISPM_ERR_INVALID, m);
+      }
+      has[at] = 1;
+      v[at] = csr_value(A, k);
+      nup += c > i;
+    }
+  for (int i = 0; i < n; i++)
+    for (int k = A->row_ptr[i]; k < A->row_ptr[i + 1]; k++) {
+      const int c = A->cols[k];
+      if (c >= i) continue;
+      const size_t at = (size_t)idx(i - c) * n_pad + c;
+      if (!has[at] || has[at] == 2 || v[at] != csr_value(A, k)) {
+        char m[96];
+        std::snprintf(m, sizeof m, "symmetric plan: entry (%d, %d) differs from (%d, %d)", i, c, c, i);
+        return fail_plain(ISPM_ERR_INVALID, m);
+      }
+      has[at] = 2;
+      nlo++;
+    }
+  if (nup != nlo) return fail_plain(ISPM_ERR_INVALID, "symmetric plan: the matrix is not symmetric");
+  /* x-window pattern over the offsets -d[K-1] .. d[K-1] (positions relative to the slice's
+   * first row + 256, runs of offsets no more than 8 apart, as build_pattern makes them) */
+  std::vector<long long> R;
+  for (unsigned k = K; k-- > 1;) R.push_back(-d[k] + 256);
+  for (unsigned k = 0; k < K; k++) R.push_back(d[k] + 256);
+  sym_desc &sd = p->symd;
+  sd = sym_desc{};
+  diag_pattern &pat = sd.pat;
+  pat.n = (unsigned)R.size();
+  unsigned used = 0;
+  for (unsigned f = 0; f < pat.n;) {
+    unsigned l = f;
+    while (l + 1 < pat.n && R[l + 1] - R[l] <= 8) l++;
+    const unsigned span = (unsigned)(R[l] - R[f]);
+    pat.rrel[pat.nrun] = (int)(R[f] - 256);
+    pat.rlen[pat.nrun] = 256 + span;
+    pat.rbase[pat.nrun] = used;
+    for (unsigned q = f; q <= l; q++) pat.poff[q] = used + (unsigned)(R[q] - R[f]);
+    used += 256 + span;
+    pat.nrun++;
+    f = l + 1;
+  }
+  pat.lds_len = used;
+  if ((size_t)used * 8 > 65536) return fail_plain(ISPM_ERR_UNSUPPORTED, "symmetric plan: x windows exceed 64 KiB of LDS");
+  for (unsigned k = 0; k < K; k++) sd.d[k] = (int)d[k];
+  if (ns * K * 256 >= (1ull << 32) || n_pad >= (1ull << 31))
+    return fail_plain(ISPM_ERR_UNSUPPORTED, "symmetric plan: more than 2^32 element slots");
+  sd.K = K;
+  sd.n_slices = (unsigned)ns;
+  sd.n_cols = (unsigned)n;
+  sd.elm_type = et;
+  /* planes slice by slice: element (s K + k) 256 + r */
+  std::vector<unsigned char> eh(ns * K * 256 * tsize(et));
+  for (size_t s0 = 0; s0 < ns; s0++)
+    for (unsigned k = 0; k < K; k++)
+      for (unsigned r = 0; r < 256; r++)
+        store_value(eh.data(), et, (s0 * K + k) * 256 + r, v[(size_t)k * n_pad + s0 * 256 + r]);
+  void *de = nullptr;
+  int st = ispm_malloc(&de, eh.size());
+  if (st) return st;
+  if ((st = ispm_memcpy(de, eh.data(), eh.size(), ISPM_H2D))) { ispm_free(de); return st; }
+  sd.elms = de;
+  p->sym = true;
+  std::memset(&p->dev, 0, sizeof p->dev);
+  p->dev.n_rows = (int)n_pad;
+  p->dev.n_cols = n;
+  p->dev.n_nz = A->n_nz;
+  p->dev.n_slices = (unsigned)ns;
+  p->dev.slice_height = 256;
+  p->dev.diags = 1;
+  p->dev.elms = de;
+  p->dev.elms_len = ns * K * 256;
+  p->dev.elm_type = et;
+  p->has_dev = true;
+  p->L.n_rows = (int)n_pad;
+  p->L.n_cols = n;
+  p->L.S = 256;
+  p->L.D = true;
+  p->L.n_slices = (unsigned)ns;
+  p->launch = default_launch(p->L);
+  p->stored = tsize(et) * ns * K * 256 + tsize(p->T) * ((size_t)n + n_pad);
+  p->csr = csr_bytes(A, p->E, p->T);
+  return ISPM_OK;
+}
+}  // namespace ispm
+
 int ispm_plan_create(ispm_plan **out, int vt, int et, const ispm_csr *A, int kind, unsigned flags)
 {
   if (!out) return fail_plain(ISPM_ERR_INVALID, "plan_create: null output");
@@ -667,6 +787,12 @@ int ispm_plan_create(ispm_plan **out, int vt, int et, const ispm_csr *A, int kin
   ispm_plan *p = new ispm_plan;
   p->T = vt;
   p->E = et;
+  if (flags & ISPM_SPMV_PLAN_SYMMETRIC) {
+    st = create_sym(A, et, flags, p);
+    if (st) { ispm_plan_destroy(p); return st; }
+    *out = p;
+    return ISPM_OK;
+  }
   if (flags & ISPM_SPMV_PLAN_SPLIT_DIAG) {
     st = create_split(A, et, flags, p);
     if (st) { ispm_plan_destroy(p); return st; }
@@ -866,6 +992,7 @@ int ispm_plan_execute(ispm_plan *p, const void *x, void *y)
     const split_diag sd{p->dev_diag, p->dev_rmap, p->dev_rmask, p->split_n, p->split_m, p->E};
     return spmv_split(sd, &p->launch, &p->dev, p->T, x, y, nullptr, nullptr, nullptr, nullptr, &p->hints);
   }
+  if (p->sym) return spmv_sym(p->symd, p->T, x, y, nullptr);
   return spmv_internal(&p->launch, &p->dev, p->T, x, y, nullptr, p->has_pat ? &p->pat : nullptr,
                        p->dev_uni, &p->hints);
 }
@@ -874,14 +1001,14 @@ int ispm_plan_execute_multi(ispm_plan *p, int k, const void *X, size_t ldx, void
 {
   if (!p) return fail_plain(ISPM_ERR_INVALID, "plan_execute_multi: null plan");
   if (!p->has_dev) return fail_plain(ISPM_ERR_INVALID, "plan_execute_multi: plan has no device matrix");
-  if (p->split) return fail_plain(ISPM_ERR_UNSUPPORTED, "plan_execute_multi: split-diagonal plans multiply one vector");
+  if (plan_single_vector(p)) return fail_plain(ISPM_ERR_UNSUPPORTED, "plan_execute_multi: split-diagonal and symmetric plans multiply one vector");
   return ispm_spmm(&p->launch, &p->dev, p->T, k, X, ldx, Y, ldy);
 }
 
 int ispm_plan_update_coefs(ispm_plan *p, const ispm_csr *A)
 {
   if (!p) return fail_plain(ISPM_ERR_INVALID, "plan_update_coefs: null plan");
-  if (p->split) return fail_plain(ISPM_ERR_UNSUPPORTED, "plan_update_coefs: split-diagonal plans are rebuilt");
+  if (plan_single_vector(p)) return fail_plain(ISPM_ERR_UNSUPPORTED, "plan_update_coefs: split-diagonal and symmetric plans are rebuilt");
   int st = check_csr(A);
   if (st) return st;
   if ((long long)A->row_ptr[A->n_rows] != (long long)p->L.map.size())

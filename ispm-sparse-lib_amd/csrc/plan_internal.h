@@ -29,10 +29,15 @@ struct ispm_plan {
   unsigned *dev_rmask = nullptr;          /* bit per matrix row: a remainder row */
   unsigned split_n = 0, split_m = 0;      /* padded rows of the matrix; remainder rows */
   ispm::layout_hints hints{};             /* launch facts of the layout (streamed mapping) */
+  /* symmetric diagonal plan (ISPM_SPMV_PLAN_SYMMETRIC): dev.elms holds the upper planes */
+  bool sym = false;
+  ispm::sym_desc symd{};
 };
 
 /* rows (padded) of the matrix a plan multiplies */
 inline unsigned plan_rows(const ispm_plan *p) { return p->split ? p->split_n : (unsigned)p->L.n_rows; }
+/* a plan whose product is one vector at a time through its own kernels (no SpMM) */
+inline bool plan_single_vector(const ispm_plan *p) { return p->split || p->sym; }
 
 namespace ispm {
 size_t tsize(int t);

@@ -61,7 +61,7 @@ extern "C" int ispm_plan_update_coefs_device(ispm_plan *p, const void *vals, int
 {
   if (!p || !vals) return fail(ISPM_ERR_INVALID, "plan_update_coefs_device: null argument");
   if (!p->has_dev) return fail(ISPM_ERR_INVALID, "plan_update_coefs_device: plan has no device matrix");
-  if (p->split) return fail(ISPM_ERR_UNSUPPORTED, "plan_update_coefs_device: split-diagonal plans are rebuilt");
+  if (plan_single_vector(p)) return fail(ISPM_ERR_UNSUPPORTED, "plan_update_coefs_device: split-diagonal and symmetric plans are rebuilt");
   int st = ensure_init();
   if (st) return st;
   const long long nnz = (long long)p->L.map.size();

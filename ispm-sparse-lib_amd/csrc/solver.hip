@@ -379,6 +379,7 @@ int spmv(ispm_plan *P, int vt, const void *x, void *y, const unsigned *skip)
     const split_diag sd{P->dev_diag, P->dev_rmap, P->dev_rmask, P->split_n, P->split_m, P->E};
     return spmv_split(sd, &P->launch, &P->dev, vt, x, y, skip, nullptr, nullptr, nullptr, &P->hints);
   }
+  if (P->sym) return spmv_sym(P->symd, vt, x, y, skip);
   return spmv_internal(&P->launch, &P->dev, vt, x, y, skip, P->has_pat ? &P->pat : nullptr, P->dev_uni,
                        &P->hints);
 }
@@ -442,6 +443,8 @@ int spmv_dot(ispm_solver *S, ispm_plan *P, const void *x, void *y, const void *d
   if (P->split) {
     const split_diag sd{P->dev_diag, P->dev_rmap, P->dev_rmask, P->split_n, P->split_m, P->E};
     st = spmv_split(sd, &P->launch, &P->dev, S->T, x, y, skip, d, parts, &G, &P->hints);
+  } else if (P->sym) {
+    st = spmv_sym(P->symd, S->T, x, y, skip, d, parts, &G);
   } else {
     st = spmv_dot_internal(&P->launch, &P->dev, S->T, x, y, skip, P->has_pat ? &P->pat : nullptr,
                            P->dev_uni, d, parts, &G, &P->hints);

@@ -65,6 +65,27 @@ int spmv_split(const split_diag &sd, const ispm_launch *p, const ispm_slell_desc
   return fail(ISPM_ERR_UNSUPPORTED, "spmv: unsupported (vector, element) types (%d, %d)", vec_type, et);
 }
 
+int spmv_sym(const sym_desc &sd, int vec_type, const void *x, void *y, const unsigned *skip,
+             const void *dotv, double *dparts, unsigned *dgrid)
+{
+  if (dgrid) *dgrid = 0;
+  if (!x || !y || !sd.elms) return fail(ISPM_ERR_INVALID, "spmv: null argument");
+  if (sd.K < 1 || sd.K > spmvk::kSymPlanes) return fail(ISPM_ERR_INVALID, "spmv: %u symmetric planes", sd.K);
+  if (sd.n_slices == 0) return ISPM_OK;
+  spmvk::sym_args a{};
+  a.elms = sd.elms;
+  a.n_slices = sd.n_slices;
+  a.n_cols = sd.n_cols;
+  a.K = sd.K;
+  for (unsigned k = 0; k < spmvk::kSymPlanes; k++) a.d[k] = sd.d[k];
+  const int et = sd.elm_type;
+  if (vec_type == ISPM_F64 && et == ISPM_F64) return spmvk::launch_symdia<double, double>(a, sd.pat, x, y, skip, dotv, dparts, dgrid);
+  if (vec_type == ISPM_F64 && et == ISPM_F32) return spmvk::launch_symdia<double, float>(a, sd.pat, x, y, skip, dotv, dparts, dgrid);
+  if (vec_type == ISPM_F32 && et == ISPM_F32) return spmvk::launch_symdia<float, float>(a, sd.pat, x, y, skip, dotv, dparts, dgrid);
+  if (vec_type == ISPM_F32 && et == ISPM_BF16) return spmvk::launch_symdia<float, spmvk::bf16_t>(a, sd.pat, x, y, skip, dotv, dparts, dgrid);
+  return fail(ISPM_ERR_UNSUPPORTED, "spmv: unsupported (vector, element) types (%d, %d)", vec_type, et);
+}
+
 /* SpMM for the block solver: the checks of ispm_spmm, with the plan's layout facts */
 int spmm_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, int k,
                   const void *X, size_t ldx, void *Y, size_t ldy, const layout_hints *hints)

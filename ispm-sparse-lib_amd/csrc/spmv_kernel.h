@@ -497,6 +497,145 @@ int launch_diagpat(const ispm_slell_desc *A, const diag_pattern &pat, const unsi
   return diagpat_launch<T, E, kPatPlanes>(A, H, g, b, lds, a, c, uni);
 }
 
+/* ---- symmetric diagonal storage (ISPM_SPMV_PLAN_SYMMETRIC) ----
+ * A symmetric matrix whose entries lie on a few diagonals (stencils) stores only its upper
+ * diagonals d_0 = 0 < d_1 < ... < d_{K-1}: plane k of 256-row slice s holds A[row, row + d_k]
+ * at elms[(s K + k) 256 + r] (zero where the row has no such entry).  The lower entries are
+ * the mirrors: A[row, row - d_k] = A[row - d_k, row] = plane k at row - d_k, which an earlier
+ * slice stored and read moments ago, so it comes from the cache while only the K upper planes
+ * stream from HBM (half a 27-point stencil's bytes).  One workgroup per slice, one row per
+ * thread; x is read from LDS windows as in the pattern kernel (the pattern lists all 2K-1
+ * column offsets). */
+constexpr unsigned kSymPlanes = 16;
+
+struct sym_args {
+  const void *elms;
+  const void *x;
+  void *y;
+  unsigned n_slices, n_cols, K, per_xcd;
+  int d[kSymPlanes];             /* upper diagonal offsets, d[0] = 0 */
+  unsigned up_off[kSymPlanes];   /* LDS index of x[row0 + d[k]] (windows as launched) */
+  unsigned lo_off[kSymPlanes];   /* LDS index of x[row0 - d[k]] (k >= 1) */
+  const unsigned *skip;
+  const void *dotv;
+  double *dparts;
+};
+
+/* one element, read through the caches (the mirror reads find it there) */
+template<typename T, typename E> __device__ __forceinline__ T ld_cached(const E *p, size_t i) { return (T)p[i]; }
+template<> __device__ __forceinline__ float ld_cached<float, dev::bf16_t>(const dev::bf16_t *p, size_t i) { return dev::bf16_to_f32(p[i]); }
+
+template<typename T, typename E, unsigned NK>
+__global__ void __launch_bounds__(256) symdia_kernel(sym_args a, diag_pattern pat)
+{
+  extern __shared__ __align__(16) unsigned char sym_lds[];
+  T *win = reinterpret_cast<T *>(sym_lds);
+  constexpr unsigned S = 256, NT = 256, VX = 16 / sizeof(T), NR = 9;
+  const unsigned t = threadIdx.x;
+  const unsigned slice = (blockIdx.x & 7u) * a.per_xcd + (blockIdx.x >> 3);
+  if (a.skip && *a.skip) return;
+  if (slice >= a.n_slices) {
+    if (a.dparts && t == 0) a.dparts[blockIdx.x] = 0.0;
+    return;
+  }
+  const E *__restrict__ el = static_cast<const E *>(a.elms);
+  const T *__restrict__ xv = static_cast<const T *>(a.x);
+  T *__restrict__ yv = static_cast<T *>(a.y);
+  const unsigned ncols = a.n_cols, K = a.K;
+  const unsigned row0 = slice * S, row = row0 + t;
+  const bool xvec = (reinterpret_cast<uintptr_t>(xv) % 16) == 0;
+  const bool fast = xvec && pat.vec_runs && pat.nrun <= NR && (long long)row0 + pat.cover_lo >= 0 &&
+                    (long long)row0 + pat.cover_hi <= (long long)ncols;
+  vecx<T> w[NR];
+  if (fast) {
+#pragma unroll
+    for (unsigned r = 0; r < NR; r++) {
+      const unsigned rr = r < pat.nrun ? r : 0u;
+      const unsigned nv = pat.rlen[rr] / VX, v = t < nv ? t : nv - 1;
+      w[r] = *reinterpret_cast<const vecx<T> *>(xv + (row0 + (unsigned)pat.rrel[rr] + v * VX));
+    }
+  }
+  /* own upper planes and the mirrored lower ones, all requested before any wait (element
+   * indices fit 32 bits: the plan checks it) */
+  T ev[NK], mv[NK];
+  const unsigned KS = K * S, own = slice * KS + t;
+#pragma unroll
+  for (unsigned k = 0; k < NK; k++) {
+    ev[k] = k < K ? ld_cached<T, E>(el, own + k * S) : (T)0;
+    mv[k] = 0;
+    if (k >= 1 && k < K) {
+      const int j = (int)row - a.d[k];
+      if (j >= 0) mv[k] = ld_cached<T, E>(el, ((unsigned)j >> 8) * KS + k * S + ((unsigned)j & 255u));
+    }
+  }
+  if (fast) {
+#pragma unroll
+    for (unsigned r = 0; r < NR; r++)
+      if (r < pat.nrun && t < pat.rlen[r] / VX) *reinterpret_cast<vecx<T> *>(win + pat.rbase[r] + t * VX) = w[r];
+  } else {
+    for (unsigned r = 0; r < pat.nrun; r++) {
+      const long long x0 = (long long)row0 + pat.rrel[r];
+      const unsigned L = pat.rlen[r], b = pat.rbase[r];
+      for (unsigned i = t; i < L; i += NT) {
+        const long long c = x0 + i;
+        win[b + i] = (c >= 0 && c < (long long)ncols) ? xv[c] : (T)0;
+      }
+    }
+  }
+  __syncthreads();
+  T acc = 0;
+#pragma unroll
+  for (unsigned k = 0; k < NK; k++)
+    if (k < K) {
+      acc += ev[k] * win[a.up_off[k] + t];
+      if (k >= 1) acc += mv[k] * win[a.lo_off[k] + t];
+    }
+  yv[row] = acc;
+  if (a.dparts) {
+    __shared__ double red[4];
+    const T *__restrict__ dv = static_cast<const T *>(a.dotv);
+    double d[1] = {(double)dv[row] * (double)acc};
+    dev::block_sum<256, 1>(d, red);
+    if (t == 0) a.dparts[blockIdx.x] = d[0];
+  }
+}
+
+template<typename T, typename E>
+int launch_symdia(const sym_args &a0, const diag_pattern &pat0, const void *x, void *y,
+                  const unsigned *skip, const void *dotv, double *dparts, unsigned *dgrid)
+{
+  sym_args a = a0;
+  a.x = x;
+  a.y = y;
+  a.skip = skip;
+  a.per_xcd = (a.n_slices + 7) / 8;
+  a.dotv = nullptr;
+  a.dparts = nullptr;
+  if (dgrid) *dgrid = 0;
+  if (dotv && dparts && dgrid && a.per_xcd * 8 <= kMaxDotParts) {
+    a.dotv = dotv;
+    a.dparts = dparts;
+    *dgrid = a.per_xcd * 8;
+  }
+  diag_pattern c;
+  cover_pattern(pat0, 16 / sizeof(T), 256, 9, c);
+  const size_t lds = (size_t)c.lds_len * sizeof(T);
+  if (lds > 65536) return fail(ISPM_ERR_UNSUPPORTED, "spmv: symmetric windows need %zu bytes of LDS", lds);
+  /* pattern positions of +d[k] and -d[k] (the plan lists -d[K-1] .. -d[1], then d[0] ..
+   * d[K-1]) turned into LDS indices of the windows as launched */
+  for (unsigned k = 0; k < a.K; k++) {
+    a.up_off[k] = c.poff[(a.K - 1) + k];
+    a.lo_off[k] = k ? c.poff[(a.K - 1) - k] : 0u;
+  }
+  const dim3 g(a.per_xcd * 8), b(256);
+  hipStream_t st = stream();
+  if (a.K <= 4) hipLaunchKernelGGL((symdia_kernel<T, E, 4>), g, b, lds, st, a, c);
+  else if (a.K <= 10) hipLaunchKernelGGL((symdia_kernel<T, E, 10>), g, b, lds, st, a, c);
+  else hipLaunchKernelGGL((symdia_kernel<T, E, kSymPlanes>), g, b, lds, st, a, c);
+  ISPM_HIP(hipGetLastError());
+  return ISPM_OK;
+}
+
 /* pattern path: a plan's diagonal fixed-height slices, one row per thread, single vector */
 inline bool use_diagpat(const ispm_launch *p, const ispm_slell_desc *A, int vt, unsigned H,
                         const diag_pattern *pat, const unsigned *uni)

@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(PKG_DIR, "libispm0.so")
 F32, F64, BF16 = 0, 1, 2
 H2D, D2H, D2D, DEFAULT = 0, 1, 2, 3
 PLAN_DEFAULT, PLAN_ESTIMATE, PLAN_EXHAUSTIVE = 0, 1, 2
-SPMV_NO_TEXTURE_MEMORY, SPMV_PLAN_NO_DEVICE, SPMV_PLAN_SPLIT_DIAG = 1, 2, 4
+SPMV_NO_TEXTURE_MEMORY, SPMV_PLAN_NO_DEVICE, SPMV_PLAN_SPLIT_DIAG, SPMV_PLAN_SYMMETRIC = 1, 2, 4, 8
 
 _NP = {F32: np.float32, F64: np.float64, BF16: np.uint16}
 _TSIZE = {F32: 4, F64: 8, BF16: 2}
@@ -434,10 +434,24 @@ class Plan:
         check(lib().ispm_plan_info_get(self.p, C.byref(inf)))
         return inf
 
+    def _check_vectors(self, need_x, need_y, *arrays):
+        """The C-ABI takes bare device pointers: refuse arrays of another element type than
+        the plan's vectors, or shorter than the product reads (x) and writes (y)."""
+        want = _NP[self.vec_type]
+        for a, need in zip(arrays, (need_x, need_y)):
+            if a.dtype != want:
+                raise ValueError(f"plan vectors are {np.dtype(want).name}, got {a.dtype.name}")
+            if a.n < need:
+                raise ValueError(f"device array of {a.n} elements, the product needs {need}")
+
     def execute(self, x: DeviceArray, y: DeviceArray):
+        inf = self.info()
+        self._check_vectors(inf.device.n_cols, self.n_rows_padded, x, y)
         check(lib().ispm_plan_execute(self.p, x.vp, y.vp))
 
     def execute_multi(self, k, X: DeviceArray, ldx, Y: DeviceArray, ldy):
+        inf = self.info()
+        self._check_vectors((k - 1) * ldx + inf.device.n_cols, (k - 1) * ldy + self.n_rows_padded, X, Y)
         check(lib().ispm_plan_execute_multi(self.p, k, X.vp, ldx, Y.vp, ldy))
 
     def update_coefs(self, A: CSR):

@@ -161,7 +161,9 @@ def stencil(nx: int, ny: int, nz: int, points: int = 7, anisotropy: float = 1.0,
         lin = o[2] * nx * ny + o[1] * nx + o[0]
         nbr = np.clip(idx_ + lin, 0, n - 1)
         kn = np.where(mask, kappa[nbr], kappa)
-        w = cls[abs(o[0]) + abs(o[1]) + abs(o[2])] * anisotropy ** abs(o[2]) * 2 * kappa * kn / (kappa + kn)
+        # harmonic mean from the symmetric product and sum, so that w(i, j) == w(j, i) exactly
+        hm = 2 * (kappa * kn) / (kappa + kn)
+        w = cls[abs(o[0]) + abs(o[1]) + abs(o[2])] * anisotropy ** abs(o[2]) * hm
         diag[:] += w
         return np.where(mask, -w, 0.0)
 

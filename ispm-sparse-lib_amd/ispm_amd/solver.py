@@ -6,8 +6,8 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import (BF16, CSR, F32, F64, PLAN_ESTIMATE, SPMV_PLAN_SPLIT_DIAG, DeviceArray, Plan, check,
-               csr_multiply, lib)
+from . import (BF16, CSR, F32, F64, PLAN_ESTIMATE, SPMV_PLAN_SPLIT_DIAG, SPMV_PLAN_SYMMETRIC,
+               DeviceArray, IspmError, Plan, check, csr_multiply, lib)
 
 PPCG, PCG, PCG1 = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
@@ -59,8 +59,11 @@ def preconditioner_matrices(M, product=None):
 
 
 def apply_seconds(plans, vt, n, reps=9):
-    """Median time of applying the preconditioner plans in turn (device events)."""
+    """Median time of applying the plans in turn (device events), with vectors of the plans'
+    own type (vt must be it)."""
     from . import Event, synchronize
+    if any(p.vec_type != vt for p in plans):
+        raise ValueError("apply_seconds: plans of another vector type")
     dt = np.float64 if vt == F64 else np.float32
     bufs = [DeviceArray.from_numpy(np.ones(n, dt)), DeviceArray(n, dt), DeviceArray(n, dt)]
 
@@ -90,17 +93,35 @@ class Solver:
     residual replacement.  M: (Ainv1, Ainv2) from fastainv_sym, applied as Ainv2 (Ainv1 r) or
     as their product (see preconditioner_matrices; fsai_product chooses), or one CSR matrix;
     each matrix as a SlELL plan or a split-diagonal plan (split_diag chooses; None: the
-    faster, timed at set-up)."""
+    faster, timed at set-up).  A symmetric matrix on few diagonals may use symmetric diagonal
+    storage for A (symmetric chooses; None: the faster, timed at set-up)."""
 
     def __init__(self, A: CSR, M=None, precision="fp64", method=None, tol=1e-10, maxit=100000,
                  check_every=16, use_graph=True, reliable_delta=0.01, factor_type=None,
-                 A_plan=None, M_plans=None, fsai_product=None, split_diag=None):
+                 A_plan=None, M_plans=None, fsai_product=None, split_diag=None, symmetric=None):
         L = _bind()
         mixed = precision == "mixed"
         vt = F32 if mixed else F64
         if method is None:
             method = PCG if mixed else PPCG
-        self.A_plan = A_plan or Plan(A, F64, elm_type=F32 if mixed else F64)
+        self.A_plan = A_plan
+        if A_plan is None:
+            # A as a SlELL plan, or (symmetric stencil-like matrices) in symmetric diagonal
+            # storage that streams half the matrix: the faster of the two with the solve's
+            # vectors (the mixed solve's FP64 residuals use the same plan with FP64 vectors)
+            ea = F32 if mixed else F64
+            cands = [] if symmetric else [Plan(A, vt, elm_type=ea)]
+            if symmetric is not False:
+                try:
+                    cands.append(Plan(A, vt, elm_type=ea, flags=SPMV_PLAN_SYMMETRIC))
+                except IspmError:
+                    if symmetric:
+                        raise
+            self.A_plan = cands[0]
+            if len(cands) > 1:
+                n = cands[0].n_rows_padded
+                t = [apply_seconds((c,), vt, n) for c in cands]
+                self.A_plan = cands[int(np.argmin(t))]
         self.M_plans = None
         if M_plans is not None:
             self.M_plans = M_plans
@@ -145,6 +166,9 @@ class Solver:
             check(L.ispm_solver_set_residual_plan(self.p, self.res_plan.p))
 
     def solve_device(self, b: DeviceArray, x: DeviceArray, allow_fail=False):
+        for a in (b, x):
+            if a.dtype != np.float64 or a.n < self.n:
+                raise ValueError(f"b and x must be FP64 device arrays of at least {self.n} elements")
         st = SolveStats()
         code = lib().ispm_solver_solve(self.p, b.vp, x.vp, C.byref(st))
         if code != 0 and not (allow_fail and code == 6):

@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import numpy as np
+
 import ispm_amd as I
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -48,3 +50,22 @@ def test_split_diagonal_plan_argument_checks():
     B = I.CSR(2, 2, [0, 1, 2], [0, 1], [1.0, 2.0])
     with pytest.raises(I.IspmError):
         I.Plan(B, I.F64, flags=I.SPMV_PLAN_SPLIT_DIAG | I.SPMV_PLAN_NO_DEVICE)
+
+
+def test_symmetric_plan_argument_checks():
+    """Symmetric diagonal plans need an exactly symmetric square matrix on at most 16 upper
+    diagonals, and the device; the checks run before any device work."""
+    import pytest
+    B = I.CSR(3, 3, [0, 2, 3, 4], [0, 1, 1, 2], [2.0, 1.0, 2.0, 2.0])   # (0,1) without (1,0)
+    with pytest.raises(I.IspmError):
+        I.Plan(B, I.F64, flags=I.SPMV_PLAN_SYMMETRIC)
+    C_ = I.CSR(2, 2, [0, 2, 4], [0, 1, 0, 1], [2.0, 1.0, 1.5, 2.0])   # (0,1) != (1,0)
+    with pytest.raises(I.IspmError):
+        I.Plan(C_, I.F64, flags=I.SPMV_PLAN_SYMMETRIC)
+    n = 40
+    D = I.CSR(n, n, np.arange(n + 1) * n, np.tile(np.arange(n, dtype=np.int32), n), np.ones(n * n))  # dense: 40 diagonals
+    with pytest.raises(I.IspmError):
+        I.Plan(D, I.F64, flags=I.SPMV_PLAN_SYMMETRIC)
+    E_ = I.CSR(2, 2, [0, 2, 4], [0, 1, 0, 1], [2.0, 1.0, 1.0, 2.0])   # symmetric, but no device
+    with pytest.raises(I.IspmError):
+        I.Plan(E_, I.F64, flags=I.SPMV_PLAN_SYMMETRIC | I.SPMV_PLAN_NO_DEVICE)

@@ -61,6 +61,19 @@ def test_split_diagonal_preconditioner(gpu, prec, method, product):
 
 
 @pytest.mark.parametrize("prec,method", VARIANTS)
+def test_symmetric_storage_of_A(gpu, prec, method):
+    """A in symmetric diagonal storage (the fused dot products of its kernel included), the
+    FP64 residual of the mixed solve through the same plan with FP64 vectors."""
+    A = G.stencil27_contrast(32)
+    b = O.csr_spmv(A, G.uniform(2, A.n_rows))
+    M = I.fastainv_sym(A, I.F64, droptol=0.125)
+    ref = S.solve(A, b, M=M, tol=1e-10, precision=prec, method=method, symmetric=False)
+    res = S.solve(A, b, M=M, tol=1e-10, precision=prec, method=method, symmetric=True)
+    assert res.converged and oracle_relres(A, b, res.x) <= 1e-10
+    assert abs(res.iterations - ref.iterations) <= max(3, ref.iterations // 20)
+
+
+@pytest.mark.parametrize("prec,method", VARIANTS)
 def test_no_preconditioner_and_nonzero_guess(gpu, prec, method):
     A = G.random_sparse(3000, 41, max_len=12, spd=True)
     b = O.csr_spmv(A, G.uniform(3, A.n_rows))

@@ -325,6 +325,49 @@ def test_split_diagonal_refuses_spmm_and_refresh(mats):
         p.execute_multi(2, X, n, X, n)
 
 
+def _sym_matrix(name):
+    if name == "poisson7":
+        return G.poisson7(16)
+    if name == "stencil27_contrast":
+        return G.stencil27_contrast(24)
+    if name == "stencil19_aniso":
+        return G.stencil(20, 18, 16, 19, anisotropy=10, contrast=100, seed=3)
+    return G.stencil(7, 9, 11, 27, seed=4)      # 693 rows: one padded slice, clipped windows
+
+
+@pytest.mark.parametrize("vt,et", TYPES)
+@pytest.mark.parametrize("name", ["poisson7", "stencil27_contrast", "stencil19_aniso", "small27"])
+def test_symmetric_plan(gpu, name, vt, et):
+    """Symmetric diagonal plans (upper planes stored, lower entries read from their mirror
+    rows) give the CSR product; padded rows stay zero."""
+    A = _sym_matrix(name)
+    p = I.Plan(A, vt, elm_type=et, flags=I.SPMV_PLAN_SYMMETRIC)
+    inf = p.info()
+    assert inf.S == 256 and inf.D == 1 and p.n_rows_padded % 512 == 0
+    x = G.uniform(31, A.n_rows)
+    check(A, None, x, run_plan(p, A, x, vt), vt, et)
+
+
+def test_symmetric_plan_integer_exact(gpu):
+    A = G.stencil(12, 10, 9, 27, seed=5)
+    Ai = I.CSR(A.n_rows, A.n_cols, A.row_ptr, A.cols, np.round(A.vals * 8))
+    x = np.round(G.uniform(32, A.n_rows) * 8)
+    p = I.Plan(Ai, I.F64, flags=I.SPMV_PLAN_SYMMETRIC)
+    check(Ai, None, x, run_plan(p, Ai, x, I.F64), I.F64, I.F64, integer=True)
+
+
+def test_plan_refuses_mismatched_vectors(mats):
+    """The C-ABI takes bare pointers; the Python plan refuses arrays of the wrong element
+    type or length before anything is launched."""
+    A = mats["poisson7"]
+    p = I.Plan(A, I.F64)
+    n = p.n_rows_padded
+    with pytest.raises(ValueError):
+        p.execute(I.DeviceArray(n, np.float32), I.DeviceArray(n, np.float32))
+    with pytest.raises(ValueError):
+        p.execute(I.DeviceArray(n), I.DeviceArray(n - 1))
+
+
 def test_plan_default_and_update_coefs(mats):
     A = mats["poisson7"]
     p = I.Plan(A, I.F64)
@@ -365,6 +408,29 @@ def test_full_size_27pt_192(gpu, vt, et):
     else:
         assert np.all(np.abs(y - rs) <= 1e-5 * O.csr_spmv_abs(A, np.ones(A.n_rows)))
     x = G.uniform(7, A.n_rows)
+    xd = I.DeviceArray.from_numpy(np.concatenate([x, np.zeros(n - A.n_rows)]).astype(dt))
+    p.execute(xd, yd)
+    check(A, None, x, yd.numpy(), vt, et)
+
+
+@pytest.mark.parametrize("vt,et", [(I.F64, I.F64), (I.F64, I.F32), (I.F32, I.F32)])
+def test_full_size_27pt_192_symmetric(gpu, vt, et):
+    """The symmetric diagonal plan of the full 27-point 192^3 matrix: exact A*1 in FP64 and
+    the per-entry bound on a random x."""
+    A = _big()
+    p = I.Plan(A, vt, elm_type=et, flags=I.SPMV_PLAN_SYMMETRIC)
+    n = p.n_rows_padded
+    dt = np.float64 if vt == I.F64 else np.float32
+    ones = I.DeviceArray.from_numpy(np.ones(n, dt))
+    yd = I.DeviceArray(n, dt)
+    p.execute(ones, yd)
+    y = yd.numpy()[:A.n_rows].astype(np.float64)
+    rs = O.csr_spmv(A, np.ones(A.n_rows))
+    if vt == I.F64:
+        np.testing.assert_array_equal(y, rs)
+    else:
+        assert np.all(np.abs(y - rs) <= 1e-5 * O.csr_spmv_abs(A, np.ones(A.n_rows)))
+    x = G.uniform(8, A.n_rows)
     xd = I.DeviceArray.from_numpy(np.concatenate([x, np.zeros(n - A.n_rows)]).astype(dt))
     p.execute(xd, yd)
     check(A, None, x, yd.numpy(), vt, et)
