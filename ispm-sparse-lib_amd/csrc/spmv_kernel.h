@@ -503,9 +503,9 @@ int launch_diagpat(const ispm_slell_desc *A, const diag_pattern &pat, const unsi
  * at elms[(s K + k) 256 + r] (zero where the row has no such entry).  The lower entries are
  * the mirrors: A[row, row - d_k] = A[row - d_k, row] = plane k at row - d_k, which an earlier
  * slice stored and read moments ago, so it comes from the cache while only the K upper planes
- * stream from HBM (half a 27-point stencil's bytes).  One workgroup per slice, one row per
- * thread; x is read from LDS windows as in the pattern kernel (the pattern lists all 2K-1
- * column offsets). */
+ * stream from HBM (half a 27-point stencil's bytes).  One workgroup per slice, two
+ * consecutive rows per thread for FP32 vectors and one for FP64; x is read from LDS windows
+ * as in the pattern kernel (the pattern lists all 2K-1 column offsets). */
 constexpr unsigned kSymPlanes = 16;
 
 struct sym_args {
@@ -513,6 +513,9 @@ struct sym_args {
   const void *x;
   void *y;
   unsigned n_slices, n_cols, K, per_xcd;
+  /* level order (band != 0): slices s = z P + q; XCD x takes positions q in
+   * [x P / 8, (x + 1) P / 8) and sweeps the Z full levels z, then the slices past them */
+  unsigned band, P, Z;
   int d[kSymPlanes];             /* upper diagonal offsets, d[0] = 0 */
   unsigned up_off[kSymPlanes];   /* LDS index of x[row0 + d[k]] (windows as launched) */
   unsigned lo_off[kSymPlanes];   /* LDS index of x[row0 - d[k]] (k >= 1) */
@@ -521,18 +524,65 @@ struct sym_args {
   double *dparts;
 };
 
-/* one element, read through the caches (the mirror reads find it there) */
-template<typename T, typename E> __device__ __forceinline__ T ld_cached(const E *p, size_t i) { return (T)p[i]; }
-template<> __device__ __forceinline__ float ld_cached<float, dev::bf16_t>(const dev::bf16_t *p, size_t i) { return dev::bf16_to_f32(p[i]); }
+/* Element reads of the symmetric kernel go through the caches (the mirror reads find them
+ * there).  H consecutive elements of one plane: sym_rows from an H-aligned index, sym_rows_any
+ * from any index (a 4-byte aligned FP32 pair is one 8-byte load). */
+template<typename T, typename E> __device__ __forceinline__ T ld_cached(const E *p, unsigned i) { return (T)p[i]; }
+template<> __device__ __forceinline__ float ld_cached<float, dev::bf16_t>(const dev::bf16_t *p, unsigned i) { return dev::bf16_to_f32(p[i]); }
+
+template<typename T, typename E, unsigned H> struct sym_rows {
+  static __device__ __forceinline__ void get(const E *p, unsigned i, T (&v)[H])
+  {
+#pragma unroll
+    for (unsigned e = 0; e < H; e++) v[e] = ld_cached<T, E>(p, i + e);
+  }
+  static __device__ __forceinline__ void get_any(const E *p, unsigned i, T (&v)[H]) { get(p, i, v); }
+};
+template<> struct sym_rows<float, float, 2> {
+  typedef float f2a __attribute__((ext_vector_type(2)));
+  typedef float f2u __attribute__((ext_vector_type(2))) __attribute__((aligned(4)));
+  static __device__ __forceinline__ void get(const float *p, unsigned i, float (&v)[2])
+  {
+    const f2a q = *reinterpret_cast<const f2a *>(p + i);
+    v[0] = q.x; v[1] = q.y;
+  }
+  static __device__ __forceinline__ void get_any(const float *p, unsigned i, float (&v)[2])
+  {
+    const f2u q = *reinterpret_cast<const f2u *>(p + i);
+    v[0] = q.x; v[1] = q.y;
+  }
+};
+template<> struct sym_rows<float, dev::bf16_t, 2> {
+  static __device__ __forceinline__ void get(const dev::bf16_t *p, unsigned i, float (&v)[2])
+  {
+    const unsigned q = *reinterpret_cast<const unsigned *>(p + i);
+    v[0] = __uint_as_float(q << 16);
+    v[1] = __uint_as_float(q & 0xffff0000u);
+  }
+  static __device__ __forceinline__ void get_any(const dev::bf16_t *p, unsigned i, float (&v)[2])
+  {
+    v[0] = dev::bf16_to_f32(p[i]);
+    v[1] = dev::bf16_to_f32(p[i + 1]);
+  }
+};
+
+/* rows per thread: two for FP32 vectors (8-byte element pairs halve the load requests, which
+ * bound the FP32 kernel), one for FP64 */
+template<typename T> constexpr unsigned sym_rows_per_thread() { return sizeof(T) == 4 ? 2u : 1u; }
 
 template<typename T, typename E, unsigned NK>
-__global__ void __launch_bounds__(256) symdia_kernel(sym_args a, diag_pattern pat)
+__global__ void __launch_bounds__(256 / sym_rows_per_thread<T>()) symdia_kernel(sym_args a, diag_pattern pat)
 {
   extern __shared__ __align__(16) unsigned char sym_lds[];
   T *win = reinterpret_cast<T *>(sym_lds);
-  constexpr unsigned S = 256, NT = 256, VX = 16 / sizeof(T), NR = 9;
+  constexpr unsigned S = 256, H = sym_rows_per_thread<T>(), NT = S / H, VX = 16 / sizeof(T), NR = 9;
   const unsigned t = threadIdx.x;
-  const unsigned slice = (blockIdx.x & 7u) * a.per_xcd + (blockIdx.x >> 3);
+  const unsigned xcd = blockIdx.x & 7u, i = blockIdx.x >> 3;
+  unsigned slice = xcd * a.per_xcd + i;
+  if (a.band) {
+    const unsigned q0 = xcd * a.P / 8, w = (xcd + 1) * a.P / 8 - q0;
+    slice = i < w * a.Z ? (i / w) * a.P + q0 + i % w : a.Z * a.P + (i - w * a.Z) * 8 + xcd;
+  }
   if (a.skip && *a.skip) return;
   if (slice >= a.n_slices) {
     if (a.dparts && t == 0) a.dparts[blockIdx.x] = 0.0;
@@ -542,7 +592,7 @@ __global__ void __launch_bounds__(256) symdia_kernel(sym_args a, diag_pattern pa
   const T *__restrict__ xv = static_cast<const T *>(a.x);
   T *__restrict__ yv = static_cast<T *>(a.y);
   const unsigned ncols = a.n_cols, K = a.K;
-  const unsigned row0 = slice * S, row = row0 + t;
+  const unsigned row0 = slice * S, row = row0 + H * t;
   const bool xvec = (reinterpret_cast<uintptr_t>(xv) % 16) == 0;
   const bool fast = xvec && pat.vec_runs && pat.nrun <= NR && (long long)row0 + pat.cover_lo >= 0 &&
                     (long long)row0 + pat.cover_hi <= (long long)ncols;
@@ -555,17 +605,26 @@ __global__ void __launch_bounds__(256) symdia_kernel(sym_args a, diag_pattern pa
       w[r] = *reinterpret_cast<const vecx<T> *>(xv + (row0 + (unsigned)pat.rrel[rr] + v * VX));
     }
   }
-  /* own upper planes and the mirrored lower ones, all requested before any wait (element
-   * indices fit 32 bits: the plan checks it) */
-  T ev[NK], mv[NK];
-  const unsigned KS = K * S, own = slice * KS + t;
+  /* own upper planes (rows row .. row+H-1) and the mirrored lower ones, all requested before
+   * any wait; element indices fit 32 bits (the plan checks it).  A mirror pair that crosses a
+   * slice boundary reads one element past the slice's plane (still inside the array: the
+   * mirror rows lie below the last row) and takes its second value from the next slice. */
+  T ev[NK][H], mv[NK][H];
+  const unsigned KS = K * S, own = slice * KS + H * t;
 #pragma unroll
   for (unsigned k = 0; k < NK; k++) {
-    ev[k] = k < K ? ld_cached<T, E>(el, own + k * S) : (T)0;
-    mv[k] = 0;
+#pragma unroll
+    for (unsigned e = 0; e < H; e++) ev[k][e] = mv[k][e] = 0;
+    if (k < K) sym_rows<T, E, H>::get(el, own + k * S, ev[k]);
     if (k >= 1 && k < K) {
       const int j = (int)row - a.d[k];
-      if (j >= 0) mv[k] = ld_cached<T, E>(el, ((unsigned)j >> 8) * KS + k * S + ((unsigned)j & 255u));
+      if (j >= 0) {
+        sym_rows<T, E, H>::get_any(el, ((unsigned)j >> 8) * KS + k * S + ((unsigned)j & 255u), mv[k]);
+        if (H == 2 && ((unsigned)j & 255u) == 255u)
+          mv[k][H - 1] = ld_cached<T, E>(el, (((unsigned)j + 1) >> 8) * KS + k * S);
+      } else if (H == 2 && j == -1) {
+        mv[k][H - 1] = ld_cached<T, E>(el, k * S);
+      }
     }
   }
   if (fast) {
@@ -576,26 +635,34 @@ __global__ void __launch_bounds__(256) symdia_kernel(sym_args a, diag_pattern pa
     for (unsigned r = 0; r < pat.nrun; r++) {
       const long long x0 = (long long)row0 + pat.rrel[r];
       const unsigned L = pat.rlen[r], b = pat.rbase[r];
-      for (unsigned i = t; i < L; i += NT) {
-        const long long c = x0 + i;
-        win[b + i] = (c >= 0 && c < (long long)ncols) ? xv[c] : (T)0;
+      for (unsigned q = t; q < L; q += NT) {
+        const long long c = x0 + q;
+        win[b + q] = (c >= 0 && c < (long long)ncols) ? xv[c] : (T)0;
       }
     }
   }
   __syncthreads();
-  T acc = 0;
+  T acc[H];
+#pragma unroll
+  for (unsigned e = 0; e < H; e++) acc[e] = 0;
 #pragma unroll
   for (unsigned k = 0; k < NK; k++)
     if (k < K) {
-      acc += ev[k] * win[a.up_off[k] + t];
-      if (k >= 1) acc += mv[k] * win[a.lo_off[k] + t];
+#pragma unroll
+      for (unsigned e = 0; e < H; e++) {
+        acc[e] += ev[k][e] * win[a.up_off[k] + H * t + e];
+        if (k >= 1) acc[e] += mv[k][e] * win[a.lo_off[k] + H * t + e];
+      }
     }
-  yv[row] = acc;
+#pragma unroll
+  for (unsigned e = 0; e < H; e++) yv[row + e] = acc[e];
   if (a.dparts) {
     __shared__ double red[4];
     const T *__restrict__ dv = static_cast<const T *>(a.dotv);
-    double d[1] = {(double)dv[row] * (double)acc};
-    dev::block_sum<256, 1>(d, red);
+    double d[1] = {0.0};
+#pragma unroll
+    for (unsigned e = 0; e < H; e++) d[0] += (double)dv[row + e] * (double)acc[e];
+    dev::block_sum<NT, 1>(d, red);
     if (t == 0) a.dparts[blockIdx.x] = d[0];
   }
 }
@@ -609,6 +676,17 @@ int launch_symdia(const sym_args &a0, const diag_pattern &pat0, const void *x, v
   a.y = y;
   a.skip = skip;
   a.per_xcd = (a.n_slices + 7) / 8;
+  /* Level order when the widest offset spans P slices that split (nearly) evenly over the
+   * XCDs: the mirrors a slice reads P slices back were then read by the same XCD one band
+   * of P / 8 slices earlier instead of P, and stay in its L2. */
+  const unsigned P = (unsigned)(a.d[a.K - 1] / 256);
+  a.band = 0;
+  if (P >= 8 && (P % 8 == 0 || P >= 128) && a.n_slices / P >= 2) {
+    a.band = 1;
+    a.P = P;
+    a.Z = a.n_slices / P;
+    a.per_xcd = (P + 7) / 8 * a.Z + (a.n_slices - a.Z * P + 7) / 8;
+  }
   a.dotv = nullptr;
   a.dparts = nullptr;
   if (dgrid) *dgrid = 0;
@@ -617,8 +695,9 @@ int launch_symdia(const sym_args &a0, const diag_pattern &pat0, const void *x, v
     a.dparts = dparts;
     *dgrid = a.per_xcd * 8;
   }
+  constexpr unsigned NT = 256 / sym_rows_per_thread<T>();
   diag_pattern c;
-  cover_pattern(pat0, 16 / sizeof(T), 256, 9, c);
+  cover_pattern(pat0, 16 / sizeof(T), NT, 9, c);
   const size_t lds = (size_t)c.lds_len * sizeof(T);
   if (lds > 65536) return fail(ISPM_ERR_UNSUPPORTED, "spmv: symmetric windows need %zu bytes of LDS", lds);
   /* pattern positions of +d[k] and -d[k] (the plan lists -d[K-1] .. -d[1], then d[0] ..
@@ -627,7 +706,7 @@ int launch_symdia(const sym_args &a0, const diag_pattern &pat0, const void *x, v
     a.up_off[k] = c.poff[(a.K - 1) + k];
     a.lo_off[k] = k ? c.poff[(a.K - 1) - k] : 0u;
   }
-  const dim3 g(a.per_xcd * 8), b(256);
+  const dim3 g(a.per_xcd * 8), b(NT);
   hipStream_t st = stream();
   if (a.K <= 4) hipLaunchKernelGGL((symdia_kernel<T, E, 4>), g, b, lds, st, a, c);
   else if (a.K <= 10) hipLaunchKernelGGL((symdia_kernel<T, E, 10>), g, b, lds, st, a, c);

@@ -332,11 +332,20 @@ def _sym_matrix(name):
         return G.stencil27_contrast(24)
     if name == "stencil19_aniso":
         return G.stencil(20, 18, 16, 19, anisotropy=10, contrast=100, seed=3)
+    if name == "levels16":
+        # widest offset 4225 spans 16 slices: level order, two positions per XCD, ten
+        # slices past the last full level
+        return G.stencil(64, 65, 40, 27, seed=6)
+    if name == "levels131":
+        # widest offset 33668 spans 131 slices: level order with bands of 16 and 17
+        # positions, one slice past the last full level
+        return G.stencil(131, 256, 3, 27, seed=7)
     return G.stencil(7, 9, 11, 27, seed=4)      # 693 rows: one padded slice, clipped windows
 
 
 @pytest.mark.parametrize("vt,et", TYPES)
-@pytest.mark.parametrize("name", ["poisson7", "stencil27_contrast", "stencil19_aniso", "small27"])
+@pytest.mark.parametrize("name", ["poisson7", "stencil27_contrast", "stencil19_aniso", "small27",
+                                  "levels16", "levels131"])
 def test_symmetric_plan(gpu, name, vt, et):
     """Symmetric diagonal plans (upper planes stored, lower entries read from their mirror
     rows) give the CSR product; padded rows stay zero."""
