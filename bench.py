@@ -56,7 +56,8 @@ def spmv_leg(I, G, A, vt, et, steps, warmup, flags=0):
     stored = inf.stored_bytes
     return {
         "format": dict(S=inf.S, hblock=inf.hblock, V=inf.V, D=inf.D,
-                       N=inf.launch.N, H=inf.launch.H),
+                       N=inf.launch.N, H=inf.launch.H,
+                       symmetric=bool(flags & I.SPMV_PLAN_SYMMETRIC)),
         "avg_s": mean, "median_s": med,
         "stored_bytes": stored, "csr_bytes": inf.csr_bytes,
         "GBps": stored / mean * 1e-9, "csr_equiv_GBps": inf.csr_bytes / mean * 1e-9,
@@ -77,7 +78,8 @@ def roofline(leg, key, kernel):
         with open(path) as f:
             rec = json.load(f).get(key)
         same = rec and rec.get("stored_bytes") == leg["stored_bytes"] and all(
-            rec["format"].get(k) == leg["format"][k] for k in ("S", "hblock", "V", "D"))
+            rec["format"].get(k) == leg["format"][k] for k in ("S", "hblock", "V", "D")) and \
+            bool(rec["format"].get("symmetric", False)) == leg["format"]["symmetric"]
         if same:
             traffic = rec["traffic_bytes"]
     return {"bound": "hbm", "kernel": kernel, "achieved": leg["GBps"], "peak": HBM_PEAK_GBS,
@@ -123,6 +125,8 @@ def solve_leg(I, S, O, A, b, factors, precision, method, steps, warmup, tol):
     pre = solver.M_plans or ()
     return {"precision": precision,
             "method": {0: "pipelined-pcg", 1: "pcg", 2: "single-reduction-pcg"}[method],
+            "A_storage": ("symmetric diagonal planes" if solver.A_plan.flags & I.SPMV_PLAN_SYMMETRIC
+                          else "SlELL"),
             "preconditioner": ("product Ainv2*Ainv1" if len(pre) == 1 else "factors Ainv1, Ainv2")
                               + (", split-diagonal plans" if pre and pre[0].flags & I.SPMV_PLAN_SPLIT_DIAG
                                  else ", SlELL plans"),
@@ -360,11 +364,19 @@ def main():
         "fsai_nnz_per_row": factors[0].nnz / A.n_rows,
         "solves": solves,
         "spmv": legs,
-        "roofline": roofline(legs["fp32"], "fp32",
-                             "slell_diagpat_kernel<float,float> (FP32 27-point 192^3 SpMV, the "
-                             "largest share of the mixed solve)"),
+        # the dominant kernel of the timed (mixed) solve: its A product
+        "roofline": (roofline(legs["fp32_sym"], "fp32_sym",
+                              "symdia_kernel<float,float> (FP32 symmetric-storage 27-point 192^3 "
+                              "SpMV, the largest share of the mixed solve)")
+                     if mixed["A_storage"] != "SlELL" else
+                     roofline(legs["fp32"], "fp32",
+                              "slell_diagpat_kernel<float,float> (FP32 27-point 192^3 SpMV, the "
+                              "largest share of the mixed solve)")),
         "roofline_fp64_spmv": roofline(legs["fp64"], "fp64",
                                        "slell_diagpat_kernel<double,double> (FP64 27-point 192^3 SpMV)"),
+        "roofline_fp64_sym_spmv": roofline(legs["fp64_sym"], "fp64_sym",
+                                           "symdia_kernel<double,double> (FP64 symmetric-storage "
+                                           "27-point 192^3 SpMV, the A product of the FP64 solves)"),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -504,8 +504,8 @@ int launch_diagpat(const ispm_slell_desc *A, const diag_pattern &pat, const unsi
  * the mirrors: A[row, row - d_k] = A[row - d_k, row] = plane k at row - d_k, which an earlier
  * slice stored and read moments ago, so it comes from the cache while only the K upper planes
  * stream from HBM (half a 27-point stencil's bytes).  One workgroup per slice, two
- * consecutive rows per thread for FP32 vectors and one for FP64; x is read from LDS windows
- * as in the pattern kernel (the pattern lists all 2K-1 column offsets). */
+ * consecutive rows per thread for FP32 vectors and elements, else one; x is read from LDS
+ * windows as in the pattern kernel (the pattern lists all 2K-1 column offsets). */
 constexpr unsigned kSymPlanes = 16;
 
 struct sym_args {
@@ -525,20 +525,25 @@ struct sym_args {
 };
 
 /* Element reads of the symmetric kernel go through the caches (the mirror reads find them
- * there).  H consecutive elements of one plane: sym_rows from an H-aligned index, sym_rows_any
- * from any index (a 4-byte aligned FP32 pair is one 8-byte load). */
-template<typename T, typename E> __device__ __forceinline__ T ld_cached(const E *p, unsigned i) { return (T)p[i]; }
-template<> __device__ __forceinline__ float ld_cached<float, dev::bf16_t>(const dev::bf16_t *p, unsigned i) { return dev::bf16_to_f32(p[i]); }
+ * there) and stay raw, one element per register, until the products: a conversion, or two
+ * BF16 halves packed into one register, right after a load would make the wave wait for it
+ * before issuing the next.  H consecutive elements of one plane: sym_rows::get from an
+ * H-aligned index, get_any from any index (a 4-byte aligned FP32 pair is one 8-byte load). */
+template<typename E> struct sym_raw { typedef E type; };
+template<> struct sym_raw<dev::bf16_t> { typedef unsigned type; };
+template<typename T, typename E> __device__ __forceinline__ T sym_cvt(typename sym_raw<E>::type v) { return (T)v; }
+template<> __device__ __forceinline__ float sym_cvt<float, dev::bf16_t>(unsigned v) { return __uint_as_float(v << 16); }
 
-template<typename T, typename E, unsigned H> struct sym_rows {
-  static __device__ __forceinline__ void get(const E *p, unsigned i, T (&v)[H])
+template<typename E, unsigned H> struct sym_rows {
+  typedef typename sym_raw<E>::type R;
+  static __device__ __forceinline__ void get(const E *p, unsigned i, R (&v)[H])
   {
 #pragma unroll
-    for (unsigned e = 0; e < H; e++) v[e] = ld_cached<T, E>(p, i + e);
+    for (unsigned e = 0; e < H; e++) v[e] = (R)p[i + e];
   }
-  static __device__ __forceinline__ void get_any(const E *p, unsigned i, T (&v)[H]) { get(p, i, v); }
+  static __device__ __forceinline__ void get_any(const E *p, unsigned i, R (&v)[H]) { get(p, i, v); }
 };
-template<> struct sym_rows<float, float, 2> {
+template<> struct sym_rows<float, 2> {
   typedef float f2a __attribute__((ext_vector_type(2)));
   typedef float f2u __attribute__((ext_vector_type(2))) __attribute__((aligned(4)));
   static __device__ __forceinline__ void get(const float *p, unsigned i, float (&v)[2])
@@ -552,30 +557,17 @@ template<> struct sym_rows<float, float, 2> {
     v[0] = q.x; v[1] = q.y;
   }
 };
-template<> struct sym_rows<float, dev::bf16_t, 2> {
-  static __device__ __forceinline__ void get(const dev::bf16_t *p, unsigned i, float (&v)[2])
-  {
-    const unsigned q = *reinterpret_cast<const unsigned *>(p + i);
-    v[0] = __uint_as_float(q << 16);
-    v[1] = __uint_as_float(q & 0xffff0000u);
-  }
-  static __device__ __forceinline__ void get_any(const dev::bf16_t *p, unsigned i, float (&v)[2])
-  {
-    v[0] = dev::bf16_to_f32(p[i]);
-    v[1] = dev::bf16_to_f32(p[i + 1]);
-  }
-};
 
-/* rows per thread: two for FP32 vectors (8-byte element pairs halve the load requests, which
- * bound the FP32 kernel), one for FP64 */
-template<typename T> constexpr unsigned sym_rows_per_thread() { return sizeof(T) == 4 ? 2u : 1u; }
+/* rows per thread: two for FP32 vectors and elements (8-byte element pairs halve the load
+ * requests, which bound that kernel), else one */
+template<typename T, typename E> constexpr unsigned sym_rows_per_thread() { return sizeof(T) == 4 && sizeof(E) == 4 ? 2u : 1u; }
 
 template<typename T, typename E, unsigned NK>
-__global__ void __launch_bounds__(256 / sym_rows_per_thread<T>()) symdia_kernel(sym_args a, diag_pattern pat)
+__global__ void __launch_bounds__((256 / sym_rows_per_thread<T, E>())) symdia_kernel(sym_args a, diag_pattern pat)
 {
   extern __shared__ __align__(16) unsigned char sym_lds[];
   T *win = reinterpret_cast<T *>(sym_lds);
-  constexpr unsigned S = 256, H = sym_rows_per_thread<T>(), NT = S / H, VX = 16 / sizeof(T), NR = 9;
+  constexpr unsigned S = 256, H = sym_rows_per_thread<T, E>(), NT = S / H, VX = 16 / sizeof(T), NR = 9;
   const unsigned t = threadIdx.x;
   const unsigned xcd = blockIdx.x & 7u, i = blockIdx.x >> 3;
   unsigned slice = xcd * a.per_xcd + i;
@@ -609,21 +601,21 @@ __global__ void __launch_bounds__(256 / sym_rows_per_thread<T>()) symdia_kernel(
    * any wait; element indices fit 32 bits (the plan checks it).  A mirror pair that crosses a
    * slice boundary reads one element past the slice's plane (still inside the array: the
    * mirror rows lie below the last row) and takes its second value from the next slice. */
-  T ev[NK][H], mv[NK][H];
+  typedef typename sym_raw<E>::type R;
+  R ev[NK][H], mv[NK][H];
   const unsigned KS = K * S, own = slice * KS + H * t;
 #pragma unroll
   for (unsigned k = 0; k < NK; k++) {
 #pragma unroll
-    for (unsigned e = 0; e < H; e++) ev[k][e] = mv[k][e] = 0;
-    if (k < K) sym_rows<T, E, H>::get(el, own + k * S, ev[k]);
+    for (unsigned e = 0; e < H; e++) ev[k][e] = mv[k][e] = R(0);
+    if (k < K) sym_rows<E, H>::get(el, own + k * S, ev[k]);
     if (k >= 1 && k < K) {
       const int j = (int)row - a.d[k];
       if (j >= 0) {
-        sym_rows<T, E, H>::get_any(el, ((unsigned)j >> 8) * KS + k * S + ((unsigned)j & 255u), mv[k]);
-        if (H == 2 && ((unsigned)j & 255u) == 255u)
-          mv[k][H - 1] = ld_cached<T, E>(el, (((unsigned)j + 1) >> 8) * KS + k * S);
+        sym_rows<E, H>::get_any(el, ((unsigned)j >> 8) * KS + k * S + ((unsigned)j & 255u), mv[k]);
+        if (H == 2 && ((unsigned)j & 255u) == 255u) mv[k][H - 1] = (R)el[(((unsigned)j + 1) >> 8) * KS + k * S];
       } else if (H == 2 && j == -1) {
-        mv[k][H - 1] = ld_cached<T, E>(el, k * S);
+        mv[k][H - 1] = (R)el[k * S];
       }
     }
   }
@@ -650,8 +642,8 @@ __global__ void __launch_bounds__(256 / sym_rows_per_thread<T>()) symdia_kernel(
     if (k < K) {
 #pragma unroll
       for (unsigned e = 0; e < H; e++) {
-        acc[e] += ev[k][e] * win[a.up_off[k] + H * t + e];
-        if (k >= 1) acc[e] += mv[k][e] * win[a.lo_off[k] + H * t + e];
+        acc[e] += sym_cvt<T, E>(ev[k][e]) * win[a.up_off[k] + H * t + e];
+        if (k >= 1) acc[e] += sym_cvt<T, E>(mv[k][e]) * win[a.lo_off[k] + H * t + e];
       }
     }
 #pragma unroll
@@ -695,7 +687,7 @@ int launch_symdia(const sym_args &a0, const diag_pattern &pat0, const void *x, v
     a.dparts = dparts;
     *dgrid = a.per_xcd * 8;
   }
-  constexpr unsigned NT = 256 / sym_rows_per_thread<T>();
+  constexpr unsigned NT = 256 / sym_rows_per_thread<T, E>();
   diag_pattern c;
   cover_pattern(pat0, 16 / sizeof(T), NT, 9, c);
   const size_t lds = (size_t)c.lds_len * sizeof(T);
