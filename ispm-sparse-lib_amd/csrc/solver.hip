@@ -43,7 +43,7 @@ struct scalars {
   double rr, l1, tol2, r64;
   double rscale;
   unsigned done, iters;
-  unsigned rr_valid;   /* single-reduction PCG: the (r,r) partials hold a residual to test */
+  unsigned rr_valid;   /* PCG1, and PCG with a preconditioner: the (r,r) partials hold a residual to test */
 };
 
 enum { OP_PQ, OP_RZ, OP_GAMMA, OP_DELTA, OP_R64, OP_XR, OP_PPCG };
@@ -190,6 +190,44 @@ k_fin_cg(const double *pg, unsigned Gg, const double *pd, unsigned Gd, const dou
   s->beta = bd;
   s->rr_valid = 1;
   s->iters++;
+}
+
+/* Hestenes-Stiefel PCG with a preconditioner: one finishing kernel for the preconditioner
+ * product's (r, z) partials and the (r, r) partials k_xr left one iteration earlier.  When
+ * that residual meets the target the solve is done (the iteration that produced it is
+ * counted, as a finish right after k_xr would have) and nothing more of the batch runs;
+ * otherwise beta = (r, z) / rho and rho = (r, z). */
+__global__ void __launch_bounds__(kFinThreads)
+k_fin_rz_rr(const double *pg, unsigned Gg, const double *pr, unsigned Gr, scalars *s)
+{
+  __shared__ double lds[2 * kFinThreads / 64];
+  if (s->done) return;
+  double out[2] = {0.0, 0.0};
+  const unsigned t = threadIdx.x;
+  /* eight loads of each sum in flight per step; a fixed order keeps the sums reproducible */
+  constexpr unsigned C = 8 * kFinThreads;
+  const unsigned Gm = Gg > Gr ? Gg : Gr;
+  for (unsigned i0 = 0; i0 < Gm; i0 += C) {
+    double a[8], c[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const unsigned i = i0 + t + k * kFinThreads;
+      a[k] = i < Gg ? pg[i] : 0.0;
+      c[k] = i < Gr ? pr[i] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) { out[0] += a[k]; out[1] += c[k]; }
+  }
+  dev::block_sum<kFinThreads, 2>(out, lds);
+  if (t) return;
+  if (s->rr_valid) {
+    s->rr = out[1];
+    s->iters++;
+    if (out[1] <= s->tol2) { s->done = 1; return; }
+  }
+  s->beta = out[0] / s->rho;
+  s->rho = out[0];
+  s->rr_valid = 1;
 }
 
 /* p = z + beta p; s = w + beta s; x += alpha p; r -= alpha s; (r, r) partials */
@@ -517,6 +555,9 @@ int iterate(ispm_solver *S)
     return ISPM_OK;
   }
   unsigned G = 0;
+  /* with a preconditioner, the (r, r) test of the previous iteration's update shares the
+   * finishing kernel of (r, z) */
+  double *pr = S->parts + 2 * kMaxDotParts;
   if (S->M1) {
     /* z = M2 (M1 r), or M1 r for a one-matrix preconditioner, with (r, z) fused into the
      * last product where possible */
@@ -529,7 +570,8 @@ int iterate(ispm_solver *S)
     st = S->T == ISPM_F64 ? spmv_dot<double>(S, last, in, S->z, S->r, sk, G)
                           : spmv_dot<float>(S, last, in, S->z, S->r, sk, G);
     if (st) return st;
-    finish<OP_RZ>(S, 1, G);
+    hipLaunchKernelGGL(k_fin_rz_rr, dim3(1), dim3(kFinThreads), 0, sm, (const double *)S->parts, G,
+                       (const double *)pr, (unsigned)S->grid, S->s);
   } else {
     if ((st = apply_M(S, S->r, S->z, sk))) return st;
     if ((st = dot(S, OP_RZ, S->r, S->z, 1, S->T))) return st;
@@ -544,11 +586,11 @@ int iterate(ispm_solver *S)
   finish<OP_PQ>(S, 1, G);
   if (S->T == ISPM_F64)
     hipLaunchKernelGGL(k_xr<double>, g, b, 0, sm, (double *)S->xs, (double *)S->r, (const double *)S->p,
-                       (const double *)S->q, S->n, S->s, S->parts);
+                       (const double *)S->q, S->n, S->s, S->M1 ? pr : S->parts);
   else
     hipLaunchKernelGGL(k_xr<float>, g, b, 0, sm, (float *)S->xs, (float *)S->r, (const float *)S->p,
-                       (const float *)S->q, S->n, S->s, S->parts);
-  finish<OP_XR>(S, 1);
+                       (const float *)S->q, S->n, S->s, S->M1 ? pr : S->parts);
+  if (!S->M1) finish<OP_XR>(S, 1);
   ISPM_HIP(hipGetLastError());
   return ISPM_OK;
 }

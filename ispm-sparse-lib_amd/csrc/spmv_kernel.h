@@ -1267,8 +1267,11 @@ int launch_split(const ispm_slell_desc *R, const void *diag, const unsigned *rma
     a.n_slices = 0;
     a.per_xcd = 0;
   }
-  /* one vector of rows per thread and step, at most 2048 diagonal workgroups (grid-stride) */
-  const unsigned G1 = std::max(1u, std::min(2048u, (n / V + 255) / 256));
+  /* one vector of rows per thread and step, at most 8192 diagonal workgroups (grid-stride;
+   * 2048 measured 1-2 us slower on the 192^3 FSAI product).  The remainder workgroups come
+   * first: they are the longer chain (gathers), and measured slower after the diagonal ones
+   * or interleaved with them. */
+  const unsigned G1 = std::max(1u, std::min(8192u, (n / V + 255) / 256));
   split_args sa{diag, rmask, n, G2, nullptr};
   if (dotv && dparts && dgrid && G1 + G2 <= kMaxDotParts) {
     a.dotv = dotv;
