@@ -321,7 +321,57 @@ int ispm_solver_create(ispm_solver **out, ispm_plan *A, ispm_plan *Ainv1, ispm_p
 int ispm_solver_set_residual_plan(ispm_solver *s, ispm_plan *A64);
 /* b, x: device FP64 vectors of the padded length; x is the initial guess and the result. */
 int ispm_solver_solve(ispm_solver *s, const double *b, double *x, ispm_solve_stats *stats);
+
+/* The solve with the convergence test of the reference's caller, a foam-extend (OpenFOAM)
+ * linear solver (no counterpart in the reference library, README.md:3-9): OpenFOAM's
+ * normalised residual sum |b - A x| / normFactor, normFactor = sum |A x0 - sumA xbar| +
+ * |b - sumA xbar| + 1e-20 with sumA the row sums of A and xbar the mean of the initial x
+ * (lduMatrix::solver::normFactor), over the matrix's rows.  Converged when the residual is
+ * below tolerance, or below rel_tol times the initial residual when rel_tol > 0; an initial
+ * residual already below tolerance returns at once with 0 iterations (x unchanged).  The
+ * iteration limit is opts.maxit.  Returns ISPM_ERR_NOT_CONVERGED like ispm_solver_solve. */
+typedef struct ispm_norm_stats {
+  int iterations, replacements, converged;
+  double initial_residual;   /* normalised residual of the initial x */
+  double final_residual;     /* normalised residual at exit */
+  double norm_factor;
+  double l1_resnorm;         /* sum |r_i| of the recursive residual */
+} ispm_norm_stats;
+int ispm_solver_solve_normalised(ispm_solver *s, const double *b, double *x, double tolerance,
+                                 double rel_tol, ispm_norm_stats *stats);
 void ispm_solver_destroy(ispm_solver *s);
+
+/* ------------------------------------------------------------- lduMatrix input -------- */
+/* No counterpart in the reference library: its caller converts OpenFOAM's lduMatrix to the
+ * csr_matrix it passes in.  n cells, n_faces internal faces; face f joins cells
+ * lower_addr[f] < upper_addr[f] with coefficients upper[f] = A[lower][upper] and lower[f] =
+ * A[upper][lower] (lower = NULL: symmetric, lower = upper); diag[n].  Output: FP64 CSR with
+ * columns ascending in every row (library-owned, ispm_csr_out_free).  pos (optional, n +
+ * 2 n_faces ints): the CSR index of diag[i] at pos[i], of upper[f] at pos[n + f], of
+ * lower[f] at pos[n + n_faces + f].  Fails with ISPM_ERR_INVALID for addresses out of range
+ * or not lower < upper, and for two faces joining the same cells. */
+int ispm_ldu_to_csr(int n, int n_faces, const int *lower_addr, const int *upper_addr,
+                    const double *diag, const double *upper, const double *lower,
+                    ispm_csr_out *out, int *pos);
+/* New coefficients of the same addressing in CSR value order (vals[pos[s]] = source s),
+ * on the host, or on the device (pos, diag, upper, lower, vals device pointers; positions
+ * outside the CSR are skipped), e.g. before ispm_plan_update_coefs_device. */
+int ispm_ldu_values(int n, int n_faces, const int *pos, const double *diag, const double *upper,
+                    const double *lower, double *vals);
+int ispm_ldu_values_device(int n, int n_faces, const int *pos, const double *diag,
+                           const double *upper, const double *lower, double *vals);
+
+/* DroptolTuner (fastainv/droptol-tuning.h:4-103) for callers without the C++ header: a
+ * golden-section search over log2(droptol) in [log2_lo, log2_hi] driven by the times the
+ * caller reports for each probed drop tolerance. */
+typedef struct ispm_droptol_tuner ispm_droptol_tuner;
+int ispm_droptol_tuner_create(double log2_lo, double log2_hi, ispm_droptol_tuner **out);
+double ispm_droptol_tuner_droptol(const ispm_droptol_tuner *t);   /* getDroptol */
+double ispm_droptol_tuner_best(const ispm_droptol_tuner *t);      /* getBestDroptol */
+void ispm_droptol_tuner_note_time(ispm_droptol_tuner *t, double seconds);   /* noteTime */
+void ispm_droptol_tuner_restart(ispm_droptol_tuner *t);           /* restart */
+int ispm_droptol_tuner_iterations(const ispm_droptol_tuner *t);
+void ispm_droptol_tuner_destroy(ispm_droptol_tuner *t);
 
 /* ------------------------------------------------------------- on-disk formats -------- */
 /* No reference counterpart (the reference has no persistence).  MatrixMarket coordinate

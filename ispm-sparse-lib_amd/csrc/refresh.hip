@@ -52,10 +52,41 @@ int scatter(ispm_plan *p, const V *vals, long long nnz)
   return ISPM_OK;
 }
 
+/* lduMatrix coefficients into CSR value order: source s (diag, then upper, then lower)
+ * goes to vals[pos[s]]; positions outside [0, nnz) are skipped */
+__global__ void __launch_bounds__(256) ldu_scatter_kernel(double *__restrict__ vals, const int *__restrict__ pos,
+                                                          const double *__restrict__ diag, const double *__restrict__ upper,
+                                                          const double *__restrict__ lower, int n, int nf)
+{
+  const long long nnz = (long long)n + 2LL * nf, stride = (long long)gridDim.x * 256;
+  for (long long s = (long long)blockIdx.x * 256 + threadIdx.x; s < nnz; s += stride) {
+    const double v = s < n ? diag[s] : s < (long long)n + nf ? upper[s - n] : lower[s - n - nf];
+    const long long k = pos[s];
+    if (k >= 0 && k < nnz) vals[k] = v;
+  }
+}
+
 }  // namespace
 }  // namespace ispm
 
 using namespace ispm;
+
+extern "C" int ispm_ldu_values_device(int n, int n_faces, const int *pos, const double *diag, const double *upper,
+                                      const double *lower, double *vals)
+{
+  if (n < 0 || n_faces < 0) return fail(ISPM_ERR_INVALID, "ldu_values_device: negative size");
+  if (!vals || !pos || (n && !diag) || (n_faces && !upper)) return fail(ISPM_ERR_INVALID, "ldu_values_device: null argument");
+  int st = ensure_init();
+  if (st) return st;
+  const long long nnz = (long long)n + 2LL * n_faces;
+  if (nnz == 0) return ISPM_OK;
+  long long blocks = (nnz + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(ldu_scatter_kernel, dim3((unsigned)blocks), dim3(256), 0, stream(), vals, pos, diag, upper,
+                     lower ? lower : upper, n, n_faces);
+  ISPM_HIP(hipGetLastError());
+  return ISPM_OK;
+}
 
 extern "C" int ispm_plan_update_coefs_device(ispm_plan *p, const void *vals, int val_type)
 {

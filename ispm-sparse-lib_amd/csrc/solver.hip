@@ -44,9 +44,10 @@ struct scalars {
   double rscale;
   unsigned done, iters;
   unsigned rr_valid;   /* PCG1, and PCG with a preconditioner: the (r,r) partials hold a residual to test */
+  double xsum, normf, l1true;   /* normalised-residual solves: sum x, norm factor, sum |b - A x| */
 };
 
-enum { OP_PQ, OP_RZ, OP_GAMMA, OP_DELTA, OP_R64, OP_XR, OP_PPCG };
+enum { OP_PQ, OP_RZ, OP_GAMMA, OP_DELTA, OP_R64, OP_XR, OP_PPCG, OP_SUM, OP_NORMF, OP_L1 };
 
 template<typename T> struct vtraits {
   static constexpr int L = 16 / sizeof(T);
@@ -77,7 +78,7 @@ k_dot(const T *__restrict__ x, const T *__restrict__ y, int n, const scalars *s,
 template<int OP, unsigned GMAX>
 __global__ void __launch_bounds__(kFinThreads) k_fin(const double *parts, int G, scalars *s, int skip_if_done)
 {
-  constexpr int NS = OP == OP_PPCG ? 4 : 1;
+  constexpr int NS = OP == OP_PPCG ? 4 : OP == OP_NORMF ? 2 : 1;
   __shared__ double lds[NS * kFinThreads / 64];
   if (skip_if_done && s->done) return;
   double out[NS];
@@ -89,6 +90,9 @@ __global__ void __launch_bounds__(kFinThreads) k_fin(const double *parts, int G,
   if (OP == OP_GAMMA) s->gamma = r;
   if (OP == OP_DELTA) s->delta = r;
   if (OP == OP_R64) s->r64 = r;
+  if (OP == OP_SUM) s->xsum = r;
+  if (OP == OP_NORMF) { s->normf = out[0]; s->l1true = out[1]; }
+  if (OP == OP_L1) s->l1true = r;
   if (OP == OP_XR) {
     s->rr = r;
     s->iters++;
@@ -343,6 +347,44 @@ k_ppcg(const T *__restrict__ nn, const T *__restrict__ m, T *__restrict__ p, T *
   store_partials<4>(acc, parts, lds);
 }
 
+/* ---- OpenFOAM's normalised residual (lduMatrix::solver::normFactor): with
+ * xbar = average(x) and sumA = A 1 (row sums),
+ *   normFactor = sum_i |(A x)_i - sumA_i xbar| + |b_i - sumA_i xbar| + 1e-20,
+ *   residual   = sum_i |b_i - (A x)_i| / normFactor,
+ * over the nr rows of the matrix (not the padding). ---- */
+__global__ void __launch_bounds__(256) k_ones(double *__restrict__ v, int nr, int n)
+{
+  const int gs = gridDim.x * 256;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gs) v[i] = i < nr ? 1.0 : 0.0;
+}
+
+/* sum x_i (or sum |x_i| when ABS) over i < nr */
+template<bool ABS>
+__global__ void __launch_bounds__(256) k_sum1(const double *__restrict__ x, int nr, double *parts)
+{
+  __shared__ double lds[4];
+  double v[1] = {0.0};
+  const int gs = gridDim.x * 256;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < nr; i += gs) v[0] += ABS ? std::fabs(x[i]) : x[i];
+  store_partials<1>(v, parts, lds);
+}
+
+__global__ void __launch_bounds__(256)
+k_normf(const double *__restrict__ b, const double *__restrict__ Ax, const double *__restrict__ sumA, int nr,
+        const scalars *s, double *parts)
+{
+  __shared__ double lds[8];
+  const double xbar = s->xsum / (double)nr;
+  double v[2] = {0.0, 0.0};
+  const int gs = gridDim.x * 256;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < nr; i += gs) {
+    const double pa = sumA[i] * xbar;
+    v[0] += std::fabs(Ax[i] - pa) + std::fabs(b[i] - pa);
+    v[1] += std::fabs(b[i] - Ax[i]);
+  }
+  store_partials<2>(v, parts, lds);
+}
+
 /* r = b - Ax (FP64), (r,r) -> s->r64 */
 __global__ void __launch_bounds__(256)
 k_resid(const double *__restrict__ b, const double *__restrict__ Ax, double *__restrict__ r, int n,
@@ -397,6 +439,7 @@ struct ispm_solver {
   void *r = nullptr, *u = nullptr, *w = nullptr, *m = nullptr, *nv = nullptr, *z = nullptr,
        *q = nullptr, *sv = nullptr, *p = nullptr, *t = nullptr, *xs = nullptr;
   double *r64 = nullptr, *tmp64 = nullptr;
+  double *sumA = nullptr;  /* row sums of A (normalised-residual solves; allocated on first use) */
   scalars *s = nullptr;
   scalars *hs = nullptr;   /* pinned host mirror */
   double *parts = nullptr; /* reduction partials, private to this solver */
@@ -718,7 +761,33 @@ int ispm_solver_create(ispm_solver **out, ispm_plan *A, ispm_plan *M1, ispm_plan
   return ISPM_OK;
 }
 
-int ispm_solver_solve(ispm_solver *S, const double *b, double *x, ispm_solve_stats *stats)
+}  // extern "C"
+
+namespace {
+
+/* Convergence measure of a solve.  L2 (ispm_solver_solve): ||b - A x||_2 / ||b||_2 <= tol.
+ * Normalised (ispm_solver_solve_normalised): OpenFOAM's sum |b - A x| / normFactor, with the
+ * norm factor of the initial x, below max(tol, rel_tol * initial) (the checkConvergence rule:
+ * residual < tolerance, or residual < relTol * initial residual when relTol > 0). */
+struct measure {
+  bool normalised = false;
+  double tol = 1e-10, rel_tol = 0.0;
+  /* outputs */
+  double initial = 0.0, norm_factor = 0.0;
+};
+
+/* the normalised measure of the residual true_residual left in r64 (sum |r| over the rows) */
+int l1_residual(ispm_solver *S, int nr, double &l1)
+{
+  hipLaunchKernelGGL((k_sum1<true>), dim3(S->grid), dim3(256), 0, stream(), (const double *)S->r64, nr, S->parts);
+  finish<OP_L1>(S, 0);
+  ISPM_HIP(hipGetLastError());
+  int st = pull(S);
+  l1 = S->hs->l1true;
+  return st;
+}
+
+int solve_core(ispm_solver *S, const double *b, double *x, measure &mc, ispm_solve_stats *stats)
 {
   if (!S || !b || !x) return fail(ISPM_ERR_INVALID, "solver_solve: null argument");
   if ((reinterpret_cast<uintptr_t>(b) | reinterpret_cast<uintptr_t>(x)) & 15)
@@ -735,11 +804,41 @@ int ispm_solver_solve(ispm_solver *S, const double *b, double *x, ispm_solve_sta
   if ((st = launch_dot<double>(S, OP_R64, b, b, 0))) return st;
   if ((st = pull(S))) return st;
   const double bnorm = std::sqrt(S->hs->r64) > 0 ? std::sqrt(S->hs->r64) : 1.0;
-  const double tol = S->o.tol > 0 ? S->o.tol : 1e-10;
+  ispm_plan *Ares = S->Ares ? S->Ares : S->A;
+  /* rows of the matrix (the vectors are padded beyond them) */
+  const int nr = std::min(S->n, Ares->dev.n_cols > 0 ? Ares->dev.n_cols : S->n);
+  if (mc.normalised) {
+    /* sumA = A 1 (into tmp64 via r64 as the ones vector), then the sum of x */
+    if (!S->sumA && (st = alloc((void **)&S->sumA, 8 * (size_t)n + 64))) return st;
+    hipLaunchKernelGGL(k_ones, g, bl, 0, sm, S->r64, nr, n);
+    ISPM_HIP(hipGetLastError());
+    if ((st = spmv(Ares, ISPM_F64, S->r64, S->sumA, nullptr))) return st;
+    hipLaunchKernelGGL((k_sum1<false>), g, bl, 0, sm, (const double *)x, nr, S->parts);
+    finish<OP_SUM>(S, 0);
+    ISPM_HIP(hipGetLastError());
+  }
   /* r = b - A x */
   if ((st = true_residual(S, b, x))) return st;
   if ((st = pull(S))) return st;
-  double rel = std::sqrt(S->hs->r64) / bnorm;
+  double rel, target, abs2;   /* measure, its target, and the L2 norm of a residual that meets it */
+  if (mc.normalised) {
+    hipLaunchKernelGGL(k_normf, g, bl, 0, sm, b, (const double *)S->tmp64, (const double *)S->sumA, nr,
+                       (const scalars *)S->s, S->parts);
+    finish<OP_NORMF>(S, 0);
+    ISPM_HIP(hipGetLastError());
+    if ((st = pull(S))) return st;
+    mc.norm_factor = S->hs->normf + 1e-20;
+    rel = mc.initial = S->hs->l1true / mc.norm_factor;
+    target = std::max(mc.tol, mc.rel_tol > 0 ? mc.rel_tol * mc.initial : 0.0);
+    /* the recursion tests ||r||_2: scale the target by the initial residual's L2/L1 ratio */
+    const double l2 = std::sqrt(S->hs->r64), l1 = S->hs->l1true;
+    abs2 = target * mc.norm_factor * (l1 > 0 ? l2 / l1 : 1.0 / std::sqrt((double)nr));
+  } else {
+    rel = std::sqrt(S->hs->r64) / bnorm;
+    target = mc.tol;
+    abs2 = mc.tol * bnorm;
+  }
+  auto met = [&](double v) { return mc.normalised ? v < target : v <= target; };
   const double margin = 0.9;
   for (void *v : {S->z, S->q, S->p, S->sv}) if (v && (st = zero(v, vb))) return st;
 
@@ -788,7 +887,7 @@ int ispm_solver_solve(ispm_solver *S, const double *b, double *x, ispm_solve_sta
     return push(S);
   };
 
-  if (rel <= tol) {
+  if (met(rel)) {
     res.converged = 1;
     res.relres = rel;
     if (stats) *stats = res;
@@ -806,10 +905,10 @@ int ispm_solver_solve(ispm_solver *S, const double *b, double *x, ispm_solve_sta
   double rec_target;   /* target of the recursive residual, relative to its starting norm */
   if (S->mixed) {
     double rmax = std::sqrt(S->hs->r64);
-    rec_target = std::max(S->o.reliable_delta, margin * tol * bnorm / rmax);
+    rec_target = std::max(S->o.reliable_delta, margin * abs2 / rmax);
     st = setup_recurrence(rec_target * rec_target);
   } else {
-    double t = margin * tol * bnorm;
+    double t = margin * abs2;
     rec_target = t;
     st = setup_recurrence(t * t);
   }
@@ -828,9 +927,15 @@ int ispm_solver_solve(ispm_solver *S, const double *b, double *x, ispm_solve_sta
       ISPM_HIP(hipGetLastError());
     }
     if ((st = true_residual(S, b, x)) || (st = pull(S))) break;
-    rel = std::sqrt(S->hs->r64) / bnorm;
+    if (mc.normalised) {
+      double l1 = 0.0;
+      if ((st = l1_residual(S, nr, l1))) break;
+      rel = l1 / mc.norm_factor;
+    } else {
+      rel = std::sqrt(S->hs->r64) / bnorm;
+    }
     res.relres = rel;
-    if (rel <= tol) { res.converged = 1; break; }
+    if (met(rel)) { res.converged = 1; break; }
     if (out_of_iters || ++guard > 10000) break;
     res.replacements++;
     if (S->mixed) {
@@ -843,7 +948,7 @@ int ispm_solver_solve(ispm_solver *S, const double *b, double *x, ispm_solve_sta
       if (S->single) hipLaunchKernelGGL(k_scale<float>, g, bl, 0, sm, (float *)S->sv, n, c);
       ISPM_HIP(hipGetLastError());
       if ((st = pull(S))) break;
-      rec_target = std::max(S->o.reliable_delta, margin * tol * bnorm / rmax);
+      rec_target = std::max(S->o.reliable_delta, margin * abs2 / rmax);
       h.rho = h.rho * c * c;
       h.gamma_old = h.gamma_old * c * c;
       h.rr_valid = 0;
@@ -863,6 +968,40 @@ int ispm_solver_solve(ispm_solver *S, const double *b, double *x, ispm_solve_sta
   return res.converged ? ISPM_OK : fail(ISPM_ERR_NOT_CONVERGED, "solver: relres %.3e after %d iterations", res.relres, res.iterations);
 }
 
+}  // namespace
+
+extern "C" {
+
+int ispm_solver_solve(ispm_solver *S, const double *b, double *x, ispm_solve_stats *stats)
+{
+  measure mc;
+  if (S) mc.tol = S->o.tol > 0 ? S->o.tol : 1e-10;
+  return solve_core(S, b, x, mc, stats);
+}
+
+int ispm_solver_solve_normalised(ispm_solver *S, const double *b, double *x, double tolerance, double rel_tol,
+                                 ispm_norm_stats *stats)
+{
+  if (!(tolerance >= 0) || !(rel_tol >= 0) || (tolerance == 0 && rel_tol == 0))
+    return fail(ISPM_ERR_INVALID, "solver_solve_normalised: need tolerance >= 0, rel_tol >= 0, not both 0");
+  measure mc;
+  mc.normalised = true;
+  mc.tol = tolerance;
+  mc.rel_tol = rel_tol;
+  ispm_solve_stats res{};
+  const int st = solve_core(S, b, x, mc, &res);
+  if (stats) {
+    stats->iterations = res.iterations;
+    stats->replacements = res.replacements;
+    stats->converged = res.converged;
+    stats->initial_residual = mc.initial;
+    stats->final_residual = res.iterations || res.converged ? res.relres : mc.initial;
+    stats->norm_factor = mc.norm_factor;
+    stats->l1_resnorm = res.l1_resnorm;
+  }
+  return st;
+}
+
 int ispm_solver_set_residual_plan(ispm_solver *S, ispm_plan *A64)
 {
   if (!S) return fail(ISPM_ERR_INVALID, "solver_set_residual_plan: null solver");
@@ -880,6 +1019,7 @@ void ispm_solver_destroy(ispm_solver *S)
   for (void *v : {S->r, S->u, S->w, S->m, S->nv, S->z, S->q, S->sv, S->p, S->t, S->xs}) ispm_free(v);
   ispm_free(S->r64);
   ispm_free(S->tmp64);
+  ispm_free(S->sumA);
   ispm_free(S->s);
   ispm_free(S->parts);
   if (S->hs) (void)hipHostFree(S->hs);

@@ -157,6 +157,20 @@ _SIGS = {
     "ispm_solver_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "ispm_solver_destroy": (None, [C.c_void_p]),
     "ispm_solver_set_residual_plan": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "ispm_solver_solve_normalised": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                               C.c_void_p]),
+    "ispm_ldu_to_csr": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.POINTER(CsrOut), C.c_void_p]),
+    "ispm_ldu_values": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ispm_ldu_values_device": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p]),
+    "ispm_droptol_tuner_create": (C.c_int, [C.c_double, C.c_double, C.POINTER(C.c_void_p)]),
+    "ispm_droptol_tuner_droptol": (C.c_double, [C.c_void_p]),
+    "ispm_droptol_tuner_best": (C.c_double, [C.c_void_p]),
+    "ispm_droptol_tuner_note_time": (None, [C.c_void_p, C.c_double]),
+    "ispm_droptol_tuner_restart": (None, [C.c_void_p]),
+    "ispm_droptol_tuner_iterations": (C.c_int, [C.c_void_p]),
+    "ispm_droptol_tuner_destroy": (None, [C.c_void_p]),
     "ispm_block_solver_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "ispm_block_solver_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p]),
     "ispm_block_solver_destroy": (None, [C.c_void_p]),

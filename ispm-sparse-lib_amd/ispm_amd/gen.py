@@ -202,6 +202,28 @@ def fv7_lognormal(m: int = 128, sigma: float = 1.5, seed: int = 2001, permute_se
     return CSR.from_scipy(M)
 
 
+def ldu_mesh(nx: int, ny: int, nz: int, sigma: float = 1.0, seed: int = 4001, asymmetry: float = 0.0):
+    """An OpenFOAM-style lduMatrix of a diffusion operator on an nx*ny*nz hexahedral mesh
+    (cell x + nx (y + ny z)): internal faces in upper-triangular order (sorted by owner, then
+    neighbour, owner < neighbour), face conductances lognormal(0, sigma), upper = -c, diagonal =
+    the cell's face conductances plus 1 per boundary face (Dirichlet, so the matrix is SPD).
+    asymmetry > 0 gives lower = -c (1 + asymmetry u), u ~ U[0,1) (no longer symmetric).
+    Returns (n, lower_addr, upper_addr, diag, upper, lower or None)."""
+    n = nx * ny * nz
+    idx = np.arange(n).reshape(nz, ny, nx)
+    own = np.concatenate([idx[:, :, :-1].ravel(), idx[:, :-1, :].ravel(), idx[:-1].ravel()])
+    nei = np.concatenate([idx[:, :, 1:].ravel(), idx[:, 1:, :].ravel(), idx[1:].ravel()])
+    order = np.lexsort((nei, own))
+    own, nei = own[order].astype(np.int32), nei[order].astype(np.int32)
+    nf = len(own)
+    c = np.exp(sigma * normal(seed, nf))
+    deg = np.bincount(own, minlength=n) + np.bincount(nei, minlength=n)
+    diag = np.bincount(own, weights=c, minlength=n) + np.bincount(nei, weights=c, minlength=n) + (6 - deg)
+    upper = -c
+    lower = None if asymmetry <= 0 else -c * (1.0 + asymmetry * uniform01(seed + 1, nf))
+    return n, own, nei, diag, upper, lower
+
+
 def rgg_laplacian(n: int, k: int, seed: int, shift: float = 1e-2, dtype=np.float64) -> CSR:
     """Graph Laplacian of a 3-D random geometric kNN graph (symmetrised) plus shift*I."""
     import scipy.sparse as sp

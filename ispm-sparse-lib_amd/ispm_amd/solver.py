@@ -24,6 +24,13 @@ class SolveStats(C.Structure):
                 ("relres", C.c_double), ("l1_resnorm", C.c_double)]
 
 
+class NormStats(C.Structure):
+    """ispm_norm_stats: the solve with OpenFOAM's normalised residual."""
+    _fields_ = [("iterations", C.c_int), ("replacements", C.c_int), ("converged", C.c_int),
+                ("initial_residual", C.c_double), ("final_residual", C.c_double),
+                ("norm_factor", C.c_double), ("l1_resnorm", C.c_double)]
+
+
 def fp32_exact(A: CSR) -> bool:
     v = np.asarray(A.vals)
     return v.dtype == np.float32 or bool(np.all(v.astype(np.float32).astype(np.float64) == v))
@@ -171,6 +178,20 @@ class Solver:
                 raise ValueError(f"b and x must be FP64 device arrays of at least {self.n} elements")
         st = SolveStats()
         code = lib().ispm_solver_solve(self.p, b.vp, x.vp, C.byref(st))
+        if code != 0 and not (allow_fail and code == 6):
+            check(code)
+        return st
+
+    def solve_normalised(self, b: DeviceArray, x: DeviceArray, tolerance: float, rel_tol: float = 0.0,
+                         allow_fail=False) -> NormStats:
+        """The solve with OpenFOAM's convergence test (ispm_solver_solve_normalised): residual
+        sum|b - A x| / normFactor below `tolerance`, or below `rel_tol` times the initial one."""
+        for a in (b, x):
+            if a.dtype != np.float64 or a.n < self.n:
+                raise ValueError(f"b and x must be FP64 device arrays of at least {self.n} elements")
+        st = NormStats()
+        code = lib().ispm_solver_solve_normalised(self.p, b.vp, x.vp, float(tolerance), float(rel_tol),
+                                                  C.byref(st))
         if code != 0 and not (allow_fail and code == 6):
             check(code)
         return st

@@ -19,6 +19,14 @@ What pins each oracle function (tests/test_oracle.py):
   droptol tuner     the reference's own DroptolTuner compiled from its header
                     (oracle/_ref/libref_droptol.so) on identical time sequences, and the
                     closed-form golden-section probe sequence.
+  ldu_dense         lduMatrix (OpenFOAM owner/neighbour face addressing) to a dense matrix,
+                    entry by entry in pure Python: a hand-built 3-cell case.
+  foam_residual     OpenFOAM's normalised residual (lduMatrix::solver::normFactor): a hand-
+                    computed 2x2 case; invariance under scaling A and b together; for a
+                    constant x the first term of the norm factor vanishes (A x = sumA xbar),
+                    closed form; zero at the exact solution.  The foam-extend caller is not
+                    available, so its use of this measure is parity unpinned; the formula is
+                    OpenFOAM's.
 """
 from __future__ import annotations
 
@@ -269,3 +277,28 @@ class DroptolTuner:
             self._f("free")(self.h)
         except Exception:
             pass
+
+
+def ldu_dense(n, lower_addr, upper_addr, diag, upper, lower=None):
+    """Dense matrix of an lduMatrix: A[i][i] = diag[i]; face f: A[l][u] = upper[f],
+    A[u][l] = lower[f] (upper when lower is None)."""
+    D = np.zeros((n, n))
+    for i in range(n):
+        D[i, i] = diag[i]
+    for f in range(len(lower_addr)):
+        l, u = int(lower_addr[f]), int(upper_addr[f])
+        D[l, u] = upper[f]
+        D[u, l] = (upper if lower is None else lower)[f]
+    return D
+
+
+def foam_residual(A, b, x0, x):
+    """(normFactor, residual): OpenFOAM's normalised residual of x, with the norm factor of
+    the initial x0: xbar = mean(x0), pA = (A 1) xbar,
+    normFactor = sum |A x0 - pA| + |b - pA| + 1e-20, residual = sum |b - A x| / normFactor."""
+    n = A.n_rows
+    sumA = csr_spmv(A, np.ones(n))
+    pA = sumA * (np.sum(x0) / n)
+    nf = float(np.sum(np.abs(csr_spmv(A, x0) - pA) + np.abs(b - pA))) + 1e-20
+    return nf, float(np.sum(np.abs(b - csr_spmv(A, x)))) / nf
+

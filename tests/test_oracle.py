@@ -269,3 +269,40 @@ def test_droptol_tuner_matches_reference_build(seed):
             a.restart(); b.restart()
         if step == 30:
             a.reset_phase(); b.reset_phase()
+
+
+# ------------------------------------------------------------------ lduMatrix / normalised residual
+def test_ldu_dense_hand_case():
+    # 3 cells, faces (0,1) and (1,2): upper = A[l][u], lower = A[u][l]
+    D = O.ldu_dense(3, [0, 1], [1, 2], [4.0, 5.0, 6.0], [-1.0, -2.0], [-3.0, -4.0])
+    np.testing.assert_array_equal(D, [[4, -1, 0], [-3, 5, -2], [0, -4, 6]])
+    S = O.ldu_dense(3, [0, 1], [1, 2], [4.0, 5.0, 6.0], [-1.0, -2.0])
+    np.testing.assert_array_equal(S, S.T)
+
+
+def test_foam_residual_hand_case():
+    # A = [[2, -1], [-1, 3]], b = (1, 2), x0 = (1, 3): sumA = (1, 2), xbar = 2, pA = (2, 4);
+    # A x0 = (-1, 8): |A x0 - pA| = (3, 4), |b - pA| = (1, 2) -> normFactor = 10 (+1e-20);
+    # x = (0, 0): sum |b - A x| = 3 -> residual 0.3
+    A = I.CSR(2, 2, [0, 2, 4], [0, 1, 0, 1], [2.0, -1.0, -1.0, 3.0])
+    nf, r = O.foam_residual(A, np.array([1.0, 2.0]), np.array([1.0, 3.0]), np.zeros(2))
+    assert abs(nf - 10.0) < 1e-15 and abs(r - 0.3) < 1e-15
+
+
+def test_foam_residual_invariants():
+    A = G.stencil(6, 5, 4, 7, contrast=10, seed=2)
+    n = A.n_rows
+    b, x0, x = G.uniform(1, n), G.uniform(2, n), G.uniform(3, n)
+    nf, r = O.foam_residual(A, b, x0, x)
+    # scaling A and b together leaves the normalised residual unchanged
+    As = I.CSR(n, n, A.row_ptr, A.cols, A.vals * 8.0)
+    nf2, r2 = O.foam_residual(As, b * 8.0, x0, x)
+    assert abs(r2 - r) <= 1e-14 * r and abs(nf2 - 8 * nf) <= 1e-12 * nf
+    # a constant initial field: A x0 = sumA xbar, only |b - pA| remains
+    c = np.full(n, 0.7)
+    nf3, _ = O.foam_residual(A, b, c, x)
+    sumA = dense(A).sum(axis=1)
+    assert abs(nf3 - (np.abs(b - 0.7 * sumA).sum() + 1e-20)) <= 1e-12 * nf3
+    # zero at the exact solution
+    xs = np.linalg.solve(dense(A), b)
+    assert O.foam_residual(A, b, x0, xs)[1] < 1e-13
