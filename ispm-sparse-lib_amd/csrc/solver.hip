@@ -808,7 +808,7 @@ int solve_core(ispm_solver *S, const double *b, double *x, measure &mc, ispm_sol
   /* rows of the matrix (the vectors are padded beyond them) */
   const int nr = std::min(S->n, Ares->dev.n_cols > 0 ? Ares->dev.n_cols : S->n);
   if (mc.normalised) {
-    /* sumA = A 1 (into tmp64 via r64 as the ones vector), then the sum of x */
+    /* sumA = A 1 (r64 holds the ones vector for it), then the sum of x */
     if (!S->sumA && (st = alloc((void **)&S->sumA, 8 * (size_t)n + 64))) return st;
     hipLaunchKernelGGL(k_ones, g, bl, 0, sm, S->r64, nr, n);
     ISPM_HIP(hipGetLastError());
@@ -838,7 +838,8 @@ int solve_core(ispm_solver *S, const double *b, double *x, measure &mc, ispm_sol
     target = mc.tol;
     abs2 = mc.tol * bnorm;
   }
-  auto met = [&](double v) { return mc.normalised ? v < target : v <= target; };
+  /* an exact initial x has nothing left to reduce, whatever the targets */
+  auto met = [&](double v) { return v == 0.0 || (mc.normalised ? v < target : v <= target); };
   const double margin = 0.9;
   for (void *v : {S->z, S->q, S->p, S->sv}) if (v && (st = zero(v, vb))) return st;
 
