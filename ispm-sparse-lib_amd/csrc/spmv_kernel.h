@@ -519,6 +519,10 @@ struct sym_args {
   int d[kSymPlanes];             /* upper diagonal offsets, d[0] = 0 */
   unsigned up_off[kSymPlanes];   /* LDS index of x[row0 + d[k]] (windows as launched) */
   unsigned lo_off[kSymPlanes];   /* LDS index of x[row0 - d[k]] (k >= 1) */
+  /* the window covers as one list of 16-byte vectors (LDS vector v at v * VX): run r's
+   * vectors start at wvbase[r] and read x[row0 + wdelta[r] + v * VX] */
+  unsigned wvbase[9], wtotal;
+  int wdelta[9];
   const unsigned *skip;
   const void *dotv;
   double *dparts;
@@ -588,13 +592,18 @@ __global__ void __launch_bounds__((256 / sym_rows_per_thread<T, E>())) symdia_ke
   const bool xvec = (reinterpret_cast<uintptr_t>(xv) % 16) == 0;
   const bool fast = xvec && pat.vec_runs && pat.nrun <= NR && (long long)row0 + pat.cover_lo >= 0 &&
                     (long long)row0 + pat.cover_hi <= (long long)ncols;
-  vecx<T> w[NR];
+  /* the window vectors spread evenly over the threads (at most NW each, the launch checks) */
+  constexpr unsigned NW = 5;
+  vecx<T> w[NW];
   if (fast) {
 #pragma unroll
-    for (unsigned r = 0; r < NR; r++) {
-      const unsigned rr = r < pat.nrun ? r : 0u;
-      const unsigned nv = pat.rlen[rr] / VX, v = t < nv ? t : nv - 1;
-      w[r] = *reinterpret_cast<const vecx<T> *>(xv + (row0 + (unsigned)pat.rrel[rr] + v * VX));
+    for (unsigned q = 0; q < NW; q++) {
+      const unsigned v0 = t + q * NT, v = v0 < a.wtotal ? v0 : a.wtotal - 1;
+      int dl = a.wdelta[0];
+#pragma unroll
+      for (unsigned r = 1; r < NR; r++)
+        if (r < pat.nrun && v >= a.wvbase[r]) dl = a.wdelta[r];
+      w[q] = *reinterpret_cast<const vecx<T> *>(xv + (unsigned)((int)row0 + dl + (int)(v * VX)));
     }
   }
   /* own upper planes (rows row .. row+H-1) and the mirrored lower ones, all requested before
@@ -621,8 +630,8 @@ __global__ void __launch_bounds__((256 / sym_rows_per_thread<T, E>())) symdia_ke
   }
   if (fast) {
 #pragma unroll
-    for (unsigned r = 0; r < NR; r++)
-      if (r < pat.nrun && t < pat.rlen[r] / VX) *reinterpret_cast<vecx<T> *>(win + pat.rbase[r] + t * VX) = w[r];
+    for (unsigned q = 0; q < NW; q++)
+      if (t + q * NT < a.wtotal) *reinterpret_cast<vecx<T> *>(win + (t + q * NT) * VX) = w[q];
   } else {
     for (unsigned r = 0; r < pat.nrun; r++) {
       const long long x0 = (long long)row0 + pat.rrel[r];
@@ -687,9 +696,21 @@ int launch_symdia(const sym_args &a0, const diag_pattern &pat0, const void *x, v
     a.dparts = dparts;
     *dgrid = a.per_xcd * 8;
   }
-  constexpr unsigned NT = 256 / sym_rows_per_thread<T, E>();
+  constexpr unsigned NT = 256 / sym_rows_per_thread<T, E>(), VX = 16 / sizeof(T);
   diag_pattern c;
-  cover_pattern(pat0, 16 / sizeof(T), NT, 9, c);
+  cover_pattern(pat0, VX, NT, 9, c);
+  /* the covers lie back to back in LDS: vector v of the list is at LDS index v * VX */
+  a.wtotal = 0;
+  for (unsigned r = 0; r < 9; r++) {
+    const unsigned rr = r < c.nrun ? r : 0u;
+    a.wvbase[r] = r < c.nrun ? c.rbase[r] / VX : ~0u;
+    a.wdelta[r] = c.rrel[rr] - (int)c.rbase[rr];
+    if (r < c.nrun) {
+      if (c.rbase[r] != a.wtotal * VX || c.rlen[r] % VX) c.vec_runs = 0;
+      a.wtotal += c.rlen[r] / VX;
+    }
+  }
+  if (a.wtotal == 0 || a.wtotal > 5 * NT) c.vec_runs = 0;
   const size_t lds = (size_t)c.lds_len * sizeof(T);
   if (lds > 65536) return fail(ISPM_ERR_UNSUPPORTED, "spmv: symmetric windows need %zu bytes of LDS", lds);
   /* pattern positions of +d[k] and -d[k] (the plan lists -d[K-1] .. -d[1], then d[0] ..
