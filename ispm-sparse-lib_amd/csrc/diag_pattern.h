@@ -24,11 +24,16 @@ struct diag_pattern {
   unsigned rlen[kPatPlanes];     /* window length (S + span of the run) */
   unsigned lds_len;              /* sum of the window lengths */
   /* Set by the SpMV launcher when it restates the windows as their aligned covers (the
-   * enclosing runs of whole 16-byte vectors of x): the covers span x[first row + cover_lo,
-   * first row + cover_hi), and vec_runs is nonzero when each cover is at most one vector per
-   * thread and all runs fit the kernel's registers. */
+   * enclosing runs of whole 16-byte vectors of x, back to back in LDS): the covers span
+   * x[first row + cover_lo, first row + cover_hi); read as one list of vectors, LDS vector v
+   * (at index v * VX) belongs to the last run r with wvbase[r] <= v and holds
+   * x[first row + wdelta[r] + v * VX]; vec_runs is nonzero when the list fits the kernel's
+   * registers (wtotal vectors, at most a fixed number per thread). */
   long long cover_lo, cover_hi;
   unsigned vec_runs;
+  unsigned wvbase[kPatPlanes];
+  int wdelta[kPatPlanes];
+  unsigned wtotal;
 };
 
 }  // namespace ispm

@@ -279,12 +279,13 @@ constexpr unsigned pattern_runs_in_regs(unsigned NPM) { return NPM <= 8 ? 8u : N
 
 /* The pattern restated for x loads of VX elements: each run's window [rrel, rrel + rlen)
  * widened to its VX-aligned cover, covers packed at VX-aligned LDS offsets, poff moved with
- * them; NT threads per workgroup and NR runs in registers decide vec_runs.  Slice first rows
- * are multiples of the slice height, so the covers are aligned in every slice. */
-inline void cover_pattern(const diag_pattern &p, unsigned VX, unsigned NT, unsigned NR, diag_pattern &c)
+ * them; the kernel's run bound NR and its NW vectors per thread (of NT threads) decide
+ * vec_runs.  Slice first rows are multiples of the slice height, so the covers are aligned in
+ * every slice. */
+inline void cover_pattern(const diag_pattern &p, unsigned VX, unsigned NT, unsigned NR, unsigned NW, diag_pattern &c)
 {
   c = p;
-  unsigned used = 0, maxnv = 0;
+  unsigned used = 0;
   long long lo = 0, hi = 0;
   for (unsigned r = 0; r < p.nrun; r++) {
     const long long rel = p.rrel[r];
@@ -296,7 +297,6 @@ inline void cover_pattern(const diag_pattern &p, unsigned VX, unsigned NT, unsig
     used += len;
     lo = r == 0 || a < lo ? a : lo;
     hi = r == 0 || a + len > hi ? a + len : hi;
-    maxnv = len / VX > maxnv ? len / VX : maxnv;
   }
   for (unsigned j = 0; j < p.n; j++)
     for (unsigned r = 0; r < p.nrun; r++)
@@ -307,7 +307,14 @@ inline void cover_pattern(const diag_pattern &p, unsigned VX, unsigned NT, unsig
   c.lds_len = used;
   c.cover_lo = lo;
   c.cover_hi = hi;
-  c.vec_runs = p.nrun <= NR && maxnv <= NT ? 1u : 0u;
+  /* the covers as one vector list, spread over the NT threads at up to NW vectors each */
+  c.wtotal = used / VX;
+  for (unsigned r = 0; r < (unsigned)kPatPlanes; r++) {
+    const unsigned rr = r < p.nrun ? r : 0u;
+    c.wvbase[r] = r < p.nrun ? c.rbase[r] / VX : ~0u;
+    c.wdelta[r] = c.rrel[rr] - (int)c.rbase[rr];
+  }
+  c.vec_runs = p.nrun <= NR && c.wtotal > 0 && c.wtotal <= NW * NT ? 1u : 0u;
 }
 
 /* H consecutive elements (aligned to H) of one plane, converted to T */
@@ -366,13 +373,18 @@ slell_diagpat_kernel(spmv_args a, diag_pattern pat, const unsigned *__restrict__
                       (long long)row0 + pat.cover_hi <= (long long)ncols;
     vecx<T> w[NR];
     if (fast) {
-      /* runs past the pattern's count load run 0 again: a fixed number of loads */
+      /* the window vectors spread evenly over the threads; rounds past the list's end are
+       * skipped by the whole workgroup, the last round's spare lanes reload its last vector */
 #pragma unroll
-      for (unsigned r = 0; r < NR; r++) {
-        const unsigned rr = r < nrun ? r : 0u;
-        const unsigned nv = pat.rlen[rr] / VX, v = t < nv ? t : nv - 1;
-        w[r] = *reinterpret_cast<const vecx<T> *>(xv + (row0 + (unsigned)pat.rrel[rr] + v * VX));
-      }
+      for (unsigned q = 0; q < NR; q++)
+        if (q * NT < pat.wtotal) {
+          const unsigned v0 = t + q * NT, v = v0 < pat.wtotal ? v0 : pat.wtotal - 1;
+          int dl = pat.wdelta[0];
+#pragma unroll
+          for (unsigned r = 1; r < NR; r++)
+            if (r < nrun && v >= pat.wvbase[r]) dl = pat.wdelta[r];
+          w[q] = *reinterpret_cast<const vecx<T> *>(xv + (unsigned)((int)row0 + dl + (int)(v * VX)));
+        }
     }
     /* every plane requested at once; planes past the pattern's count read one address that
      * is the same in all lanes (a single request per wave), keeping the number of loads in
@@ -383,8 +395,8 @@ slell_diagpat_kernel(spmv_args a, diag_pattern pat, const unsigned *__restrict__
       rows_load<T, E, H>::get(el + e0 + (j < np ? r0 : 0u), j < np ? (j << LOG2S) : 0u, ev[j]);
     if (fast) {
 #pragma unroll
-      for (unsigned r = 0; r < NR; r++)
-        if (r < nrun && t < pat.rlen[r] / VX) *reinterpret_cast<vecx<T> *>(win + pat.rbase[r] + t * VX) = w[r];
+      for (unsigned q = 0; q < NR; q++)
+        if (t + q * NT < pat.wtotal) *reinterpret_cast<vecx<T> *>(win + (t + q * NT) * VX) = w[q];
     } else {
       for (unsigned r = 0; r < nrun; r++) {
         const unsigned x0 = row0 + (unsigned)pat.rrel[r], L = pat.rlen[r], b = pat.rbase[r];
@@ -488,7 +500,7 @@ int launch_diagpat(const ispm_slell_desc *A, const diag_pattern &pat, const unsi
   }
   const unsigned NPM = pat.n <= 8 ? 8u : pat.n <= 27 ? 27u : (unsigned)kPatPlanes;
   diag_pattern c;
-  cover_pattern(pat, 16 / sizeof(T), A->slice_height / H, pattern_runs_in_regs(NPM), c);
+  cover_pattern(pat, 16 / sizeof(T), A->slice_height / H, pattern_runs_in_regs(NPM), pattern_runs_in_regs(NPM), c);
   const size_t lds = (size_t)c.lds_len * sizeof(T);
   if (lds > 65536) return fail(ISPM_ERR_UNSUPPORTED, "spmv: pattern windows need %zu bytes of LDS", lds);
   const dim3 g(a.per_xcd * 8), b(A->slice_height / H);
@@ -519,10 +531,6 @@ struct sym_args {
   int d[kSymPlanes];             /* upper diagonal offsets, d[0] = 0 */
   unsigned up_off[kSymPlanes];   /* LDS index of x[row0 + d[k]] (windows as launched) */
   unsigned lo_off[kSymPlanes];   /* LDS index of x[row0 - d[k]] (k >= 1) */
-  /* the window covers as one list of 16-byte vectors (LDS vector v at v * VX): run r's
-   * vectors start at wvbase[r] and read x[row0 + wdelta[r] + v * VX] */
-  unsigned wvbase[9], wtotal;
-  int wdelta[9];
   const unsigned *skip;
   const void *dotv;
   double *dparts;
@@ -598,11 +606,11 @@ __global__ void __launch_bounds__((256 / sym_rows_per_thread<T, E>())) symdia_ke
   if (fast) {
 #pragma unroll
     for (unsigned q = 0; q < NW; q++) {
-      const unsigned v0 = t + q * NT, v = v0 < a.wtotal ? v0 : a.wtotal - 1;
-      int dl = a.wdelta[0];
+      const unsigned v0 = t + q * NT, v = v0 < pat.wtotal ? v0 : pat.wtotal - 1;
+      int dl = pat.wdelta[0];
 #pragma unroll
       for (unsigned r = 1; r < NR; r++)
-        if (r < pat.nrun && v >= a.wvbase[r]) dl = a.wdelta[r];
+        if (r < pat.nrun && v >= pat.wvbase[r]) dl = pat.wdelta[r];
       w[q] = *reinterpret_cast<const vecx<T> *>(xv + (unsigned)((int)row0 + dl + (int)(v * VX)));
     }
   }
@@ -631,7 +639,7 @@ __global__ void __launch_bounds__((256 / sym_rows_per_thread<T, E>())) symdia_ke
   if (fast) {
 #pragma unroll
     for (unsigned q = 0; q < NW; q++)
-      if (t + q * NT < a.wtotal) *reinterpret_cast<vecx<T> *>(win + (t + q * NT) * VX) = w[q];
+      if (t + q * NT < pat.wtotal) *reinterpret_cast<vecx<T> *>(win + (t + q * NT) * VX) = w[q];
   } else {
     for (unsigned r = 0; r < pat.nrun; r++) {
       const long long x0 = (long long)row0 + pat.rrel[r];
@@ -698,19 +706,7 @@ int launch_symdia(const sym_args &a0, const diag_pattern &pat0, const void *x, v
   }
   constexpr unsigned NT = 256 / sym_rows_per_thread<T, E>(), VX = 16 / sizeof(T);
   diag_pattern c;
-  cover_pattern(pat0, VX, NT, 9, c);
-  /* the covers lie back to back in LDS: vector v of the list is at LDS index v * VX */
-  a.wtotal = 0;
-  for (unsigned r = 0; r < 9; r++) {
-    const unsigned rr = r < c.nrun ? r : 0u;
-    a.wvbase[r] = r < c.nrun ? c.rbase[r] / VX : ~0u;
-    a.wdelta[r] = c.rrel[rr] - (int)c.rbase[rr];
-    if (r < c.nrun) {
-      if (c.rbase[r] != a.wtotal * VX || c.rlen[r] % VX) c.vec_runs = 0;
-      a.wtotal += c.rlen[r] / VX;
-    }
-  }
-  if (a.wtotal == 0 || a.wtotal > 5 * NT) c.vec_runs = 0;
+  cover_pattern(pat0, VX, NT, 9, 5, c);
   const size_t lds = (size_t)c.lds_len * sizeof(T);
   if (lds > 65536) return fail(ISPM_ERR_UNSUPPORTED, "spmv: symmetric windows need %zu bytes of LDS", lds);
   /* pattern positions of +d[k] and -d[k] (the plan lists -d[K-1] .. -d[1], then d[0] ..

@@ -23,6 +23,12 @@ enum spmv_plan_flags {
   SPMV_PLAN_NO_DEVICE = 002
 };
 
+namespace ispm_detail {
+/* the minimum slice height plans convert with: a tuned fixed height below 4 (the library
+ * times heights from 1) is taken as it is instead of being refused */
+inline int plan_min_height(unsigned S, unsigned V) { return !V && S < 4 ? (int)S : 4; }
+}  // namespace ispm_detail
+
 template<class T, class E = T>
 struct spmv_plan
 {
@@ -54,7 +60,7 @@ struct spmv_plan
   {
     delete host_matrix;
     host_matrix = new slell_matrix<E, host_memory_space_tag>(
-        csr_to_slell(m, mparams.S, mparams.H, mparams.V, mparams.D));
+        csr_to_slell(m, mparams.S, mparams.H, mparams.V, mparams.D, ispm_detail::plan_min_height(mparams.S, mparams.V)));
   }
 
   void execute_spmv(vector<T, device_memory_space_tag> &x, vector<T, device_memory_space_tag> &y)
@@ -79,7 +85,7 @@ spmv_plan<T, E>::spmv_plan(const csr_matrix<E> &m, spmv_plan_kind kind, unsigned
     double secs;
     ISPM_CHECK(ispm_plan_tune(ispm_type_code<T>::value, ispm_type_code<E>::value, &c, &S, &W, &V, &D, &lp, &secs));
     mparams = slell_params{S, W, (unsigned)V, (unsigned)D};
-    host_matrix = new slell_matrix<E, host_memory_space_tag>(csr_to_slell(m, S, W, V != 0, D != 0));
+    host_matrix = new slell_matrix<E, host_memory_space_tag>(csr_to_slell(m, S, W, V != 0, D != 0, ispm_detail::plan_min_height(S, V)));
     upload();
   } else if (kind == PLAN_ESTIMATE) {
     ispm_csr c = ispm_detail::csr_view(m);
@@ -87,7 +93,7 @@ spmv_plan<T, E>::spmv_plan(const csr_matrix<E> &m, spmv_plan_kind kind, unsigned
     int V, D;
     ISPM_CHECK(ispm_plan_estimate(ispm_type_code<T>::value, ispm_type_code<E>::value, &c, &S, &W, &V, &D, &lp));
     mparams = slell_params{S, W, (unsigned)V, (unsigned)D};
-    host_matrix = new slell_matrix<E, host_memory_space_tag>(csr_to_slell(m, S, W, V != 0, D != 0));
+    host_matrix = new slell_matrix<E, host_memory_space_tag>(csr_to_slell(m, S, W, V != 0, D != 0, ispm_detail::plan_min_height(S, V)));
     if (up) upload();
   } else {
     host_matrix = new slell_matrix<E, host_memory_space_tag>(csr_to_slell_guess_format(m));

@@ -48,6 +48,7 @@ static void count(void *v) { ++*static_cast<long *>(v); }
 
 int main()
 {
+  std::setvbuf(stdout, nullptr, _IOLBF, 0);
   ispm_initialize();
   std::printf("device %s\n", get_device_string().str);
   csr_matrix<double> A = poisson(12);
@@ -96,6 +97,47 @@ int main()
     vector<double, host_memory_space_tag> hz(dz);
     for (int i = 0; i < n; i++) REQUIRE(hz[i] == 0.0);
     std::printf("ok coefficient refresh err=%.1e\n", err);
+  }
+  {
+    /* short irregular rows: the estimated plan takes slice height 1 (below the conversion's
+     * default minimum of 4), also when the coefficients are refreshed */
+    csr_matrix<double> R;
+    const int nr = 3000;
+    R.n_rows = R.n_cols = nr;
+    R.row_ptr.push_back(0);
+    unsigned long long s = 12345;
+    auto next = [&s]() { s = s * 6364136223846793005ULL + 1442695040888963407ULL; return (unsigned)(s >> 33); };
+    for (int r = 0; r < nr; r++) {
+      const int len = 1 + (int)(next() % 3);
+      for (int k = 0; k < len; k++) {
+        R.cols.push_back((int)(next() % nr));
+        R.elms.push_back(1.0 + (next() % 7));
+      }
+      R.row_ptr.push_back((int)R.cols.size());
+    }
+    R.n_nz = (int)R.cols.size();
+    std::vector<double> rx(nr), ry(nr, 0.0);
+    for (int i = 0; i < nr; i++) rx[i] = std::cos(0.11 * i);
+    for (int r = 0; r < nr; r++)
+      for (int k = R.row_ptr[r]; k < R.row_ptr[r + 1]; k++) ry[r] += R.elms[k] * rx[R.cols[k]];
+    spmv_plan<double> plan(R, PLAN_ESTIMATE);
+    for (int pass = 0; pass < 2; pass++) {
+      if (pass) {
+        csr_matrix<double> R2 = R;
+        for (size_t k = 0; k < R2.elms.size(); k++) R2.elms[k] *= 3.0;
+        plan.change_host_matrix(R2);
+        plan.update_device_matrix_coefs();
+      }
+      vector<double, host_memory_space_tag> hx(R.n_rows, 0.0);
+      for (int i = 0; i < nr; i++) hx[i] = rx[i];
+      vector<double, device_memory_space_tag> dx(hx), dy(R.n_rows);
+      plan.execute_spmv(dx, dy);
+      vector<double, host_memory_space_tag> hy(dy);
+      double err = 0;
+      for (int i = 0; i < nr; i++) err = std::fmax(err, std::fabs(hy[i] - (pass ? 3.0 : 1.0) * ry[i]));
+      REQUIRE(err < 1e-12);
+    }
+    std::printf("ok estimated plan S=%u with refresh\n", plan.mparams.S);
   }
   {
     csr_matrix<double> B = poisson(6);
