@@ -276,6 +276,8 @@ template<typename T> using vecx = T __attribute__((ext_vector_type(16 / sizeof(T
 
 /* x window runs a pattern kernel instance keeps in registers (bound by its plane count) */
 constexpr unsigned pattern_runs_in_regs(unsigned NPM) { return NPM <= 8 ? 8u : NPM <= 27 ? 9u : 16u; }
+/* x-window vectors per thread the pattern kernel stages in registers */
+constexpr unsigned pattern_window_vectors(unsigned NPM) { return NPM <= 27 ? 5u : 8u; }
 
 /* The pattern restated for x loads of VX elements: each run's window [rrel, rrel + rlen)
  * widened to its VX-aligned cover, covers packed at VX-aligned LDS offsets, poff moved with
@@ -371,20 +373,21 @@ slell_diagpat_kernel(spmv_args a, diag_pattern pat, const unsigned *__restrict__
      * them waits for them alone; otherwise a bounds-checked loop stages them after. */
     const bool fast = xvec && pat.vec_runs && (long long)row0 + pat.cover_lo >= 0 &&
                       (long long)row0 + pat.cover_hi <= (long long)ncols;
-    vecx<T> w[NR];
+    constexpr unsigned NW = pattern_window_vectors(NPM);
+    vecx<T> w[NW];
     if (fast) {
-      /* the window vectors spread evenly over the threads; rounds past the list's end are
-       * skipped by the whole workgroup, the last round's spare lanes reload its last vector */
+      /* the window vectors spread evenly over the threads, a fixed NW loads each (lanes past
+       * the list's end reload its last vector: one request per wave), all issued before the
+       * element loads so that staging them waits for them alone */
 #pragma unroll
-      for (unsigned q = 0; q < NR; q++)
-        if (q * NT < pat.wtotal) {
-          const unsigned v0 = t + q * NT, v = v0 < pat.wtotal ? v0 : pat.wtotal - 1;
-          int dl = pat.wdelta[0];
+      for (unsigned q = 0; q < NW; q++) {
+        const unsigned v0 = t + q * NT, v = v0 < pat.wtotal ? v0 : pat.wtotal - 1;
+        int dl = pat.wdelta[0];
 #pragma unroll
-          for (unsigned r = 1; r < NR; r++)
-            if (r < nrun && v >= pat.wvbase[r]) dl = pat.wdelta[r];
-          w[q] = *reinterpret_cast<const vecx<T> *>(xv + (unsigned)((int)row0 + dl + (int)(v * VX)));
-        }
+        for (unsigned r = 1; r < NR; r++)
+          if (r < nrun && v >= pat.wvbase[r]) dl = pat.wdelta[r];
+        w[q] = *reinterpret_cast<const vecx<T> *>(xv + (unsigned)((int)row0 + dl + (int)(v * VX)));
+      }
     }
     /* every plane requested at once; planes past the pattern's count read one address that
      * is the same in all lanes (a single request per wave), keeping the number of loads in
@@ -395,7 +398,7 @@ slell_diagpat_kernel(spmv_args a, diag_pattern pat, const unsigned *__restrict__
       rows_load<T, E, H>::get(el + e0 + (j < np ? r0 : 0u), j < np ? (j << LOG2S) : 0u, ev[j]);
     if (fast) {
 #pragma unroll
-      for (unsigned q = 0; q < NR; q++)
+      for (unsigned q = 0; q < NW; q++)
         if (t + q * NT < pat.wtotal) *reinterpret_cast<vecx<T> *>(win + (t + q * NT) * VX) = w[q];
     } else {
       for (unsigned r = 0; r < nrun; r++) {
@@ -500,7 +503,7 @@ int launch_diagpat(const ispm_slell_desc *A, const diag_pattern &pat, const unsi
   }
   const unsigned NPM = pat.n <= 8 ? 8u : pat.n <= 27 ? 27u : (unsigned)kPatPlanes;
   diag_pattern c;
-  cover_pattern(pat, 16 / sizeof(T), A->slice_height / H, pattern_runs_in_regs(NPM), pattern_runs_in_regs(NPM), c);
+  cover_pattern(pat, 16 / sizeof(T), A->slice_height / H, pattern_runs_in_regs(NPM), pattern_window_vectors(NPM), c);
   const size_t lds = (size_t)c.lds_len * sizeof(T);
   if (lds > 65536) return fail(ISPM_ERR_UNSUPPORTED, "spmv: pattern windows need %zu bytes of LDS", lds);
   const dim3 g(a.per_xcd * 8), b(A->slice_height / H);
