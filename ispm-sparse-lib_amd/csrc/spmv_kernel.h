@@ -474,9 +474,9 @@ int launch_diagpat(const ispm_slell_desc *A, const diag_pattern &pat, const unsi
                    double *dparts = nullptr, unsigned *dgrid = nullptr)
 {
   /* Rows per thread is the kernel's choice (each row's sum is the same either way): one,
-   * measured fastest on a 27-point 192^3 stencil with 256-row slices (FP32: 141-146 us with
-   * one row per thread, 151 us with two; FP64: 292 us with one, 350 us with two, whose LDS
-   * windows then halve the waves resident per CU). */
+   * measured fastest on a 27-point 192^3 stencil with 256-row slices (FP32: 137-140 us with
+   * one row per thread, 146 us with two; FP64: 281-288 us with one, 348 us with two, whose
+   * windows then exceed five vectors per thread and take the bounds-checked path). */
   unsigned H = 1;
   if (const char *e = std::getenv("ISPM_DIAGPAT_H")) H = (unsigned)std::atoi(e);
   if ((H != 1 && H != 2 && H != 4) || A->slice_height / H < 64 ||
