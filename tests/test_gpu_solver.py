@@ -93,14 +93,17 @@ def test_pipelined_l1_matches_oracle_semantics(gpu):
     assert res.l1_resnorm <= np.sqrt(A.n_rows) * 1e-10 * np.linalg.norm(b) * 2
 
 
+@pytest.mark.parametrize("sym", [False, True])
 @pytest.mark.parametrize("method", [S.PCG, S.PCG1])
-def test_graph_and_eager_agree(gpu, method):
+def test_graph_and_eager_agree(gpu, method, sym):
+    """Captured and eager iterations give identical results, with SlELL and with symmetric
+    storage of A."""
     A = G.poisson7(16)
     b = O.csr_spmv(A, G.uniform(6, A.n_rows))
     M = I.fastainv_sym(A, I.F64, droptol=0.05)
     # fixed storage choices: the timed set-up choices could differ between the two solvers
     # (equal results need the same summation orders)
-    fixed = dict(symmetric=False, split_diag=True, fsai_product=True)
+    fixed = dict(symmetric=sym, split_diag=True, fsai_product=True)
     r1 = S.solve(A, b, M=M, tol=1e-10, precision="fp64", method=method, use_graph=True, **fixed)
     r2 = S.solve(A, b, M=M, tol=1e-10, precision="fp64", method=method, use_graph=False, **fixed)
     assert r1.iterations == r2.iterations
