@@ -671,9 +671,12 @@ __global__ void __launch_bounds__((256 / sym_rows_per_thread<T, E>())) symdia_ke
   if (a.dparts) {
     __shared__ double red[4];
     const T *__restrict__ dv = static_cast<const T *>(a.dotv);
+    /* CG's (p, A p) has x itself as the dot vector: its rows are in the window of offset 0 */
+    const bool dx = dv == xv;
     double d[1] = {0.0};
 #pragma unroll
-    for (unsigned e = 0; e < H; e++) d[0] += (double)dv[row + e] * (double)acc[e];
+    for (unsigned e = 0; e < H; e++)
+      d[0] += (double)(dx ? win[a.up_off[0] + H * t + e] : dv[row + e]) * (double)acc[e];
     dev::block_sum<NT, 1>(d, red);
     if (t == 0) a.dparts[blockIdx.x] = d[0];
   }
@@ -1235,9 +1238,11 @@ __device__ __forceinline__ void diag_body(const spmv_args &a, const split_args &
         if (!((bits >> q) & 1u)) y[(size_t)i * V + q] = yy[q];
     }
     if (sa.dparts) {
+      /* the preconditioner's dot (r, M r) has x itself as the dot vector: no second read */
+      const bool dx = dv == x;
 #pragma unroll
       for (unsigned q = 0; q < V; q++)
-        if (!((bits >> q) & 1u)) acc[0] += (double)dv[(size_t)i * V + q] * (double)yy[q];
+        if (!((bits >> q) & 1u)) acc[0] += (double)(dx ? xx[q] : dv[(size_t)i * V + q]) * (double)yy[q];
     }
   }
   if (sa.dparts) {
