@@ -23,6 +23,7 @@ struct diag_pattern {
   unsigned rbase[kPatPlanes];    /* LDS index of run r's window */
   unsigned rlen[kPatPlanes];     /* window length (S + span of the run) */
   unsigned lds_len;              /* sum of the window lengths */
+  unsigned diag_plane1;          /* 1 + the plane of the main diagonal (offset 0), 0: none */
   /* Set by the SpMV launcher when it restates the windows as their aligned covers (the
    * enclosing runs of whole 16-byte vectors of x, back to back in LDS): the covers span
    * x[first row + cover_lo, first row + cover_hi); read as one list of vectors, LDS vector v

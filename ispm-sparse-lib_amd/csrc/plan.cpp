@@ -245,6 +245,8 @@ bool build_pattern(const conv::layout &L, diag_pattern &pat, std::vector<unsigne
   }
   pat.lds_len = used;
   if ((size_t)used * 8 > 65536) return false;
+  for (unsigned d = 0; d < pat.n; d++)
+    if (R[d] == (long long)S) pat.diag_plane1 = d + 1;
   mask.assign((ns + 31) / 32, 0u);
   for (unsigned s = 0; s < ns; s++)
     if (rel_of(s, rel) && rel == R) mask[s >> 5] |= 1u << (s & 31);

@@ -365,7 +365,8 @@ slell_diagpat_kernel(spmv_args a, diag_pattern pat, const unsigned *__restrict__
   T acc[H];
 #pragma unroll
   for (int h = 0; h < H; h++) acc[h] = 0;
-  if ((uni[slice >> 5] >> (slice & 31u)) & 1u) {
+  const bool on_pattern = (uni[slice >> 5] >> (slice & 31u)) & 1u;
+  if (on_pattern) {
     const unsigned np = pat.n, nrun = pat.nrun;
     /* x windows, staged with the rest of their 16-byte aligned covers (rrel, rlen, rbase and
      * poff describe the covers).  When every cover lies inside x and needs at most one vector
@@ -439,9 +440,13 @@ slell_diagpat_kernel(spmv_args a, diag_pattern pat, const unsigned *__restrict__
   if (a.dparts) {
     __shared__ double red[NT / 64];
     const T *__restrict__ dv = static_cast<const T *>(a.dotv);
+    /* CG's (p, A p) has x itself as the dot vector: on a pattern slice its rows are in the
+     * main diagonal's window */
+    const bool dx = dv == xv && on_pattern && pat.diag_plane1;
+    const unsigned o = dx ? pat.poff[pat.diag_plane1 - 1] + r0 : 0u;
     double d[1] = {0.0};
 #pragma unroll
-    for (int h = 0; h < H; h++) d[0] += (double)dv[row0 + r0 + h] * (double)acc[h];
+    for (int h = 0; h < H; h++) d[0] += (double)(dx ? win[o + h] : dv[row0 + r0 + h]) * (double)acc[h];
     dev::block_sum<NT, 1>(d, red);
     if (t == 0) a.dparts[blockIdx.x] = d[0];
   }
@@ -888,8 +893,10 @@ __device__ __forceinline__ void stream_body(const spmv_args &a, unsigned cap, un
     dterm[k] = 0;
     dvv[k] = 0;
     if (dst[k] != ~0u) {
-      if (a.rmap) dterm[k] = elt<T, E>::get(static_cast<const E *>(a.rdiag), dst[k]) * xv[dst[k]];
-      if (a.dparts) dvv[k] = dv[dst[k]];
+      /* the preconditioner's dot (r, M r) has x itself as the dot vector: one read of x */
+      const T xd = a.rmap || dv == xv ? xv[dst[k]] : (T)0;
+      if (a.rmap) dterm[k] = elt<T, E>::get(static_cast<const E *>(a.rdiag), dst[k]) * xd;
+      if (a.dparts) dvv[k] = dv == xv ? xd : dv[dst[k]];
     }
   }
   /* the element range comes from two (scalar) loads, so the element loads need not wait for
