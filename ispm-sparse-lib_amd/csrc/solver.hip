@@ -850,7 +850,9 @@ int solve_core(ispm_solver *S, const double *b, double *x, measure &mc, ispm_sol
       /* r = r64 / ||r64||; rho restarts the Krylov sequence (p = 0) */
       hipLaunchKernelGGL(k_cast_scaled<float>, g, bl, 0, sm, (float *)S->r, (const double *)S->r64, n, S->s);
       ISPM_HIP(hipGetLastError());
-      if ((e = pull(S))) return e;
+      /* the kernel's rscale, known here without a read-back (the r64 just pulled; sqrt is
+       * correctly rounded on both sides) */
+      h.rscale = std::sqrt(h.r64);
       h.rho = INFINITY;
       h.gamma_old = INFINITY;
       h.alpha_old = 1.0;
@@ -948,7 +950,7 @@ int solve_core(ispm_solver *S, const double *b, double *x, measure &mc, ispm_sol
       /* single-reduction PCG: s = A p is rescaled with p */
       if (S->single) hipLaunchKernelGGL(k_scale<float>, g, bl, 0, sm, (float *)S->sv, n, c);
       ISPM_HIP(hipGetLastError());
-      if ((st = pull(S))) break;
+      h.rscale = rmax;   /* what k_cast_scaled stored (sqrt of the pulled r64) */
       rec_target = std::max(S->o.reliable_delta, margin * abs2 / rmax);
       h.rho = h.rho * c * c;
       h.gamma_old = h.gamma_old * c * c;
