@@ -16,3 +16,15 @@ def test_cxx_api_smoke(gpu):
     print(out.stdout, out.stderr)
     assert out.returncode == 0, out.stderr
     assert "ok all" in out.stdout
+
+
+def test_cxx_foam_call_pattern(gpu):
+    """The foam-extend call pattern through the C-ABI alone (tests/cxx/foam_smoke.cpp):
+    lduMatrix to CSR, FSAI-PCG with OpenFOAM's normalised residual (checked against the
+    formula on the host), and a second time step refreshed on the device."""
+    exe = os.path.join(ROOT, "tests", "cxx", "foam_smoke.bin")
+    assert os.path.exists(exe), "build the library first (make -C ispm-sparse-lib_amd)"
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(out.stdout, out.stderr)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.count("ok step") == 2 and "ok all" in out.stdout
