@@ -87,6 +87,13 @@ def roofline(leg, key, kernel):
             "algorithmic_bytes": leg["stored_bytes"]}
 
 
+def host_spmv(A, x):
+    """A x in FP64 on the host (scipy), to make right-hand sides of known solutions."""
+    import scipy.sparse as sp
+    M = sp.csr_matrix((np.asarray(A.vals, np.float64), A.cols, A.row_ptr), shape=(A.n_rows, A.n_cols))
+    return M @ np.asarray(x, np.float64)
+
+
 def cpu_spmv_baseline(O, A, seconds=10.0):
     threads = O.set_threads(0)
     x = np.random.default_rng(0).uniform(-1, 1, A.n_rows)
@@ -159,6 +166,41 @@ def cpu_solve_baseline(O, A, b, factors, iters_needed, seconds=15.0):
             "seconds_per_iteration": per_iter}
 
 
+def foam_legs(I, G, O, m=128, steps=4, tol=1e-8):
+    """The foam-extend call pattern on an m^3 diffusion mesh given as an lduMatrix: the CSR
+    built once, then `steps` time steps whose new coefficients (lognormal conductances of a
+    new seed) are scattered into the same A plan on the device; each step computes the
+    approximate inverse at the DroptolTuner's probe and solves (FP64 pipelined PCG) to
+    OpenFOAM's normalised residual `tol` from the previous step's field, towards a new field
+    every step.  The approximate inverse is computed on the host, as the reference does."""
+    from ispm_amd import foam as F
+    t = time.perf_counter()
+    L = F.LduMatrix(*G.ldu_mesh(m, m, m, sigma=1.0, seed=5001))
+    drv = F.FoamPCG(L, precision="fp64")
+    log(f"foam {m}^3: {L.n} cells, {L.n_faces} faces, CSR and plan in {time.perf_counter() - t:.1f}s")
+    psi = np.zeros(L.n)
+    legs = []
+    for k in range(steps):
+        Lk = F.LduMatrix(*G.ldu_mesh(m, m, m, sigma=1.0, seed=5001 + k))
+        t = time.perf_counter()
+        if k:
+            drv.update(Lk)
+        upd = time.perf_counter() - t
+        A, _ = F.ldu_to_csr(Lk)
+        b = host_spmv(A, G.uniform(5100 + k, L.n))   # a new field to reach every step
+        psi0 = psi.copy()
+        psi, perf = drv.solve(b, psi0, tolerance=tol)
+        nf, r = O.foam_residual(A, b, psi0, psi)
+        legs.append({"step": k, "droptol": perf.droptol, "refresh_s": upd, "setup_s": perf.setup_seconds,
+                     "fsai_s": perf.fsai_seconds, "solve_s": perf.solve_seconds, "iterations": perf.n_iterations,
+                     "initial_residual": perf.initial_residual, "final_residual": perf.final_residual,
+                     "oracle_final_residual": r, "converged": perf.converged})
+        log(f"foam step {k}: droptol {perf.droptol:.4g}, refresh {upd * 1e3:.1f} ms, set-up "
+            f"{perf.setup_seconds:.2f} s ({perf.fsai_seconds:.2f} s approximate inverse), solve {perf.solve_seconds * 1e3:.1f} ms, {perf.n_iterations} it, "
+            f"residual {perf.initial_residual:.2e} -> {perf.final_residual:.2e} (oracle {r:.2e})")
+    return {"mesh": f"{m}^3", "cells": L.n, "faces": L.n_faces, "tolerance": tol, "steps": legs}
+
+
 def batching_legs(I, G, O, S):
     """The other workload configurations, timed on the device with inputs resident: the 128^3
     finite-volume operator (FP64 SpMV and FSAI-PCG to 1e-8), its 3 and 8 right-hand sides as
@@ -173,7 +215,7 @@ def batching_legs(I, G, O, S):
     log(f"generated 128^3 finite-volume: n={A.n_rows} nnz={A.nnz} in {time.perf_counter() - t:.1f}s")
     out["cfg2_spmv_fp64"] = spmv_leg(I, G, A, I.F64, I.F64, 20, 3)
     M = I.fastainv_sym(A, I.F64, max_elts=64 * A.n_rows)
-    b = O.csr_spmv(A, G.uniform(1, A.n_rows))
+    b = host_spmv(A, G.uniform(1, A.n_rows))
     sol, _ = solve_leg(I, S, O, A, b, M, "fp64", S.PCG, 3, 1, 1e-8)
     out["cfg2_pcg_fp64"] = sol
     log(f"cfg2 pcg: {sol['seconds'] * 1e3:.1f} ms, {sol['iterations']} it")
@@ -186,7 +228,7 @@ def batching_legs(I, G, O, S):
     for k in (3, 8):
         Bm = np.zeros((k, n))
         for j in range(k):
-            Bm[j, :A.n_rows] = O.csr_spmv(A, G.uniform(300 + j, A.n_rows))
+            Bm[j, :A.n_rows] = host_spmv(A, G.uniform(300 + j, A.n_rows))
         opts = S.SolverOpts(S.PCG, S.PREC_FP64, 1e-8, 100000, 16, 0, 0.1)
         bs = C.c_void_p()
         I.check(I.lib().ispm_block_solver_create(C.byref(bs), Ap.p, Mp[0].p, Mp[1].p, k, C.byref(opts)))
@@ -228,7 +270,7 @@ def batching_legs(I, G, O, S):
     t = time.perf_counter()
     systems = [G.rgg_laplacian(int(m), int(kk), seed=5000 + i, dtype=np.float32)
                for i, (m, kk) in enumerate(zip(sizes, ks))]
-    rhs = [O.csr_spmv(Ai, G.uniform(6000 + i, Ai.n_rows)).astype(np.float32) for i, Ai in enumerate(systems)]
+    rhs = [host_spmv(Ai, G.uniform(6000 + i, Ai.n_rows)).astype(np.float32) for i, Ai in enumerate(systems)]
     log(f"generated 512 systems ({sum(a.n_rows for a in systems)} rows) in {time.perf_counter() - t:.1f}s")
     bt = B.Batch(systems)
     bd = I.DeviceArray.from_numpy(bt.pack(rhs))
@@ -278,6 +320,9 @@ def main():
     ap.add_argument("--batching", action="store_true",
                     help="time the 128^3 finite-volume, block-RHS and 512-system batched configurations "
                          "instead (one JSON line of its own)")
+    ap.add_argument("--foam", action="store_true",
+                    help="time the foam-extend call pattern (lduMatrix input, device coefficient refresh, "
+                         "normalised-residual solves, drop tolerance tuned across time steps) on a 128^3 mesh")
     ap.add_argument("--calibrate", action="store_true",
                     help="first copy 1 GiB device-to-device 3 times (a plain copy of known size, "
                          "the reference for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -299,6 +344,9 @@ def main():
         I.synchronize()
         del src, dst
         log("calibration: 3 device-to-device copies of 1 GiB")
+    if args.foam:
+        print(json.dumps({"metric": "foam-extend call pattern", "legs": foam_legs(I, G, O)}))
+        return
     if args.batching:
         from ispm_amd import solver as S
         print(json.dumps({"metric": "batching configurations", "legs": batching_legs(I, G, O, S)}))
@@ -325,7 +373,7 @@ def main():
     factors = I.fastainv_sym(A, I.F64, max_elts=64 * A.n_rows, droptol=args.droptol)
     setup_s = time.perf_counter() - t
     log(f"fastainv_sym droptol={args.droptol}: nnz(Ainv1)/n = {factors[0].nnz / A.n_rows:.2f} in {setup_s:.1f}s")
-    b = O.csr_spmv(A, G.uniform(1, A.n_rows))
+    b = host_spmv(A, G.uniform(1, A.n_rows))
     solves = {}
     steps = max(1, min(args.steps, 5))
     warm = max(1, min(args.warmup, 1))

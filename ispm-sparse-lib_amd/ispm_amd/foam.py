@@ -106,7 +106,8 @@ class DroptolTuner:
 
 @dataclass
 class SolverPerformance:
-    """What an OpenFOAM solver reports for one solve, plus the set-up figures."""
+    """What an OpenFOAM solver reports for one solve, plus the set-up figures (setup_seconds
+    includes fsai_seconds, the host computation of the approximate inverse)."""
     initial_residual: float
     final_residual: float
     n_iterations: int
@@ -114,6 +115,7 @@ class SolverPerformance:
     droptol: float
     setup_seconds: float
     solve_seconds: float
+    fsai_seconds: float = 0.0
 
 
 class FoamPCG:
@@ -176,6 +178,7 @@ class FoamPCG:
         t0 = time.perf_counter()
         ft = F32 if self.mixed else F64
         factors = fastainv_sym(self.csr, ft, max_elts=self.max_elts_per_row * self.n, droptol=dt)
+        tf = time.perf_counter()
         solver = Solver(self.csr, M=factors, precision=self.precision, method=self.method,
                         A_plan=self.A_plan, maxit=self.maxit)
         if self.res_plan is not None:
@@ -192,5 +195,5 @@ class FoamPCG:
             self.tuner.note_time(t2 - t0)
         self.steps += 1
         perf = SolverPerformance(st.initial_residual, st.final_residual, st.iterations, bool(st.converged),
-                                 dt, t1 - t0, t2 - t1)
+                                 dt, t1 - t0, t2 - t1, tf - t0)
         return x.numpy(self.n), perf
