@@ -7,7 +7,8 @@
  *    block and then the eight recurrences with their three sums (sblas.cu:177-217), plus
  *    (r,r) for the convergence test.
  *  - FP64 PCG (Hestenes-Stiefel): z = M r, rho = (r,z), p = z + beta p, q = A p, alpha,
- *    x += alpha p, r -= alpha q.
+ *    x += alpha p, r -= alpha q; x += alpha p runs in the next iteration's p update, which
+ *    reads p anyway, and once more when the iterations stop.
  *  - single-reduction PCG (Chronopoulos-Gear): z = M r with (r,z), w = A z with (z,w), one
  *    finishing kernel (alpha, beta by the reference's formulas; the convergence test on the
  *    (r,r) of the previous update) and one fused update p = z + beta p, s = w + beta s,
@@ -116,11 +117,12 @@ __global__ void __launch_bounds__(kFinThreads) k_fin(const double *parts, int G,
   }
 }
 
-/* x += alpha p; r -= alpha q; (r,r); done when (r,r) <= tol2 */
+/* r -= alpha q; (r,r) partials.  The matching x += alpha p is deferred to the next k_px,
+ * which reads p anyway (one vector fewer per iteration), or to k_xflush when the loop stops;
+ * x gets the same values in the same order. */
 template<typename T>
 __global__ void __launch_bounds__(256)
-k_xr(T *__restrict__ x, T *__restrict__ r, const T *__restrict__ p, const T *__restrict__ q, int n,
-     const scalars *s, double *parts)
+k_r(T *__restrict__ r, const T *__restrict__ q, int n, const scalars *s, double *parts)
 {
   __shared__ double lds[4];
   if (s->done) return;
@@ -129,18 +131,15 @@ k_xr(T *__restrict__ x, T *__restrict__ r, const T *__restrict__ p, const T *__r
   double v[1] = {0.0};
   const int gt = blockIdx.x * 256 + threadIdx.x, gs = gridDim.x * 256, nv = n / L;
   for (int i = gt; i < nv; i += gs) {
-    pack16<T> xx = ld16(x, i), rr = ld16(r, i), pp = ld16(p, i), qq = ld16(q, i);
+    pack16<T> rr = ld16(r, i), qq = ld16(q, i);
 #pragma unroll
     for (int l = 0; l < L; l++) {
-      xx.v[l] = xx.v[l] + a * pp.v[l];
       rr.v[l] = rr.v[l] - a * qq.v[l];
       v[0] += (double)rr.v[l] * (double)rr.v[l];
     }
-    st16(x, i, xx);
     st16(r, i, rr);
   }
   for (int i = nv * L + gt; i < n; i += gs) {
-    x[i] = x[i] + a * p[i];
     T ri = r[i] - a * q[i];
     r[i] = ri;
     v[0] += (double)ri * (double)ri;
@@ -197,9 +196,9 @@ k_fin_cg(const double *pg, unsigned Gg, const double *pd, unsigned Gd, const dou
 }
 
 /* Hestenes-Stiefel PCG with a preconditioner: one finishing kernel for the preconditioner
- * product's (r, z) partials and the (r, r) partials k_xr left one iteration earlier.  When
+ * product's (r, z) partials and the (r, r) partials k_r left one iteration earlier.  When
  * that residual meets the target the solve is done (the iteration that produced it is
- * counted, as a finish right after k_xr would have) and nothing more of the batch runs;
+ * counted, as a finish right after k_r would have) and nothing more of the batch runs;
  * otherwise beta = (r, z) / rho and rho = (r, z). */
 __global__ void __launch_bounds__(kFinThreads)
 k_fin_rz_rr(const double *pg, unsigned Gg, const double *pr, unsigned Gr, scalars *s)
@@ -268,21 +267,39 @@ k_cg_update(const T *__restrict__ z, const T *__restrict__ w, T *__restrict__ p,
   store_partials<1>(v, parts, lds);
 }
 
-/* p = z + beta p */
+/* x += alpha p (the previous iteration's update: alpha is still that iteration's, p not yet
+ * overwritten); p = z + beta p */
 template<typename T>
-__global__ void __launch_bounds__(256) k_p(T *__restrict__ p, const T *__restrict__ z, int n, const scalars *s)
+__global__ void __launch_bounds__(256)
+k_px(T *__restrict__ x, T *__restrict__ p, const T *__restrict__ z, int n, const scalars *s)
 {
   if (s->done) return;
   constexpr int L = vtraits<T>::L;
-  const T b = (T)s->beta;
+  const T a = (T)s->alpha, b = (T)s->beta;
   const int gt = blockIdx.x * 256 + threadIdx.x, gs = gridDim.x * 256, nv = n / L;
   for (int i = gt; i < nv; i += gs) {
-    pack16<T> pp = ld16(p, i), zz = ld16(z, i);
+    pack16<T> xx = ld16(x, i), pp = ld16(p, i), zz = ld16(z, i);
 #pragma unroll
-    for (int l = 0; l < L; l++) pp.v[l] = zz.v[l] + b * pp.v[l];
+    for (int l = 0; l < L; l++) {
+      xx.v[l] = xx.v[l] + a * pp.v[l];
+      pp.v[l] = zz.v[l] + b * pp.v[l];
+    }
+    st16(x, i, xx);
     st16(p, i, pp);
   }
-  for (int i = nv * L + gt; i < n; i += gs) p[i] = z[i] + b * p[i];
+  for (int i = nv * L + gt; i < n; i += gs) {
+    x[i] = x[i] + a * p[i];
+    p[i] = z[i] + b * p[i];
+  }
+}
+
+/* x += alpha p: the update k_px would have made, when the iterations stop */
+template<typename T>
+__global__ void __launch_bounds__(256) k_xflush(T *__restrict__ x, const T *__restrict__ p, int n, const scalars *s)
+{
+  const T a = (T)s->alpha;
+  const int gt = blockIdx.x * 256 + threadIdx.x, gs = gridDim.x * 256;
+  for (int i = gt; i < n; i += gs) x[i] = x[i] + a * p[i];
 }
 
 template<typename T>
@@ -619,8 +636,10 @@ int iterate(ispm_solver *S)
     if ((st = apply_M(S, S->r, S->z, sk))) return st;
     if ((st = dot(S, OP_RZ, S->r, S->z, 1, S->T))) return st;
   }
-  if (S->T == ISPM_F64) hipLaunchKernelGGL(k_p<double>, g, b, 0, sm, (double *)S->p, (const double *)S->z, S->n, S->s);
-  else hipLaunchKernelGGL(k_p<float>, g, b, 0, sm, (float *)S->p, (const float *)S->z, S->n, S->s);
+  if (S->T == ISPM_F64)
+    hipLaunchKernelGGL(k_px<double>, g, b, 0, sm, (double *)S->xs, (double *)S->p, (const double *)S->z, S->n, S->s);
+  else
+    hipLaunchKernelGGL(k_px<float>, g, b, 0, sm, (float *)S->xs, (float *)S->p, (const float *)S->z, S->n, S->s);
   ISPM_HIP(hipGetLastError());
   /* q = A p with (p, q) fused where possible */
   st = S->T == ISPM_F64 ? spmv_dot<double>(S, S->A, S->p, S->q, S->p, sk, G)
@@ -628,11 +647,11 @@ int iterate(ispm_solver *S)
   if (st) return st;
   finish<OP_PQ>(S, 1, G);
   if (S->T == ISPM_F64)
-    hipLaunchKernelGGL(k_xr<double>, g, b, 0, sm, (double *)S->xs, (double *)S->r, (const double *)S->p,
-                       (const double *)S->q, S->n, S->s, S->M1 ? pr : S->parts);
+    hipLaunchKernelGGL(k_r<double>, g, b, 0, sm, (double *)S->r, (const double *)S->q, S->n, S->s,
+                       S->M1 ? pr : S->parts);
   else
-    hipLaunchKernelGGL(k_xr<float>, g, b, 0, sm, (float *)S->xs, (float *)S->r, (const float *)S->p,
-                       (const float *)S->q, S->n, S->s, S->M1 ? pr : S->parts);
+    hipLaunchKernelGGL(k_r<float>, g, b, 0, sm, (float *)S->r, (const float *)S->q, S->n, S->s,
+                       S->M1 ? pr : S->parts);
   if (!S->M1) finish<OP_XR>(S, 1);
   ISPM_HIP(hipGetLastError());
   return ISPM_OK;
@@ -856,6 +875,7 @@ int solve_core(ispm_solver *S, const double *b, double *x, measure &mc, ispm_sol
       h.rho = INFINITY;
       h.gamma_old = INFINITY;
       h.alpha_old = 1.0;
+      h.alpha = 0.0;   /* no pending x update (p = 0) */
       h.rr_valid = 0;
       h.done = 0;
       h.tol2 = tol2;
@@ -884,6 +904,7 @@ int solve_core(ispm_solver *S, const double *b, double *x, measure &mc, ispm_sol
     h.rho = INFINITY;
     h.gamma_old = INFINITY;
     h.alpha_old = 1.0;
+    h.alpha = 0.0;
     h.rr_valid = 0;
     if ((e = zero(S->p, vb))) return e;
     if (S->single && (e = zero(S->sv, vb))) return e;
@@ -925,6 +946,14 @@ int solve_core(ispm_solver *S, const double *b, double *x, measure &mc, ispm_sol
     res.l1_resnorm = h.l1;
     const bool out_of_iters = (int)h.iters >= S->o.maxit;
     if (!h.done && !out_of_iters) continue;
+    if (!S->pipelined && !S->single) {
+      /* the x update k_px defers to the next iteration */
+      if (S->T == ISPM_F64)
+        hipLaunchKernelGGL(k_xflush<double>, g, bl, 0, sm, (double *)S->xs, (const double *)S->p, n, S->s);
+      else
+        hipLaunchKernelGGL(k_xflush<float>, g, bl, 0, sm, (float *)S->xs, (const float *)S->p, n, S->s);
+      ISPM_HIP(hipGetLastError());
+    }
     if (S->mixed) {
       hipLaunchKernelGGL(k_x64, g, bl, 0, sm, x, (float *)S->xs, n, S->s);
       ISPM_HIP(hipGetLastError());
@@ -954,6 +983,7 @@ int solve_core(ispm_solver *S, const double *b, double *x, measure &mc, ispm_sol
       rec_target = std::max(S->o.reliable_delta, margin * abs2 / rmax);
       h.rho = h.rho * c * c;
       h.gamma_old = h.gamma_old * c * c;
+      h.alpha = 0.0;   /* the pending x update was applied above: the next k_px adds nothing */
       h.rr_valid = 0;
       h.done = 0;
       h.tol2 = rec_target * rec_target;
