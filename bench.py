@@ -221,8 +221,8 @@ def batching_legs(I, G, O, S):
     log(f"cfg2 pcg: {sol['seconds'] * 1e3:.1f} ms, {sol['iterations']} it")
     # cfg4a: block PCG (SpMM over k columns, the two factors as SlELL plans) vs k single solves
     Ap = I.Plan(A, I.F64, elm_type=I.F64)
-    Mp = (I.Plan(M[0], I.F64, elm_type=I.F64, kind=I.PLAN_ESTIMATE),
-          I.Plan(M[1], I.F64, elm_type=I.F64, kind=I.PLAN_ESTIMATE))
+    Mp = (I.Plan(M[0], I.F64, elm_type=I.F64, kind=I.PLAN_MIN_BYTES),
+          I.Plan(M[1], I.F64, elm_type=I.F64, kind=I.PLAN_MIN_BYTES))
     n = Ap.n_rows_padded
     single = S.Solver(A, M_plans=Mp, A_plan=Ap, precision="fp64", method=S.PCG, tol=1e-8)
     for k in (3, 8):

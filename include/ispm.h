@@ -4,8 +4,10 @@
  * boundary.  The C++ drop-in headers (formats/formats.h, plan/plan.h, fastainv/fastainv.h,
  * util/cuda/sblas.h, util/cuda/initialize.h under ispm-sparse-lib_amd/) are thin inline
  * wrappers over these functions, so a caller built against the reference's headers
- * recompiles unchanged against ours.  Each declaration names the reference interface it
- * replaces (paths relative to the reference tree).
+ * recompiles against ours, provided it calls no CUDA runtime function directly and does not
+ * register existing host memory (cuda_pinned_region, unsupported on this target).  Each
+ * declaration names the reference interface it replaces (paths relative to the reference
+ * tree).
  *
  * Conventions
  *  - Return value: ISPM_OK (0) or an ISPM_ERR_* code; ispm_last_error() gives a message.
@@ -198,7 +200,12 @@ void ispm_slell_device_free(ispm_slell_desc *d);
 
 /* ------------------------------------------------------------- SpMV plan --------------- */
 
-enum ispm_plan_kind { ISPM_PLAN_DEFAULT = 0, ISPM_PLAN_ESTIMATE = 1, ISPM_PLAN_EXHAUSTIVE = 2 };
+/* ISPM_PLAN_DEFAULT and ISPM_PLAN_ESTIMATE (the same kind, as in plan/plan.h:74-75): the
+ * guessed format (conversion.h:416-492); ISPM_PLAN_EXHAUSTIVE: plan/plan.h:97-144;
+ * ISPM_PLAN_MIN_BYTES (no reference counterpart): the format with the fewest stored bytes,
+ * as ispm_plan_estimate chooses it. */
+enum ispm_plan_kind { ISPM_PLAN_DEFAULT = 0, ISPM_PLAN_ESTIMATE = 1, ISPM_PLAN_EXHAUSTIVE = 2,
+                      ISPM_PLAN_MIN_BYTES = 3 };
 /* ISPM_SPMV_PLAN_SPLIT_DIAG (no reference counterpart): store a square matrix as its main
  * diagonal (a dense vector) plus the rows that have off-diagonal entries (a compressed
  * SlELL layout of those rows and a row map, streamed mapping); y = diag .* x, then those
@@ -240,8 +247,8 @@ int ispm_plan_info_get(const ispm_plan *p, ispm_plan_info *info);
  * workgroups of max(64, min(1024, S)) threads; one 64-lane wave per slice for
  * variable-height slices). */
 int ispm_plan_default_launch(unsigned slice_height, int var_height, ispm_launch *out);
-/* PLAN_ESTIMATE (in the reference an alias of PLAN_DEFAULT, plan/plan.h:74-96): the format
- * with the fewest stored bytes (exact for plain ELL at every slice height, fixed and
+/* No reference counterpart (ISPM_PLAN_MIN_BYTES; the reference's PLAN_ESTIMATE is an alias of
+ * PLAN_DEFAULT, plan/plan.h:74-96): the format with the fewest stored bytes (exact for plain ELL at every slice height, fixed and
  * variable; the diagonal/hblock variant suggested by the sampled statistics converted once
  * and compared), with the default launch shape. */
 int ispm_plan_estimate(int vec_type, int elm_type, const ispm_csr *A, unsigned *S,

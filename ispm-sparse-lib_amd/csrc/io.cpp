@@ -7,6 +7,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 #include "ispm.h"
@@ -14,10 +16,8 @@
 namespace ispm { int set_error_message(int code, const char *msg); }
 using ispm::set_error_message;
 
-extern "C" int ispm_mm_read(const char *path, ispm_csr_out *out)
+static int mm_read(const char *path, ispm_csr_out *out)
 {
-  if (!path || !out) return set_error_message(ISPM_ERR_INVALID, "mm_read: null argument");
-  std::memset(out, 0, sizeof *out);
   FILE *f = std::fopen(path, "r");
   if (!f) return set_error_message(ISPM_ERR_INVALID, "mm_read: cannot open file");
   char line[1024];
@@ -38,6 +38,10 @@ extern "C" int ispm_mm_read(const char *path, ispm_csr_out *out)
     if (std::sscanf(line, "%lld %lld %lld", &nr, &nc, &ne) == 3) break;
   }
   if (nr <= 0 || nc <= 0 || nr > 0x7fffffff || nc > 0x7fffffff) { std::fclose(f); return set_error_message(ISPM_ERR_INVALID, "mm_read: bad size line"); }
+  /* entries are stored with 32-bit row pointers: more than INT_MAX of them (after the
+   * mirror entries of a symmetric file) cannot be represented */
+  if (ne < 0) { std::fclose(f); return set_error_message(ISPM_ERR_INVALID, "mm_read: negative entry count"); }
+  if ((symmetric ? 2 * ne : ne) > 0x7fffffffLL) { std::fclose(f); return set_error_message(ISPM_ERR_UNSUPPORTED, "mm_read: more than 2^31-1 entries"); }
   std::vector<long long> key;
   std::vector<double> val;
   key.reserve(symmetric ? 2 * ne : ne);
@@ -80,6 +84,28 @@ extern "C" int ispm_mm_read(const char *path, ispm_csr_out *out)
   std::copy(ci.begin(), ci.end(), out->cols);
   std::copy(v.begin(), v.end(), (double *)out->elms);
   return ISPM_OK;
+}
+
+extern "C" int ispm_mm_read(const char *path, ispm_csr_out *out)
+{
+  if (!path || !out) return set_error_message(ISPM_ERR_INVALID, "mm_read: null argument");
+  std::memset(out, 0, sizeof *out);
+  int st;
+  try {
+    st = mm_read(path, out);
+  } catch (const std::bad_alloc &) {
+    st = set_error_message(ISPM_ERR_NOMEM, "mm_read: out of memory");
+  } catch (const std::exception &) {
+    st = set_error_message(ISPM_ERR_INVALID, "mm_read: cannot represent the file's matrix");
+  }
+  if (st != ISPM_OK) {
+    /* nothing half-built reaches the caller */
+    std::free(out->row_ptr);
+    std::free(out->cols);
+    std::free(out->elms);
+    std::memset(out, 0, sizeof *out);
+  }
+  return st;
 }
 
 extern "C" int ispm_mm_write(const char *path, const ispm_csr *A)

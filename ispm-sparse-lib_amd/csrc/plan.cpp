@@ -1,7 +1,9 @@
 /* plan.cpp -- C-ABI for host format conversion, device upload and SpMV plans.
  *
  * ispm_plan mirrors spmv_plan<T,E> (plan/plan.h:20-161 of the reference):
- *  - DEFAULT / ESTIMATE: the format comes from the guess heuristic (conversion.h:416-492);
+ *  - MIN_BYTES (no reference counterpart): the format with the fewest stored bytes.
+ *  - DEFAULT / ESTIMATE (one kind, plan.h:74-75): the format comes from the guess heuristic
+ *    (conversion.h:416-492);
  *    the launch shape is chosen for CDNA4 instead of the reference's fixed 512 threads:
  *    one row per thread and one wave (64 lanes) per 64 rows, so that every plane load of
  *    a wave is a single 512-byte (FP64) run and no cross-wave reduction is needed.
@@ -786,6 +788,8 @@ int ispm_plan_create(ispm_plan **out, int vt, int et, const ispm_csr *A, int kin
   if (et < 0) et = A->elm_type;
   if (!((vt == ISPM_F64 && (et == ISPM_F64 || et == ISPM_F32)) || (vt == ISPM_F32 && (et == ISPM_F32 || et == ISPM_BF16))))
     return fail_plain(ISPM_ERR_UNSUPPORTED, "plan_create: unsupported (T,E) pair");
+  if (kind < ISPM_PLAN_DEFAULT || kind > ISPM_PLAN_MIN_BYTES)
+    return fail_plain(ISPM_ERR_INVALID, "plan_create: unknown plan kind");
   ispm_plan *p = new ispm_plan;
   p->T = vt;
   p->E = et;
@@ -819,7 +823,7 @@ int ispm_plan_create(ispm_plan **out, int vt, int et, const ispm_csr *A, int kin
     st = make_layout(A, et, S, W, V, D, p->L);
     if (st) { delete p; return st; }
     p->launch = lp;
-  } else if (kind == ISPM_PLAN_ESTIMATE) {
+  } else if (kind == ISPM_PLAN_MIN_BYTES) {
     unsigned S = 64, W = 1;
     int V = 0, D = 0;
     st = estimate_format(A, vt, et, S, W, V, D);

@@ -120,7 +120,9 @@ int ispm_device_string(char *buf, int len)
   ISPM_HIP(hipGetDevice(&dev));
   hipDeviceProp_t p;
   ISPM_HIP(hipGetDeviceProperties(&p, dev));
-  snprintf(buf, len, "%04x:%02x:%02x (%s)", p.pciDomainID, p.pciBusID, p.pciDeviceID, p.name);
+  /* the marketing name can be empty on this part: fall back to the ISA name */
+  const char *name = p.name[0] ? p.name : p.gcnArchName;
+  snprintf(buf, len, "%04x:%02x:%02x (%s)", p.pciDomainID, p.pciBusID, p.pciDeviceID, name);
   return ISPM_OK;
 }
 

@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(PKG_DIR, "libispm0.so")
 
 F32, F64, BF16 = 0, 1, 2
 H2D, D2H, D2D, DEFAULT = 0, 1, 2, 3
-PLAN_DEFAULT, PLAN_ESTIMATE, PLAN_EXHAUSTIVE = 0, 1, 2
+PLAN_DEFAULT, PLAN_ESTIMATE, PLAN_EXHAUSTIVE, PLAN_MIN_BYTES = 0, 1, 2, 3
 SPMV_NO_TEXTURE_MEMORY, SPMV_PLAN_NO_DEVICE, SPMV_PLAN_SPLIT_DIAG, SPMV_PLAN_SYMMETRIC = 1, 2, 4, 8
 
 _NP = {F32: np.float32, F64: np.float64, BF16: np.uint16}
@@ -26,6 +26,7 @@ _TSIZE = {F32: 4, F64: 8, BF16: 2}
 
 ERRORS = {0: "OK", 1: "INVALID", 2: "UNSUPPORTED", 3: "RUNTIME", 4: "NOMEM", 5: "DEVICE",
           6: "NOT_CONVERGED"}
+(ERR_INVALID, ERR_UNSUPPORTED, ERR_RUNTIME, ERR_NOMEM, ERR_DEVICE, ERR_NOT_CONVERGED) = range(1, 7)
 
 
 class IspmError(RuntimeError):

@@ -5,7 +5,7 @@ import ctypes as C
 
 import numpy as np
 
-from . import CSR, F64, PLAN_ESTIMATE, DeviceArray, Plan, check, lib
+from . import CSR, F64, PLAN_MIN_BYTES, DeviceArray, Plan, check, lib
 from .solver import PCG, PREC_FP64, SolveStats, SolverOpts
 
 
@@ -17,7 +17,7 @@ def block_solve(A: CSR, Bm, M=None, tol=1e-10, maxit=100000, check_every=16):
     Ap = Plan(A, F64, elm_type=F64)
     Mp = None
     if M is not None:
-        Mp = (Plan(M[0], F64, elm_type=F64, kind=PLAN_ESTIMATE), Plan(M[1], F64, elm_type=F64, kind=PLAN_ESTIMATE))
+        Mp = (Plan(M[0], F64, elm_type=F64, kind=PLAN_MIN_BYTES), Plan(M[1], F64, elm_type=F64, kind=PLAN_MIN_BYTES))
     n = Ap.n_rows_padded
     opts = SolverOpts(PCG, PREC_FP64, tol, maxit, check_every, 0, 0.1)
     s = C.c_void_p()

@@ -6,7 +6,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import (BF16, CSR, F32, F64, PLAN_ESTIMATE, SPMV_PLAN_SPLIT_DIAG, SPMV_PLAN_SYMMETRIC,
+from . import (BF16, CSR, F32, F64, PLAN_MIN_BYTES, SPMV_PLAN_SPLIT_DIAG, SPMV_PLAN_SYMMETRIC,
                DeviceArray, IspmError, Plan, check, csr_multiply, lib)
 
 PPCG, PCG, PCG1 = 0, 1, 2
@@ -139,7 +139,7 @@ class Solver:
             def plans(mats, split=False):
                 if split:
                     return tuple(Plan(m, vt, elm_type=ft, flags=SPMV_PLAN_SPLIT_DIAG) for m in mats)
-                return tuple(Plan(m, vt, elm_type=ft, kind=PLAN_ESTIMATE) for m in mats)
+                return tuple(Plan(m, vt, elm_type=ft, kind=PLAN_MIN_BYTES) for m in mats)
             n = self.A_plan.n_rows_padded
             if fsai_product is None and not isinstance(M, CSR):
                 # two factors or their product, each as SlELL or split: whichever applies

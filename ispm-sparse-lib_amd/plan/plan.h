@@ -1,7 +1,8 @@
 /* plan.h -- SpMV plan (plan/plan.h:1-161 of the reference), same members and methods.
- * PLAN_DEFAULT takes the guessed format with a CDNA4 launch shape; PLAN_ESTIMATE the format
- * with the fewest stored bytes (ispm_plan_estimate);
- * PLAN_EXHAUSTIVE times every format and launch shape on the device (ispm_plan_tune). */
+ * PLAN_DEFAULT and PLAN_ESTIMATE (one case, as in plan.h:74-75 of the reference) take the
+ * guessed format with a CDNA4 launch shape; PLAN_EXHAUSTIVE times every format and launch
+ * shape on the device (ispm_plan_tune).  The fewest-stored-bytes planner is reached through
+ * the C-ABI only (ISPM_PLAN_MIN_BYTES, ispm_plan_estimate). */
 #ifndef ISPM_PLAN_H
 #define ISPM_PLAN_H
 
@@ -87,15 +88,8 @@ spmv_plan<T, E>::spmv_plan(const csr_matrix<E> &m, spmv_plan_kind kind, unsigned
     mparams = slell_params{S, W, (unsigned)V, (unsigned)D};
     host_matrix = new slell_matrix<E, host_memory_space_tag>(csr_to_slell(m, S, W, V != 0, D != 0, ispm_detail::plan_min_height(S, V)));
     upload();
-  } else if (kind == PLAN_ESTIMATE) {
-    ispm_csr c = ispm_detail::csr_view(m);
-    unsigned S, W;
-    int V, D;
-    ISPM_CHECK(ispm_plan_estimate(ispm_type_code<T>::value, ispm_type_code<E>::value, &c, &S, &W, &V, &D, &lp));
-    mparams = slell_params{S, W, (unsigned)V, (unsigned)D};
-    host_matrix = new slell_matrix<E, host_memory_space_tag>(csr_to_slell(m, S, W, V != 0, D != 0, ispm_detail::plan_min_height(S, V)));
-    if (up) upload();
   } else {
+    /* PLAN_DEFAULT, PLAN_ESTIMATE */
     host_matrix = new slell_matrix<E, host_memory_space_tag>(csr_to_slell_guess_format(m));
     if (up) upload();
     mparams = slell_params{host_matrix->slice_height, host_matrix->hblock,

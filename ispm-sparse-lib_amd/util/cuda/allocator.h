@@ -56,4 +56,8 @@ inline bool operator==(const ispm_pinned_allocator<A> &, const ispm_pinned_alloc
 template<typename A, typename B>
 inline bool operator!=(const ispm_pinned_allocator<A> &, const ispm_pinned_allocator<B> &) { return false; }
 
+/* the reference's names for the two allocators (util/cuda/allocator.h:8,45) */
+template<typename T> using cuda_device_allocator = ispm_device_allocator<T>;
+template<typename T> using cuda_pinned_allocator = ispm_pinned_allocator<T>;
+
 #endif

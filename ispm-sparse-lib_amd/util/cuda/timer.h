@@ -35,4 +35,9 @@ struct ispm_timer
   }
 };
 
+/* the reference's names (util/cuda/timer.h:6,33); events always record on the library
+ * stream */
+typedef ispm_event_scoped cudaEventScoped;
+typedef ispm_timer cudatimer;
+
 #endif

@@ -59,8 +59,16 @@ int main()
   for (int r = 0; r < n; r++)
     for (int k = A.row_ptr[r]; k < A.row_ptr[r + 1]; k++) yref[r] += A.elms[k] * x[A.cols[k]];
 
-  for (int kind = 0; kind <= 2; kind += 2) {
+  for (int kind = 0; kind <= 2; kind++) {
     spmv_plan<double> plan(A, (spmv_plan_kind)kind);
+    if (kind == PLAN_ESTIMATE) {
+      /* PLAN_ESTIMATE is PLAN_DEFAULT (plan/plan.h:74-75 of the reference) */
+      spmv_plan<double> def(A, PLAN_DEFAULT);
+      REQUIRE(plan.mparams.S == def.mparams.S && plan.mparams.H == def.mparams.H &&
+              plan.mparams.V == def.mparams.V && plan.mparams.D == def.mparams.D &&
+              plan.params.N == def.params.N && plan.params.S == def.params.S &&
+              plan.params.H == def.params.H);
+    }
     REQUIRE(A.n_rows % 512 == 0);     /* the plan pads the caller's matrix */
     vector<double, host_memory_space_tag> hx(A.n_rows, 0.0);
     for (int i = 0; i < n; i++) hx[i] = x[i];
@@ -99,8 +107,9 @@ int main()
     std::printf("ok coefficient refresh err=%.1e\n", err);
   }
   {
-    /* short irregular rows: the estimated plan takes slice height 1 (below the conversion's
-     * default minimum of 4), also when the coefficients are refreshed */
+    /* short irregular rows: the exhaustive plan may take a slice height below the
+     * conversion's default minimum of 4 (the library times heights from 1); the plan and its
+     * coefficient refresh convert with that height */
     csr_matrix<double> R;
     const int nr = 3000;
     R.n_rows = R.n_cols = nr;
@@ -120,7 +129,7 @@ int main()
     for (int i = 0; i < nr; i++) rx[i] = std::cos(0.11 * i);
     for (int r = 0; r < nr; r++)
       for (int k = R.row_ptr[r]; k < R.row_ptr[r + 1]; k++) ry[r] += R.elms[k] * rx[R.cols[k]];
-    spmv_plan<double> plan(R, PLAN_ESTIMATE);
+    spmv_plan<double> plan(R, PLAN_EXHAUSTIVE);
     for (int pass = 0; pass < 2; pass++) {
       if (pass) {
         csr_matrix<double> R2 = R;
@@ -137,7 +146,15 @@ int main()
       for (int i = 0; i < nr; i++) err = std::fmax(err, std::fabs(hy[i] - (pass ? 3.0 : 1.0) * ry[i]));
       REQUIRE(err < 1e-12);
     }
-    std::printf("ok estimated plan S=%u with refresh\n", plan.mparams.S);
+    std::printf("ok exhaustive plan S=%u with refresh\n", plan.mparams.S);
+    /* the reference's timer (util/cuda/timer.h:33), as plan.h:121 uses it */
+    vector<double, device_memory_space_tag> dx(R.n_rows), dy(R.n_rows);
+    cudatimer t;
+    t.start();
+    plan.execute_spmv(dx, dy);
+    t.stop();
+    REQUIRE(t.elapsed_seconds() > 0 && t.elapsed_seconds() < 1.0);
+    std::printf("ok cudatimer %.3g ms\n", 1e3 * t.elapsed_seconds());
   }
   {
     csr_matrix<double> B = poisson(6);
@@ -159,12 +176,20 @@ int main()
     for (int i = 0; i < m; i++) ref += a[i] * b[i];
     REQUIRE(std::fabs(hs[0] - ref) < 1e-12 * std::fabs(ref));
     /* result written through a mapped pinned scalar */
-    std::vector<double, ispm_pinned_allocator<double> > pinned(1, 0.0);
+    std::vector<double, cuda_pinned_allocator<double> > pinned(1, 0.0);
     pinmemptr<double> ps(pinned.data());
     sblas::dot(da.data(), db.data(), ps(), m);
     ispm_synchronize();
     REQUIRE(std::fabs(*ps - ref) < 1e-12 * std::fabs(ref));
     std::printf("ok sblas dot %.6f\n", hs[0]);
+  }
+  {
+    /* the reference's device allocator (util/cuda/allocator.h:8): zero-filled memory */
+    std::vector<double, cuda_device_allocator<double> > dv(257);
+    std::vector<double> back(dv.size(), 1.0);
+    copy(back.data(), reinterpret_cast<const devmem<double> *>(dv.data()), dv.size());
+    for (double v : back) REQUIRE(v == 0.0);
+    std::printf("ok cuda_device_allocator\n");
   }
   {
     DroptolTuner t;

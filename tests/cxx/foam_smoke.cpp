@@ -111,8 +111,8 @@ static std::vector<double> step(ispm_plan *A, const ispm_csr &csr, int n_pad, co
   CHECK(ispm_fastainv_sym(ISPM_F64, &csr, nullptr, nullptr, 64LL * L.n, 0.125, &a1, &a2));
   const ispm_csr v1 = view(a1), v2 = view(a2);
   ispm_plan *M1 = nullptr, *M2 = nullptr;
-  CHECK(ispm_plan_create(&M1, ISPM_F64, ISPM_F64, &v1, ISPM_PLAN_ESTIMATE, 0));
-  CHECK(ispm_plan_create(&M2, ISPM_F64, ISPM_F64, &v2, ISPM_PLAN_ESTIMATE, 0));
+  CHECK(ispm_plan_create(&M1, ISPM_F64, ISPM_F64, &v1, ISPM_PLAN_MIN_BYTES, 0));
+  CHECK(ispm_plan_create(&M2, ISPM_F64, ISPM_F64, &v2, ISPM_PLAN_MIN_BYTES, 0));
   ispm_solver_opts o;
   o.method = ISPM_SOLVER_PPCG; o.precision = ISPM_PREC_FP64; o.tol = 1e-10; o.maxit = 2000;
   o.check_every = 16; o.use_graph = 1; o.reliable_delta = 0.01;

@@ -69,3 +69,46 @@ def test_symmetric_plan_argument_checks():
     E_ = I.CSR(2, 2, [0, 2, 4], [0, 1, 0, 1], [2.0, 1.0, 1.0, 2.0])   # symmetric, but no device
     with pytest.raises(I.IspmError):
         I.Plan(E_, I.F64, flags=I.SPMV_PLAN_SYMMETRIC | I.SPMV_PLAN_NO_DEVICE)
+
+
+def _compile(tmp_path, body):
+    import subprocess
+    src = tmp_path / "tu.cpp"
+    src.write_text(body)
+    pkg = I.PKG_DIR
+    return subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", pkg, "-I", os.path.join(ROOT, "include"),
+                           str(src)], capture_output=True, text=True)
+
+
+def test_reference_names_compile(tmp_path):
+    """A translation unit naming the reference's allocators, timer, plan kinds and pinned
+    helpers (util/cuda/allocator.h:8,45, timer.h:6,33, plan/plan.h:12-16, pinner.h:7-48)
+    compiles against the drop-in headers."""
+    r = _compile(tmp_path, """
+#include <vector>
+#include "plan/plan.h"
+#include "util/cuda/allocator.h"
+#include "util/cuda/pinner.h"
+#include "util/cuda/timer.h"
+void f(const csr_matrix<double> &m) {
+  std::vector<double, cuda_device_allocator<double> > d(4);
+  std::vector<double, cuda_pinned_allocator<double> > h(4);
+  cudatimer t; t.start(); t.stop(); (void)t.elapsed_seconds();
+  cudaEventScoped e; e.record(); e.await();
+  spmv_plan<double> p(m, PLAN_ESTIMATE);
+  pinmemptr<double> pm(h.data()); (void)pm();
+  cuda_pinned_region<double> unused;
+}
+""")
+    assert r.returncode == 0, r.stderr
+
+
+def test_pinned_region_registration_is_a_compile_error(tmp_path):
+    """Registering existing host memory is not available on this target: constructing a
+    cuda_pinned_region from a pointer fails at compile time with a message saying so."""
+    r = _compile(tmp_path, """
+#include "util/cuda/pinner.h"
+void f(double *p) { cuda_pinned_region<double> r(p, 4); }
+""")
+    assert r.returncode != 0
+    assert "registering existing host memory is not supported" in r.stderr

@@ -67,3 +67,32 @@ def test_plan_cache_store_lookup(tmp_path):
     assert (S.value, W.value, V.value, D.value, out.N, out.S, out.H) == (128, 2, 1, 0, 512, 128, 1)
     assert L.ispm_plan_cache_lookup(path, 0x1234, I.F64, I.F32, C.byref(S), C.byref(W), C.byref(V),
                                     C.byref(D), C.byref(out), C.byref(sec)) != 0
+
+
+@pytest.mark.parametrize("size_line, code", [
+    ("3 3 -1", "INVALID"),                 # negative entry count
+    ("3 3 3000000000", "UNSUPPORTED"),     # beyond 32-bit row pointers
+])
+def test_mm_rejects_bad_entry_counts(tmp_path, size_line, code):
+    q = tmp_path / "count.mtx"
+    q.write_text(f"%%MatrixMarket matrix coordinate real general\n{size_line}\n1 1 1.0\n")
+    with pytest.raises(I.IspmError) as e:
+        I.mm_read(q)
+    assert e.value.code == getattr(I, "ERR_" + code)
+
+
+def test_mm_symmetric_count_doubles_before_the_limit(tmp_path):
+    # 1.5e9 stored entries fit 32 bits, their mirrors do not: refused before reading any
+    q = tmp_path / "sym.mtx"
+    q.write_text("%%MatrixMarket matrix coordinate real symmetric\n3 3 1500000000\n1 1 1.0\n")
+    with pytest.raises(I.IspmError) as e:
+        I.mm_read(q)
+    assert e.value.code == I.ERR_UNSUPPORTED
+
+
+def test_mm_truncated_file_is_an_error(tmp_path):
+    q = tmp_path / "short.mtx"
+    q.write_text("%%MatrixMarket matrix coordinate real general\n3 3 4\n1 1 1.0\n2 2 1.0\n")
+    with pytest.raises(I.IspmError) as e:
+        I.mm_read(q)
+    assert e.value.code == I.ERR_INVALID
