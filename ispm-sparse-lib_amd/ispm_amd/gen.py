@@ -59,33 +59,60 @@ def normal(seed: int, n: int) -> np.ndarray:
     return z[:n]
 
 
-def _stencil_csr(nx, ny, nz, offsets, val_fn, dtype=np.float64):
-    """CSR of a structured-grid stencil.  offsets: list of (dx,dy,dz); val_fn(mask, rows,
-    cols, off_index) -> values for the valid entries of one offset.  Rows are sorted by
-    column because offsets are visited in increasing linear offset."""
+def _stencil_csr(nx, ny, nz, offsets, val_fn, dtype=np.float64, diag=None, chunk=1 << 15):
+    """CSR of a structured-grid stencil.  offsets: list of (dx,dy,dz); val_fn(mask, rows, off)
+    -> values of one offset for the given rows (one per row; only the valid entries, `mask`,
+    are kept).  Offsets are visited in increasing linear offset, so rows come out sorted by
+    column.  diag: an array that val_fn accumulates into for its rows; when given, it becomes
+    the (0,0,0) entries once every offset of those rows has been visited.  Rows are built in
+    independent chunks on several threads; each row's offsets are still visited in order, so
+    the result does not depend on the chunking."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
     n = nx * ny * nz
     lin = [dz * nx * ny + dy * nx + dx for dx, dy, dz in offsets]
     order = np.argsort(lin, kind="stable")
-    idx = np.arange(n, dtype=np.int64)
-    x = idx % nx
-    y = (idx // nx) % ny
-    z = idx // (nx * ny)
-    K = len(offsets)
-    cols = np.empty((n, K), np.int32)
-    vals = np.zeros((n, K), dtype)
-    mask = np.empty((n, K), bool)
-    for j, o in enumerate(order):
-        dx, dy, dz = offsets[o]
-        m = (x + dx >= 0) & (x + dx < nx) & (y + dy >= 0) & (y + dy < ny) & (z + dz >= 0) & (z + dz < nz)
-        mask[:, j] = m
-        cols[:, j] = (idx + lin[o]).astype(np.int32)
-        vals[:, j] = val_fn(m, idx, (dx, dy, dz))
-    counts = mask.sum(1)
+    offs = [offsets[o] for o in order]
+    lins = [lin[o] for o in order]
+    K = len(offs)
+    center = offs.index((0, 0, 0)) if diag is not None else None
+
+    def masks(r0, r1):
+        rows = np.arange(r0, r1, dtype=np.int64)
+        x, y, z = rows % nx, (rows // nx) % ny, rows // (nx * ny)
+        mask = np.empty((r1 - r0, K), bool)
+        for j, (dx, dy, dz) in enumerate(offs):
+            mask[:, j] = (x + dx >= 0) & (x + dx < nx) & (y + dy >= 0) & (y + dy < ny) & (z + dz >= 0) & (z + dz < nz)
+        return rows, mask
+
+    bounds = [(r0, min(n, r0 + chunk)) for r0 in range(0, n, chunk)]
     row_ptr = np.zeros(n + 1, np.int64)
-    np.cumsum(counts, out=row_ptr[1:])
-    c = cols[mask]
-    v = vals[mask]
-    del cols, vals, mask
+    threads = max(1, min(16, os.cpu_count() or 1))
+
+    def count(b):
+        _, mask = masks(*b)
+        row_ptr[b[0] + 1:b[1] + 1] = mask.sum(1)
+
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(count, bounds))
+    np.cumsum(row_ptr, out=row_ptr)
+    c = np.empty(int(row_ptr[-1]), np.int32)
+    v = np.empty(int(row_ptr[-1]), dtype)
+
+    def fill(b):
+        rows, mask = masks(*b)
+        cols = np.empty(mask.shape, np.int32)
+        vals = np.empty(mask.shape, dtype)
+        for j, (o, ln) in enumerate(zip(offs, lins)):
+            cols[:, j] = rows + ln
+            vals[:, j] = val_fn(mask[:, j], rows, o)
+        if center is not None:
+            vals[:, center] = diag[rows]
+        c[row_ptr[b[0]]:row_ptr[b[1]]] = cols[mask]
+        v[row_ptr[b[0]]:row_ptr[b[1]]] = vals[mask]
+
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(fill, bounds))
     return CSR(n, n, row_ptr.astype(np.int32), c, v)
 
 
@@ -110,23 +137,18 @@ def stencil27_contrast(m: int = 192, block: int = 8, contrast: float = 1e4, seed
     offs = [(dx, dy, dz) for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1)]
     diag = np.zeros(m ** 3)
 
-    def val(mask, idx_, o):
+    def val(mask, rows, o):
         if o == (0, 0, 0):
-            return np.zeros(idx_.size)
+            return np.zeros(rows.size)
         lin = o[2] * m * m + o[1] * m + o[0]
-        nbr = np.clip(idx_ + lin, 0, m ** 3 - 1)
-        kn = np.where(mask, kappa[nbr], kappa)        # off-grid neighbour: Dirichlet, kappa_i
-        c = 0.5 * (kappa + kn)
-        diag[:] += c
+        nbr = np.clip(rows + lin, 0, m ** 3 - 1)
+        ki = kappa[rows]
+        kn = np.where(mask, kappa[nbr], ki)           # off-grid neighbour: Dirichlet, kappa_i
+        c = 0.5 * (ki + kn)
+        diag[rows] += c
         return np.where(mask, -c, 0.0)
 
-    A = _stencil_csr(m, m, m, offs, val)
-    # write the diagonal (one entry per row, the 14th offset in sorted order when interior)
-    rp = A.row_ptr.astype(np.int64)
-    rows = np.repeat(np.arange(A.n_rows), np.diff(rp))
-    is_diag = A.cols == rows
-    A.vals[is_diag] = diag
-    return A
+    return _stencil_csr(m, m, m, offs, val, diag=diag)
 
 
 def stencil(nx: int, ny: int, nz: int, points: int = 7, anisotropy: float = 1.0,
@@ -155,22 +177,20 @@ def stencil(nx: int, ny: int, nz: int, points: int = 7, anisotropy: float = 1.0,
     diag = np.full(n, float(shift))
     cls = {1: 1.0, 2: 0.5, 3: 0.25}
 
-    def val(mask, idx_, o):
+    def val(mask, rows, o):
         if o == (0, 0, 0):
-            return np.zeros(idx_.size)
+            return np.zeros(rows.size)
         lin = o[2] * nx * ny + o[1] * nx + o[0]
-        nbr = np.clip(idx_ + lin, 0, n - 1)
-        kn = np.where(mask, kappa[nbr], kappa)
+        nbr = np.clip(rows + lin, 0, n - 1)
+        ki = kappa[rows]
+        kn = np.where(mask, kappa[nbr], ki)
         # harmonic mean from the symmetric product and sum, so that w(i, j) == w(j, i) exactly
-        hm = 2 * (kappa * kn) / (kappa + kn)
+        hm = 2 * (ki * kn) / (ki + kn)
         w = cls[abs(o[0]) + abs(o[1]) + abs(o[2])] * anisotropy ** abs(o[2]) * hm
-        diag[:] += w
+        diag[rows] += w
         return np.where(mask, -w, 0.0)
 
-    A = _stencil_csr(nx, ny, nz, offs, val)
-    rows = np.repeat(np.arange(A.n_rows), np.diff(A.row_ptr.astype(np.int64)))
-    A.vals[A.cols == rows] = diag
-    return A
+    return _stencil_csr(nx, ny, nz, offs, val, diag=diag)
 
 
 def fv7_lognormal(m: int = 128, sigma: float = 1.5, seed: int = 2001, permute_seed: int = 2002,
