@@ -28,15 +28,30 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def time_spmv(I, plan, x, y, steps, warmup):
+class CacheFlush:
+    """Evicts the 256 MB Infinity Cache between timed launches: a device-to-device copy of
+    512 MiB (1 GiB of traffic) on the library stream, outside the timed events."""
+
+    def __init__(self, I, nbytes=512 << 20):
+        self.I, self.n = I, nbytes
+        self.src, self.dst = I.DeviceArray(nbytes // 8), I.DeviceArray(nbytes // 8)
+
+    def __call__(self):
+        self.I.check(self.I.lib().ispm_memcpy_async(self.dst.vp, self.src.vp, self.n, 2))   # ISPM_D2D
+
+
+def time_spmv(I, plan, x, y, steps, warmup, flush=None):
     """Average launch duration of the plan's SpMV kernel, timed with HIP events recorded on
-    the library stream (the stream the kernel runs on)."""
+    the library stream (the stream the kernel runs on); with `flush`, the cache is flushed
+    before every timed launch (for matrices that fit in the Infinity Cache)."""
     for _ in range(warmup):
         plan.execute(x, y)
     I.synchronize()
     e0, e1 = I.Event(), I.Event()
     ts = []
     for _ in range(steps):
+        if flush:
+            flush()
         e0.record()
         plan.execute(x, y)
         e1.record()
@@ -45,14 +60,14 @@ def time_spmv(I, plan, x, y, steps, warmup):
     return float(np.mean(ts)), float(np.median(ts))
 
 
-def spmv_leg(I, G, A, vt, et, steps, warmup, flags=0):
+def spmv_leg(I, G, A, vt, et, steps, warmup, flags=0, flush=None):
     plan = I.Plan(A, vt, elm_type=et, flags=flags)
     inf = plan.info()
     n = plan.n_rows_padded
     dt = np.float64 if vt == I.F64 else np.float32
     x = I.DeviceArray.from_numpy(np.concatenate([G.uniform(0, A.n_rows), np.zeros(n - A.n_rows)]).astype(dt))
     y = I.DeviceArray(n, dt)
-    mean, med = time_spmv(I, plan, x, y, steps, warmup)
+    mean, med = time_spmv(I, plan, x, y, steps, warmup, flush)
     stored = inf.stored_bytes
     return {
         "format": dict(S=inf.S, hblock=inf.hblock, V=inf.V, D=inf.D,
@@ -63,6 +78,7 @@ def spmv_leg(I, G, A, vt, et, steps, warmup, flags=0):
         "GBps": stored / mean * 1e-9, "csr_equiv_GBps": inf.csr_bytes / mean * 1e-9,
         "GFLOPs": 2.0 * A.nnz / mean * 1e-9,
         "frac_hbm": stored / mean * 1e-9 / HBM_PEAK_GBS,
+        "cache_flushed": flush is not None,
     }
 
 
@@ -144,26 +160,37 @@ def solve_leg(I, S, O, A, b, factors, precision, method, steps, warmup, tol):
             "oracle_relres": rel}, x
 
 
-def cpu_solve_baseline(O, A, b, factors, iters_needed, seconds=15.0):
-    """FP64 pipelined PCG of the oracle (host CSR, all cores) on the same system and
-    preconditioner: a bounded number of iterations is timed and scaled to the iteration count
-    the GPU FP64 solve needed."""
+def cpu_solve_baseline(O, A, b, tol, max_reps=5, budget_s=45.0, droptol=0.125):
+    """The CPU path on the host cores: the oracle's FP64 PCG (Hestenes-Stiefel, host CSR
+    products over all cores, the method of the fastest GPU FP64 solve) from x = 0 to the same
+    FP64 relative residual with the same preconditioner, timed to convergence, median of up
+    to `max_reps` solves within `budget_s`; and the restated fastainv_sym (one core: its
+    column loop is sequential), timed once, whose factors the solves use."""
+    import platform
+    t = time.perf_counter()
+    f = O.fastainv_sym(A, out_double=True, max_elts=64 * A.n_rows, droptol=droptol)
+    fsai_s = time.perf_counter() - t
+    M = [type(A)(A.n_rows, A.n_cols, r, c, v) for r, c, v in f]
     threads = O.set_threads(0)
+    ts, it, rel = [], 0, None
     t0 = time.perf_counter()
-    k = 2
-    while True:
-        t1 = time.perf_counter()
-        O.pcg(A, b, M1=factors[0], M2=factors[1], tol=1e-300, maxit=k, pipelined=True)
-        el = time.perf_counter() - t1
-        if el > seconds / 3 or time.perf_counter() - t0 > seconds:
-            break
-        k *= 2
-    per_iter = el / k
-    return {"value": per_iter * iters_needed, "unit": "s (FP64 FSAI-PCG solve to 1e-10, extrapolated)",
+    while len(ts) < max_reps and (not ts or time.perf_counter() - t0 + ts[-1] <= budget_s):
+        t = time.perf_counter()
+        _, it, rel = O.pcg(A, b, M1=M[0], M2=M[1], tol=tol, maxit=100000)
+        ts.append(time.perf_counter() - t)
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            cpu = next(l.split(":", 1)[1].strip() for l in fh if l.startswith("model name"))
+    except (OSError, StopIteration):
+        cpu = platform.processor() or cpu
+    return {"value": float(np.median(ts)), "unit": "s (FP64 FSAI-PCG solve to 1e-10 on the host)",
             "cores": threads, "kind": "port",
-            "sample": f"{k} iterations of the oracle's FP64 pipelined PCG on the full matrix "
-                      f"({per_iter * 1e3:.1f} ms/iteration) x {iters_needed} iterations",
-            "seconds_per_iteration": per_iter}
+            "sample": f"{len(ts)} complete solves of the full matrix from x = 0 (the oracle's FP64 "
+                      f"Hestenes-Stiefel PCG, {it} iterations, true relres {rel:.2e}), median; "
+                      f"the oracle's fastainv_sym on 1 core: {fsai_s:.2f} s",
+            "cpu_model": cpu, "solve_seconds": ts, "iterations": it, "relres": rel,
+            "fastainv_seconds_1core": fsai_s}
 
 
 def foam_legs(I, G, O, m=128, steps=4, tol=1e-8):
@@ -213,7 +240,7 @@ def batching_legs(I, G, O, S):
     t = time.perf_counter()
     A = G.fv7_lognormal(128)
     log(f"generated 128^3 finite-volume: n={A.n_rows} nnz={A.nnz} in {time.perf_counter() - t:.1f}s")
-    out["cfg2_spmv_fp64"] = spmv_leg(I, G, A, I.F64, I.F64, 20, 3)
+    out["cfg2_spmv_fp64"] = spmv_leg(I, G, A, I.F64, I.F64, 20, 3, flush=CacheFlush(I))
     M = I.fastainv_sym(A, I.F64, max_elts=64 * A.n_rows)
     b = host_spmv(A, G.uniform(1, A.n_rows))
     sol, _ = solve_leg(I, S, O, A, b, M, "fp64", S.PCG, 3, 1, 1e-8)
@@ -375,8 +402,7 @@ def main():
     log(f"fastainv_sym droptol={args.droptol}: nnz(Ainv1)/n = {factors[0].nnz / A.n_rows:.2f} in {setup_s:.1f}s")
     b = host_spmv(A, G.uniform(1, A.n_rows))
     solves = {}
-    steps = max(1, min(args.steps, 5))
-    warm = max(1, min(args.warmup, 1))
+    steps, warm = args.steps, args.warmup
     for name, prec, meth in (("fp64_ppcg", "fp64", S.PPCG), ("fp64_pcg", "fp64", S.PCG),
                              ("fp64_pcg1", "fp64", S.PCG1), ("mixed_pcg", "mixed", S.PCG),
                              ("mixed_pcg1", "mixed", S.PCG1)):
@@ -390,8 +416,10 @@ def main():
     mixed = min((v for k, v in solves.items() if k.startswith("mixed")), key=lambda v: v["seconds"])
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_solve_baseline(O, A, b, factors, solves["fp64_ppcg"]["iterations"])
+        cpu = cpu_solve_baseline(O, A, b, tol, droptol=args.droptol)
         cpu["spmv"] = cpu_spmv_baseline(O, A, seconds=5.0)
+        log(f"cpu: solve {cpu['value']:.2f} s ({cpu['iterations']} it, {cpu['cores']} threads), "
+            f"fastainv {cpu['fastainv_seconds_1core']:.2f} s")
     out = {
         "metric": "FSAI-PCG time to 1e-10 residual, mixed vs FP64 on 1 GPU; SpMV HBM GB/s",
         "value": mixed["seconds"],
