@@ -79,13 +79,27 @@ int fill_values(const conv::layout &L, const ispm_csr *A, int et, void *elms)
   return 0;
 }
 
+/* The conversions assume what the reference's callers provide (conversion.h): row_ptr starts
+ * at 0 and never decreases, columns lie in [0, n_cols) and are sorted within each row
+ * (repeated columns allowed).  The diagonal encoder of an unsorted row would place entries
+ * past the slice's planes, so every conversion checks this first (O(nnz)). */
 static int check_csr(const ispm_csr *A)
 {
-  if (!A || !A->row_ptr || (!A->cols && A->row_ptr[A->n_rows] > 0) || (!A->elms && A->row_ptr[A->n_rows] > 0))
-    return fail_plain(ISPM_ERR_INVALID, "csr: null array");
+  if (!A || !A->row_ptr) return fail_plain(ISPM_ERR_INVALID, "csr: null array");
   if (A->n_rows <= 0 || A->n_cols <= 0) return fail_plain(ISPM_ERR_INVALID, "csr: empty matrix");
+  if ((!A->cols && A->row_ptr[A->n_rows] > 0) || (!A->elms && A->row_ptr[A->n_rows] > 0))
+    return fail_plain(ISPM_ERR_INVALID, "csr: null array");
   if (A->elm_type != ISPM_F64 && A->elm_type != ISPM_F32 && A->elm_type != ISPM_BF16)
     return fail_plain(ISPM_ERR_INVALID, "csr: bad element type");
+  const int *rp = A->row_ptr, *ci = A->cols;
+  if (rp[0] != 0) return fail_plain(ISPM_ERR_INVALID, "csr: row_ptr[0] != 0");
+  for (int r = 0; r < A->n_rows; r++) {
+    if (rp[r + 1] < rp[r]) return fail_plain(ISPM_ERR_INVALID, "csr: row_ptr decreases");
+    for (int k = rp[r]; k < rp[r + 1]; k++) {
+      if (ci[k] < 0 || ci[k] >= A->n_cols) return fail_plain(ISPM_ERR_INVALID, "csr: column out of range");
+      if (k > rp[r] && ci[k] < ci[k - 1]) return fail_plain(ISPM_ERR_INVALID, "csr: columns of a row not sorted");
+    }
+  }
   return ISPM_OK;
 }
 

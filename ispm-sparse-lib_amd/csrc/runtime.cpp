@@ -120,8 +120,12 @@ int ispm_device_string(char *buf, int len)
   ISPM_HIP(hipGetDevice(&dev));
   hipDeviceProp_t p;
   ISPM_HIP(hipGetDeviceProperties(&p, dev));
-  /* the marketing name can be empty on this part: fall back to the ISA name */
-  const char *name = p.name[0] ? p.name : p.gcnArchName;
+  /* the marketing name can be empty on this part: fall back to the ISA name without its
+   * feature suffixes (":sramecc+:xnack-"), so that the string fits the reference's 32 bytes */
+  char arch[sizeof p.gcnArchName];
+  std::snprintf(arch, sizeof arch, "%s", p.gcnArchName);
+  if (char *c = std::strchr(arch, ':')) *c = 0;
+  const char *name = p.name[0] ? p.name : arch;
   snprintf(buf, len, "%04x:%02x:%02x (%s)", p.pciDomainID, p.pciBusID, p.pciDeviceID, name);
   return ISPM_OK;
 }

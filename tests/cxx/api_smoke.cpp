@@ -2,6 +2,7 @@
  * formats/formats.h, fastainv/fastainv.h, util/cuda/sblas.h, util/cuda/initialize.h),
  * compiled with g++ against the drop-in headers and linked to libispm0.so.  Prints
  * "ok" lines; any failure aborts with a message.  Run by tests/test_gpu_cxx_api.py. */
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -117,9 +118,16 @@ int main()
     unsigned long long s = 12345;
     auto next = [&s]() { s = s * 6364136223846793005ULL + 1442695040888963407ULL; return (unsigned)(s >> 33); };
     for (int r = 0; r < nr; r++) {
+      /* 1..3 distinct columns, sorted as CSR rows must be */
       const int len = 1 + (int)(next() % 3);
-      for (int k = 0; k < len; k++) {
-        R.cols.push_back((int)(next() % nr));
+      std::vector<int> c;
+      while ((int)c.size() < len) {
+        const int v = (int)(next() % nr);
+        if (std::find(c.begin(), c.end(), v) == c.end()) c.push_back(v);
+      }
+      std::sort(c.begin(), c.end());
+      for (int v : c) {
+        R.cols.push_back(v);
         R.elms.push_back(1.0 + (next() % 7));
       }
       R.row_ptr.push_back((int)R.cols.size());
