@@ -144,6 +144,20 @@ int ispm_ppcg_update_vectors(int type, void *resnorm, void *gamma, void *delta,
                              const void *n, const void *m,
                              void *p, void *s, void *q, void *z, void *x, void *r,
                              void *u, void *w, int n_elts);
+/* No counterpart in the reference: ispm_ppcg_update_vectors for callers that cannot allocate
+ * device memory (code under hipGraph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, kernels and results (bit-identical); the partial sums of its
+ * reductions live in a static device array of 8192 x 4 doubles (256 KiB per GPU, held until
+ * the process ends) instead of the library's allocated scratch.  Every n_elts in
+ * [0, 2^31-1] fits (at most 8192 workgroups with grid-stride loops); n_elts < 0 is rejected.
+ * Requires ispm_init() on the current device beforehand (ISPM_ERR_INVALID otherwise), and
+ * calls no allocating routine.  All calls on one GPU share that array: calls that overlap
+ * (on different streams, or from several threads) race. */
+int ispm_ppcg_update_vectors_noalloc(int type, void *resnorm, void *gamma, void *delta,
+                                     const void *alpha, const void *beta,
+                                     const void *n, const void *m,
+                                     void *p, void *s, void *q, void *z, void *x, void *r,
+                                     void *u, void *w, int n_elts);
 
 /* ------------------------------------------------------ host formats (CPU) ------------ */
 

@@ -29,6 +29,16 @@ hipStream_t stream();
 /* Make sure the device is initialised (gfx950 check); returns ISPM_OK or an error. */
 int ensure_init();
 
+/* Whether the current device has been initialised successfully (allocates nothing). */
+bool initialized();
+
+/* Devices the library keeps state for. */
+constexpr int kMaxDevices = 16;
+
+/* Take the device address of the no-allocation entry point's static scratch on device dev
+ * (current); called once per device during initialisation. */
+int noalloc_scratch_init(int dev);
+
 /* Per-device reduction scratch: block partial sums of the two-phase reductions of the
  * sblas entry points.  Allocated once per device at initialisation. */
 struct reduce_scratch {

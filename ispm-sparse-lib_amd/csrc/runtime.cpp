@@ -40,12 +40,12 @@ struct device_state {
   int cus = 0;
 };
 std::mutex g_mu;
-device_state g_dev[16];
+device_state g_dev[kMaxDevices];
 
 device_state &cur()
 {
   int d = 0;
-  if (hipGetDevice(&d) != hipSuccess || d < 0 || d >= 16) d = 0;
+  if (hipGetDevice(&d) != hipSuccess || d < 0 || d >= kMaxDevices) d = 0;
   return g_dev[d];
 }
 
@@ -69,6 +69,8 @@ int init_locked(device_state &s)
   if (e != hipSuccess) return s.init_status = hip_status(e, "hipStreamCreate");
   e = hipMalloc((void **)&s.red.partials, sizeof(double) * kMaxBlocks * kMaxSums);
   if (e != hipSuccess) return s.init_status = hip_status(e, "hipMalloc(scratch)");
+  int st = noalloc_scratch_init(dev);
+  if (st) return s.init_status = st;
   e = hipDeviceSynchronize();
   if (e != hipSuccess) return s.init_status = hip_status(e, "hipDeviceSynchronize");
   return s.init_status = ISPM_OK;
@@ -90,6 +92,12 @@ hipStream_t stream()
 }
 
 reduce_scratch &scratch() { return cur().red; }
+
+bool initialized()
+{
+  const device_state &s = cur();
+  return s.ready && s.init_status == ISPM_OK;
+}
 
 int num_cus() { return cur().cus; }
 

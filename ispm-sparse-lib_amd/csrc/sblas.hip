@@ -97,11 +97,10 @@ ppcg_update_finish(const double *parts, int G, T *resnorm, T *gamma, T *delta)
 }
 
 template<typename T>
-void launch_ppcg_update(bool vec, int grid, T *resnorm, T *gamma, T *delta, const T *alpha,
-                        const T *beta, const T *n, const T *m, T *p, T *s, T *q, T *z, T *x,
-                        T *r, T *u, T *w, int n_elts)
+void launch_ppcg_update(bool vec, int grid, double *parts, T *resnorm, T *gamma, T *delta,
+                        const T *alpha, const T *beta, const T *n, const T *m, T *p, T *s, T *q,
+                        T *z, T *x, T *r, T *u, T *w, int n_elts)
 {
-  double *parts = scratch().partials;
   if (vec) hipLaunchKernelGGL((ppcg_update_kernel<T, true>), dim3(grid), dim3(256), 0, stream(), alpha, beta, n, m, p, s, q, z, x, r, u, w, n_elts, parts);
   else     hipLaunchKernelGGL((ppcg_update_kernel<T, false>), dim3(grid), dim3(256), 0, stream(), alpha, beta, n, m, p, s, q, z, x, r, u, w, n_elts, parts);
   hipLaunchKernelGGL(ppcg_update_finish<T>, dim3(1), dim3(kFinThreads), 0, stream(), (const double *)parts, grid, resnorm, gamma, delta);
@@ -109,6 +108,28 @@ void launch_ppcg_update(bool vec, int grid, T *resnorm, T *gamma, T *delta, cons
 
 }  // namespace
 
+}  // namespace ispm
+
+/* Scratch of ispm_ppcg_update_vectors_noalloc: the per-workgroup partials of its four sums
+ * (parts[s*G + block], G <= kMaxBlocks workgroups) in a file-scope device array instead of
+ * the library's allocated scratch, so that the call allocates nothing.  Every GPU holds its
+ * own copy until the process ends.  In host code the array's name is not a device address:
+ * the address is taken once per device with hipGetSymbolAddress at initialisation and kept in
+ * g_noalloc_parts. */
+static constexpr int kNoAllocSums = 4;
+static __device__ __align__(256) double noalloc_partials[ispm::kMaxBlocks * kNoAllocSums];
+static double *g_noalloc_parts[ispm::kMaxDevices];
+
+namespace ispm {
+int noalloc_scratch_init(int dev)
+{
+  if (dev < 0 || dev >= kMaxDevices) return fail(ISPM_ERR_DEVICE, "noalloc scratch: device %d", dev);
+  void *p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_partials));
+  if (e != hipSuccess) return hip_status(e, "hipGetSymbolAddress(noalloc_partials)");
+  g_noalloc_parts[dev] = static_cast<double *>(p);
+  return ISPM_OK;
+}
 }  // namespace ispm
 
 using namespace ispm;
@@ -226,16 +247,62 @@ extern "C" int ispm_ppcg_update_vectors(int type, void *resnorm, void *gamma, vo
   const bool vec = aligned16(n) && aligned16(m) && aligned16(p) && aligned16(s) && aligned16(q) &&
                    aligned16(z) && aligned16(x) && aligned16(r) && aligned16(u) && aligned16(w);
   const int grid = grid_for(n_elts);
+  double *parts = scratch().partials;
   if (type == ISPM_F64) {
     typedef double T;
-    launch_ppcg_update<T>(vec, grid, (T *)resnorm, (T *)gamma, (T *)delta, (const T *)alpha, (const T *)beta,
+    launch_ppcg_update<T>(vec, grid, parts, (T *)resnorm, (T *)gamma, (T *)delta, (const T *)alpha, (const T *)beta,
                           (const T *)n, (const T *)m, (T *)p, (T *)s, (T *)q, (T *)z, (T *)x, (T *)r,
                           (T *)u, (T *)w, n_elts);
   } else {
     typedef float T;
-    launch_ppcg_update<T>(vec, grid, (T *)resnorm, (T *)gamma, (T *)delta, (const T *)alpha, (const T *)beta,
+    launch_ppcg_update<T>(vec, grid, parts, (T *)resnorm, (T *)gamma, (T *)delta, (const T *)alpha, (const T *)beta,
                           (const T *)n, (const T *)m, (T *)p, (T *)s, (T *)q, (T *)z, (T *)x, (T *)r,
                           (T *)u, (T *)w, n_elts);
+  }
+  ISPM_HIP(hipGetLastError());
+  return ISPM_OK;
+}
+
+extern "C" int ispm_ppcg_update_vectors_noalloc(int type, void *resnorm, void *gamma, void *delta,
+                                                const void *alpha, const void *beta,
+                                                const void *n, const void *m, void *p, void *s,
+                                                void *q, void *z, void *x, void *r, void *u,
+                                                void *w, int n_elts)
+{
+  /* no ensure_init(): initialising a device allocates; the caller has done it */
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return fail(ISPM_ERR_DEVICE, "ppcg_update_vectors_noalloc: no usable current device");
+  if (!initialized())
+    return fail(ISPM_ERR_INVALID, "ppcg_update_vectors_noalloc: the library is not initialised on "
+                                  "device %d; call ispm_init() there first (this entry point "
+                                  "allocates nothing)", dev);
+  double *parts = g_noalloc_parts[dev];
+  if (!parts) return fail(ISPM_ERR_RUNTIME, "ppcg_update_vectors_noalloc: no scratch address for device %d", dev);
+  if (!resnorm || !gamma || !delta || !alpha || !beta || !n || !m || !p || !s || !q || !z || !x || !r || !u || !w)
+    return fail(ISPM_ERR_INVALID, "ppcg_update_vectors_noalloc: null pointer");
+  if (n_elts < 0) return fail(ISPM_ERR_INVALID, "ppcg_update_vectors_noalloc: n < 0");
+  if (type != ISPM_F64 && type != ISPM_F32)
+    return fail(ISPM_ERR_UNSUPPORTED, "ppcg_update_vectors_noalloc: type %d", type);
+  const bool vec = aligned16(n) && aligned16(m) && aligned16(p) && aligned16(s) && aligned16(q) &&
+                   aligned16(z) && aligned16(x) && aligned16(r) && aligned16(u) && aligned16(w);
+  /* the same grid as ispm_ppcg_update_vectors, so both sum the partials in the same order;
+   * grid_for caps it at kMaxBlocks (grid-stride loops take any n_elts), which is what the
+   * array holds */
+  const int grid = grid_for(n_elts);
+  if (grid > kMaxBlocks)
+    return fail(ISPM_ERR_INVALID, "ppcg_update_vectors_noalloc: %d workgroups exceed the %d partials "
+                                  "of the static scratch", grid, kMaxBlocks);
+  if (type == ISPM_F64) {
+    typedef double T;
+    launch_ppcg_update<T>(vec, grid, parts, (T *)resnorm, (T *)gamma, (T *)delta, (const T *)alpha,
+                          (const T *)beta, (const T *)n, (const T *)m, (T *)p, (T *)s, (T *)q,
+                          (T *)z, (T *)x, (T *)r, (T *)u, (T *)w, n_elts);
+  } else {
+    typedef float T;
+    launch_ppcg_update<T>(vec, grid, parts, (T *)resnorm, (T *)gamma, (T *)delta, (const T *)alpha,
+                          (const T *)beta, (const T *)n, (const T *)m, (T *)p, (T *)s, (T *)q,
+                          (T *)z, (T *)x, (T *)r, (T *)u, (T *)w, n_elts);
   }
   ISPM_HIP(hipGetLastError());
   return ISPM_OK;

@@ -109,6 +109,7 @@ _SIGS = {
     "ispm_dot": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "ispm_ppcg_update_scalars": (C.c_int, [C.c_int] + [C.c_void_p] * 5),
     "ispm_ppcg_update_vectors": (C.c_int, [C.c_int] + [C.c_void_p] * 15 + [C.c_int]),
+    "ispm_ppcg_update_vectors_noalloc": (C.c_int, [C.c_int] + [C.c_void_p] * 15 + [C.c_int]),
     "ispm_csr_to_slell": (C.c_int, [C.POINTER(Csr), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_int, C.POINTER(SlellHost)]),
     "ispm_csr_to_slell_guess": (C.c_int, [C.POINTER(Csr), C.c_int, C.c_int, C.c_int,
