@@ -424,14 +424,17 @@ int ispm_block_solver_solve(ispm_block_solver *s, const double *B, size_t ldb, d
                             size_t ldx, ispm_solve_stats *stats);
 void ispm_block_solver_destroy(ispm_block_solver *s);
 
-/* Many independent small SPD systems (FP32) solved with Jacobi-preconditioned CG in ONE
- * persistent launch: workgroups pull whole systems from a device work queue (largest
- * first).  The systems are given as one block-diagonal CSR (global row/column numbering,
- * system i owns sys_rows[i] consecutive rows and may not couple to another system).  b and x
- * use the padded per-system layout reported by ispm_batch_layout (each system padded to a
- * multiple of 64 rows; padding entries of b must be 0).  iters / relres (host arrays of
- * n_sys, may be NULL) receive each system's iteration count and FP32 recursive relative
- * residual.  No reference counterpart. */
+/* Many independent small SPD systems (FP32) solved together with Jacobi-preconditioned
+ * single-reduction CG: every launch covers the segments (1024-row pieces) of all systems still
+ * iterating, iterations are replayed from a captured hipGraph, and each system stops on its
+ * own residual.  The systems are given as one block-diagonal CSR (global row/column
+ * numbering, system i owns sys_rows[i] consecutive rows, may not couple to another system and
+ * has rows of at most 255 entries); the library renumbers each system internally (reverse
+ * Cuthill-McKee).  b and x use the padded per-system layout reported by ispm_batch_layout
+ * (each system padded to a multiple of 64 rows; padding entries of b must be 0; x is written
+ * on return and need not be initialised).  iters / relres (host arrays of n_sys, may be NULL)
+ * receive each system's iteration count and FP32 recursive relative residual.  No reference
+ * counterpart. */
 typedef struct ispm_batch ispm_batch;
 int ispm_batch_create(ispm_batch **out, int n_sys, const int *sys_rows, const int *row_ptr,
                       const int *cols, const float *vals);
