@@ -40,12 +40,12 @@ class CacheFlush:
         self.I.check(self.I.lib().ispm_memcpy_async(self.dst.vp, self.src.vp, self.n, 2))   # ISPM_D2D
 
 
-def time_spmv(I, plan, x, y, steps, warmup, flush=None):
-    """Average launch duration of the plan's SpMV kernel, timed with HIP events recorded on
-    the library stream (the stream the kernel runs on); with `flush`, the cache is flushed
-    before every timed launch (for matrices that fit in the Infinity Cache)."""
+def time_spmv(I, run, steps, warmup, flush=None):
+    """Average launch duration of one SpMV (`run()` launches it), timed with HIP events
+    recorded on the library stream (the stream the kernel runs on); with `flush`, the cache is
+    flushed before every timed launch (for matrices that fit in the Infinity Cache)."""
     for _ in range(warmup):
-        plan.execute(x, y)
+        run()
     I.synchronize()
     e0, e1 = I.Event(), I.Event()
     ts = []
@@ -53,21 +53,27 @@ def time_spmv(I, plan, x, y, steps, warmup, flush=None):
         if flush:
             flush()
         e0.record()
-        plan.execute(x, y)
+        run()
         e1.record()
         e1.synchronize()
         ts.append(e0.elapsed_ms(e1) * 1e-3)
     return float(np.mean(ts)), float(np.median(ts))
 
 
-def spmv_leg(I, G, A, vt, et, steps, warmup, flags=0, flush=None):
+def spmv_leg(I, G, A, vt, et, steps, warmup, flags=0, flush=None, guarded=False):
+    """One plan's SpMV timed; `guarded`: x is held with the plan's zero guard zones and the
+    product runs through ispm_plan_execute_guarded, as the solver's own vectors are."""
     plan = I.Plan(A, vt, elm_type=et, flags=flags)
     inf = plan.info()
     n = plan.n_rows_padded
     dt = np.float64 if vt == I.F64 else np.float32
-    x = I.DeviceArray.from_numpy(np.concatenate([G.uniform(0, A.n_rows), np.zeros(n - A.n_rows)]).astype(dt))
+    g = plan.x_guard() if guarded else 0
+    xh = np.zeros(n + 2 * g, dt)
+    xh[g:g + A.n_rows] = G.uniform(0, A.n_rows)
+    x = I.DeviceArray.from_numpy(xh)
     y = I.DeviceArray(n, dt)
-    mean, med = time_spmv(I, plan, x, y, steps, warmup, flush)
+    run = (lambda: plan.execute_guarded(x, g, y)) if guarded else (lambda: plan.execute(x, y))
+    mean, med = time_spmv(I, run, steps, warmup, flush)
     stored = inf.stored_bytes
     return {
         "format": dict(S=inf.S, hblock=inf.hblock, V=inf.V, D=inf.D,
@@ -79,6 +85,7 @@ def spmv_leg(I, G, A, vt, et, steps, warmup, flags=0, flush=None):
         "GFLOPs": 2.0 * A.nnz / mean * 1e-9,
         "frac_hbm": stored / mean * 1e-9 / HBM_PEAK_GBS,
         "cache_flushed": flush is not None,
+        "x_guard": g,
     }
 
 
@@ -95,7 +102,8 @@ def roofline(leg, key, kernel):
             rec = json.load(f).get(key)
         same = rec and rec.get("stored_bytes") == leg["stored_bytes"] and all(
             rec["format"].get(k) == leg["format"][k] for k in ("S", "hblock", "V", "D")) and \
-            bool(rec["format"].get("symmetric", False)) == leg["format"]["symmetric"]
+            bool(rec["format"].get("symmetric", False)) == leg["format"]["symmetric"] and \
+            bool(rec.get("x_guarded", False)) == (leg["x_guard"] > 0)
         if same:
             traffic = rec["traffic_bytes"]
     return {"bound": "hbm", "kernel": kernel, "achieved": leg["GBps"], "peak": HBM_PEAK_GBS,
@@ -382,11 +390,14 @@ def main():
     A = G.stencil27_contrast(192)
     log(f"generated 27-point 192^3: n={A.n_rows} nnz={A.nnz} in {time.perf_counter() - t:.1f}s")
     legs = {}
-    for name, vt, et, fl in (("fp64", I.F64, I.F64, 0), ("fp32", I.F32, I.F32, 0),
-                             ("fp64_sym", I.F64, I.F64, I.SPMV_PLAN_SYMMETRIC),
-                             ("fp32_sym", I.F32, I.F32, I.SPMV_PLAN_SYMMETRIC)):
+    # fp32_sym: x with guard zones, as the solver holds it (the four-rows-per-lane kernel);
+    # fp32_sym_checked: the same plan through the bounds-checked kernel (x without guards)
+    for name, vt, et, fl, gd in (("fp64", I.F64, I.F64, 0, False), ("fp32", I.F32, I.F32, 0, False),
+                                 ("fp64_sym", I.F64, I.F64, I.SPMV_PLAN_SYMMETRIC, False),
+                                 ("fp32_sym", I.F32, I.F32, I.SPMV_PLAN_SYMMETRIC, True),
+                                 ("fp32_sym_checked", I.F32, I.F32, I.SPMV_PLAN_SYMMETRIC, False)):
         t = time.perf_counter()
-        legs[name] = spmv_leg(I, G, A, vt, et, max(args.steps, 10), args.warmup, fl)
+        legs[name] = spmv_leg(I, G, A, vt, et, max(args.steps, 10), args.warmup, fl, guarded=gd)
         log(f"spmv {name}: {legs[name]['GBps']:.0f} GB/s ({legs[name]['avg_s'] * 1e6:.1f} us) "
             f"[{time.perf_counter() - t:.1f}s]")
     if args.spmv_only:
@@ -442,8 +453,9 @@ def main():
         "spmv": legs,
         # the dominant kernel of the timed (mixed) solve: its A product
         "roofline": (roofline(legs["fp32_sym"], "fp32_sym",
-                              "symdia_kernel<float,float> (FP32 symmetric-storage 27-point 192^3 "
-                              "SpMV, the largest share of the mixed solve)")
+                              "symdia_r4_kernel<4,14> (FP32 symmetric-storage 27-point 192^3 "
+                              "SpMV on x with guard zones, four rows per lane; the largest "
+                              "share of the mixed solve)")
                      if mixed["A_storage"] != "SlELL" else
                      roofline(legs["fp32"], "fp32",
                               "slell_diagpat_kernel<float,float> (FP32 27-point 192^3 SpMV, the "

@@ -279,6 +279,13 @@ int ispm_plan_tune(int vec_type, int elm_type, const ispm_csr *A, unsigned *S, u
                    int *V, int *D, ispm_launch *launch, double *seconds);
 /* spmv_plan::execute_spmv (plan/plan.h:148-161). */
 int ispm_plan_execute(ispm_plan *p, const void *x, void *y);
+/* No reference counterpart: ispm_plan_execute for a caller whose x is readable and zero on
+ * x_guard elements before x[0] and after x[n_cols - 1] (a vector allocated with guard zones).
+ * Symmetric plans with FP32 vectors then take a kernel that reads its x windows without
+ * bounds checks (at least ispm_plan_x_guard(p) elements are needed, else the checked kernel
+ * runs); other plans ignore the guard.  Results equal ispm_plan_execute's bit for bit. */
+int ispm_plan_execute_guarded(ispm_plan *p, const void *x, void *y, unsigned x_guard);
+unsigned ispm_plan_x_guard(const ispm_plan *p);
 int ispm_plan_execute_multi(ispm_plan *p, int k, const void *X, size_t ldx, void *Y, size_t ldy);
 /* change_host_matrix + update_device_matrix_coefs (plan/plan.h:39-56): new values, same
  * sparsity pattern; conversion reuses the plan's format parameters. */

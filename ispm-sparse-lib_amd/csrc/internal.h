@@ -88,9 +88,10 @@ int spmv_split(const split_diag &sd, const ispm_launch *p, const ispm_slell_desc
 
 /* Symmetric diagonal storage of a symmetric matrix whose entries lie on at most 16 upper
  * diagonals (ISPM_SPMV_PLAN_SYMMETRIC): plane k of 256-row slice s holds A[row, row + d[k]]
- * at elms[(s K + k) 256 + r], d[0] = 0 < d[1] < ...; the lower entries are read from the
- * mirror rows.  pat lists the 2K-1 column offsets -d[K-1] .. -d[1], d[0] .. d[K-1] in
+ * at elms[(s K + k) kSymPlane + r], d[0] = 0 < d[1] < ..., with rows 256 .. 259 of a plane
+ * repeating the next slice's rows 0 .. 3; the lower entries are read from the mirror rows.  pat lists the 2K-1 column offsets -d[K-1] .. -d[1], d[0] .. d[K-1] in
  * that order for the x windows. */
+constexpr unsigned kSymPlane = 260;   /* stored length of a 256-row plane */
 struct sym_desc {
   const void *elms;
   int elm_type;
@@ -100,9 +101,14 @@ struct sym_desc {
 };
 
 /* y = A x for symmetric diagonal storage, with the fused dot product as spmv_dot_internal
- * (dotv, dparts, dgrid may be null) */
+ * (dotv, dparts, dgrid may be null).  x_guard: elements before x[0] and after x[n_cols - 1]
+ * that are readable and zero (vectors the solvers allocate with guard zones); with enough of
+ * them (sym_x_guard) FP32 products take the four-rows-per-lane kernel. */
 int spmv_sym(const sym_desc &sd, int vec_type, const void *x, void *y, const unsigned *skip,
-             const void *dotv = nullptr, double *dparts = nullptr, unsigned *dgrid = nullptr);
+             const void *dotv = nullptr, double *dparts = nullptr, unsigned *dgrid = nullptr,
+             unsigned x_guard = 0);
+/* the guard (elements on each side) the four-rows-per-lane symmetric kernel needs */
+unsigned sym_x_guard(const sym_desc &sd);
 
 /* Partial sums a fused SpMV + dot may store (one per workgroup). */
 constexpr unsigned kMaxDotParts = 32768;

@@ -752,18 +752,22 @@ static int create_sym(const ispm_csr *A, int et, unsigned flags, ispm_plan *p)
   pat.lds_len = used;
   if ((size_t)used * 8 > 65536) return fail_plain(ISPM_ERR_UNSUPPORTED, "symmetric plan: x windows exceed 64 KiB of LDS");
   for (unsigned k = 0; k < K; k++) sd.d[k] = (int)d[k];
-  if (ns * K * 256 >= (1ull << 32) || n_pad >= (1ull << 31))
+  if (ns * K * kSymPlane >= (1ull << 32) || n_pad >= (1ull << 31))
     return fail_plain(ISPM_ERR_UNSUPPORTED, "symmetric plan: more than 2^32 element slots");
   sd.K = K;
   sd.n_slices = (unsigned)ns;
   sd.n_cols = (unsigned)n;
   sd.elm_type = et;
-  /* planes slice by slice: element (s K + k) 256 + r */
-  std::vector<unsigned char> eh(ns * K * 256 * tsize(et));
+  /* planes slice by slice: element (s K + k) kSymPlane + r; the plane's rows 256 .. 259 repeat
+   * the next slice's first rows of the same diagonal (0 after the last slice), so that a
+   * mirror read of up to 4 consecutive rows never leaves its plane */
+  std::vector<unsigned char> eh(ns * K * kSymPlane * tsize(et));
   for (size_t s0 = 0; s0 < ns; s0++)
     for (unsigned k = 0; k < K; k++)
-      for (unsigned r = 0; r < 256; r++)
-        store_value(eh.data(), et, (s0 * K + k) * 256 + r, v[(size_t)k * n_pad + s0 * 256 + r]);
+      for (unsigned r = 0; r < kSymPlane; r++) {
+        const size_t row = s0 * 256 + r;
+        store_value(eh.data(), et, (s0 * K + k) * kSymPlane + r, row < n_pad ? v[(size_t)k * n_pad + row] : 0.0);
+      }
   void *de = nullptr;
   int st = ispm_malloc(&de, eh.size());
   if (st) return st;
@@ -778,7 +782,7 @@ static int create_sym(const ispm_csr *A, int et, unsigned flags, ispm_plan *p)
   p->dev.slice_height = 256;
   p->dev.diags = 1;
   p->dev.elms = de;
-  p->dev.elms_len = ns * K * 256;
+  p->dev.elms_len = ns * K * kSymPlane;
   p->dev.elm_type = et;
   p->has_dev = true;
   p->L.n_rows = (int)n_pad;
@@ -1015,6 +1019,21 @@ int ispm_plan_execute(ispm_plan *p, const void *x, void *y)
   if (p->sym) return spmv_sym(p->symd, p->T, x, y, nullptr);
   return spmv_internal(&p->launch, &p->dev, p->T, x, y, nullptr, p->has_pat ? &p->pat : nullptr,
                        p->dev_uni, &p->hints);
+}
+
+int ispm_plan_execute_guarded(ispm_plan *p, const void *x, void *y, unsigned x_guard)
+{
+  if (!p) return fail_plain(ISPM_ERR_INVALID, "plan_execute_guarded: null plan");
+  if (!p->sym) return ispm_plan_execute(p, x, y);
+  if (!p->has_dev) return fail_plain(ISPM_ERR_INVALID, "plan_execute_guarded: plan has no device matrix");
+  int st = ensure_init();
+  if (st) return st;
+  return spmv_sym(p->symd, p->T, x, y, nullptr, nullptr, nullptr, nullptr, x_guard);
+}
+
+unsigned ispm_plan_x_guard(const ispm_plan *p)
+{
+  return p && p->sym ? sym_x_guard(p->symd) : 0u;
 }
 
 int ispm_plan_execute_multi(ispm_plan *p, int k, const void *X, size_t ldx, void *Y, size_t ldy)

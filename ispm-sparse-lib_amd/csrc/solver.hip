@@ -452,6 +452,10 @@ struct ispm_solver {
   int n = 0;
   int T = ISPM_F64;
   size_t ts = 8;
+  /* the working vectors carry zero guard zones of xguard elements (gbytes bytes) on both
+   * sides, so that products of a symmetric A may read their x windows unchecked */
+  unsigned xguard = 0;
+  size_t gbytes = 0;
   bool mixed = false, pipelined = false, single = false;
   void *r = nullptr, *u = nullptr, *w = nullptr, *m = nullptr, *nv = nullptr, *z = nullptr,
        *q = nullptr, *sv = nullptr, *p = nullptr, *t = nullptr, *xs = nullptr;
@@ -471,13 +475,27 @@ namespace {
 
 int alloc(void **p, size_t bytes) { return ispm_malloc(p, bytes); }
 
-int spmv(ispm_plan *P, int vt, const void *x, void *y, const unsigned *skip)
+/* a working vector with the solver's guard zones (zero-filled like every allocation) */
+int alloc_guarded(ispm_solver *S, void **p, size_t bytes)
+{
+  void *base = nullptr;
+  int st = ispm_malloc(&base, bytes + 2 * S->gbytes);
+  *p = base ? static_cast<char *>(base) + S->gbytes : nullptr;
+  return st;
+}
+
+void free_guarded(ispm_solver *S, void *p)
+{
+  if (p) ispm_free(static_cast<char *>(p) - S->gbytes);
+}
+
+int spmv(ispm_plan *P, int vt, const void *x, void *y, const unsigned *skip, unsigned x_guard = 0)
 {
   if (P->split) {
     const split_diag sd{P->dev_diag, P->dev_rmap, P->dev_rmask, P->split_n, P->split_m, P->E};
     return spmv_split(sd, &P->launch, &P->dev, vt, x, y, skip, nullptr, nullptr, nullptr, &P->hints);
   }
-  if (P->sym) return spmv_sym(P->symd, vt, x, y, skip);
+  if (P->sym) return spmv_sym(P->symd, vt, x, y, skip, nullptr, nullptr, nullptr, x_guard);
   return spmv_internal(&P->launch, &P->dev, vt, x, y, skip, P->has_pat ? &P->pat : nullptr, P->dev_uni,
                        &P->hints);
 }
@@ -542,7 +560,7 @@ int spmv_dot(ispm_solver *S, ispm_plan *P, const void *x, void *y, const void *d
     const split_diag sd{P->dev_diag, P->dev_rmap, P->dev_rmask, P->split_n, P->split_m, P->E};
     st = spmv_split(sd, &P->launch, &P->dev, S->T, x, y, skip, d, parts, &G, &P->hints);
   } else if (P->sym) {
-    st = spmv_sym(P->symd, S->T, x, y, skip, d, parts, &G);
+    st = spmv_sym(P->symd, S->T, x, y, skip, d, parts, &G, S->xguard);   /* x: a working vector */
   } else {
     st = spmv_dot_internal(&P->launch, &P->dev, S->T, x, y, skip, P->has_pat ? &P->pat : nullptr,
                            P->dev_uni, d, parts, &G, &P->hints);
@@ -566,7 +584,7 @@ int iterate(ispm_solver *S)
   dim3 g(S->grid), b(256);
   if (S->pipelined) {
     if ((st = apply_M(S, S->w, S->m, sk))) return st;
-    if ((st = spmv(S->A, S->T, S->m, S->nv, sk))) return st;
+    if ((st = spmv(S->A, S->T, S->m, S->nv, sk, S->xguard))) return st;
     hipLaunchKernelGGL(k_ppcg<double>, g, b, 0, sm, (const double *)S->nv, (const double *)S->m,
                        (double *)S->p, (double *)S->sv, (double *)S->q, (double *)S->z,
                        (double *)S->xs, (double *)S->r, (double *)S->u, (double *)S->w, S->n, S->s,
@@ -761,13 +779,17 @@ int ispm_solver_create(ispm_solver **out, ispm_plan *A, ispm_plan *M1, ispm_plan
     const long long per_block = 256LL * (16 / (long long)S->ts);
     S->grid = (int)std::max(1LL, std::min((long long)kMaxBlocks, ((long long)S->n + per_block - 1) / per_block));
   }
+  if (A->sym) {
+    S->xguard = sym_x_guard(A->symd);
+    S->gbytes = (size_t)S->xguard * S->ts;
+  }
   void **vecs[] = {&S->r, &S->z, &S->q, &S->p, &S->t, &S->xs};
-  for (void **v : vecs) if ((st = alloc(v, vb))) { ispm_solver_destroy(S); return st; }
+  for (void **v : vecs) if ((st = alloc_guarded(S, v, vb))) { ispm_solver_destroy(S); return st; }
   if (S->pipelined) {
     void **pv[] = {&S->u, &S->w, &S->m, &S->nv, &S->sv};
-    for (void **v : pv) if ((st = alloc(v, vb))) { ispm_solver_destroy(S); return st; }
+    for (void **v : pv) if ((st = alloc_guarded(S, v, vb))) { ispm_solver_destroy(S); return st; }
   }
-  if (S->single && (st = alloc(&S->sv, vb))) { ispm_solver_destroy(S); return st; }
+  if (S->single && (st = alloc_guarded(S, &S->sv, vb))) { ispm_solver_destroy(S); return st; }
   if ((st = alloc((void **)&S->r64, 8 * (size_t)S->n + 64)) || (st = alloc((void **)&S->tmp64, 8 * (size_t)S->n + 64)) ||
       (st = alloc((void **)&S->s, sizeof(scalars))) || (st = alloc((void **)&S->parts, sizeof(double) * (2 * (size_t)kMaxDotParts + 4 * (size_t)kMaxBlocks)))) {
     ispm_solver_destroy(S);
@@ -1049,7 +1071,7 @@ void ispm_solver_destroy(ispm_solver *S)
   if (!S) return;
   if (S->exec) (void)hipGraphExecDestroy(S->exec);
   if (S->graph) (void)hipGraphDestroy(S->graph);
-  for (void *v : {S->r, S->u, S->w, S->m, S->nv, S->z, S->q, S->sv, S->p, S->t, S->xs}) ispm_free(v);
+  for (void *v : {S->r, S->u, S->w, S->m, S->nv, S->z, S->q, S->sv, S->p, S->t, S->xs}) free_guarded(S, v);
   ispm_free(S->r64);
   ispm_free(S->tmp64);
   ispm_free(S->sumA);

@@ -66,7 +66,7 @@ int spmv_split(const split_diag &sd, const ispm_launch *p, const ispm_slell_desc
 }
 
 int spmv_sym(const sym_desc &sd, int vec_type, const void *x, void *y, const unsigned *skip,
-             const void *dotv, double *dparts, unsigned *dgrid)
+             const void *dotv, double *dparts, unsigned *dgrid, unsigned x_guard)
 {
   if (dgrid) *dgrid = 0;
   if (!x || !y || !sd.elms) return fail(ISPM_ERR_INVALID, "spmv: null argument");
@@ -79,10 +79,10 @@ int spmv_sym(const sym_desc &sd, int vec_type, const void *x, void *y, const uns
   a.K = sd.K;
   for (unsigned k = 0; k < spmvk::kSymPlanes; k++) a.d[k] = sd.d[k];
   const int et = sd.elm_type;
-  if (vec_type == ISPM_F64 && et == ISPM_F64) return spmvk::launch_symdia<double, double>(a, sd.pat, x, y, skip, dotv, dparts, dgrid);
-  if (vec_type == ISPM_F64 && et == ISPM_F32) return spmvk::launch_symdia<double, float>(a, sd.pat, x, y, skip, dotv, dparts, dgrid);
-  if (vec_type == ISPM_F32 && et == ISPM_F32) return spmvk::launch_symdia<float, float>(a, sd.pat, x, y, skip, dotv, dparts, dgrid);
-  if (vec_type == ISPM_F32 && et == ISPM_BF16) return spmvk::launch_symdia<float, spmvk::bf16_t>(a, sd.pat, x, y, skip, dotv, dparts, dgrid);
+  if (vec_type == ISPM_F64 && et == ISPM_F64) return spmvk::launch_symdia<double, double>(a, sd.pat, x, y, skip, dotv, dparts, dgrid, x_guard);
+  if (vec_type == ISPM_F64 && et == ISPM_F32) return spmvk::launch_symdia<double, float>(a, sd.pat, x, y, skip, dotv, dparts, dgrid, x_guard);
+  if (vec_type == ISPM_F32 && et == ISPM_F32) return spmvk::launch_symdia<float, float>(a, sd.pat, x, y, skip, dotv, dparts, dgrid, x_guard);
+  if (vec_type == ISPM_F32 && et == ISPM_BF16) return spmvk::launch_symdia<float, spmvk::bf16_t>(a, sd.pat, x, y, skip, dotv, dparts, dgrid, x_guard);
   return fail(ISPM_ERR_UNSUPPORTED, "spmv: unsupported (vector, element) types (%d, %d)", vec_type, et);
 }
 
@@ -131,4 +131,9 @@ extern "C" int ispm_spmm(const ispm_launch *p, const ispm_slell_desc *A, int vec
   if (A->n_slices == 0) return ISPM_OK;
   if (k == 1) return spmv_internal(p, A, vec_type, X, Y, nullptr);
   return spmm_dispatch(p, A, vec_type, k, X, ldx, Y, ldy, nullptr);
+}
+
+unsigned ispm::sym_x_guard(const sym_desc &sd)
+{
+  return spmvk::sym_r4_guard(sd.pat, sd.n_slices, sd.n_cols);
 }

@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cstdlib>
+#include <type_traits>
 #include "device.h"
 #include "internal.h"
 #include "diag_pattern.h"
@@ -520,7 +521,8 @@ int launch_diagpat(const ispm_slell_desc *A, const diag_pattern &pat, const unsi
 /* ---- symmetric diagonal storage (ISPM_SPMV_PLAN_SYMMETRIC) ----
  * A symmetric matrix whose entries lie on a few diagonals (stencils) stores only its upper
  * diagonals d_0 = 0 < d_1 < ... < d_{K-1}: plane k of 256-row slice s holds A[row, row + d_k]
- * at elms[(s K + k) 256 + r] (zero where the row has no such entry).  The lower entries are
+ * at elms[(s K + k) kSymPlane + r] (zero where the row has no such entry; rows 256 .. 259 of
+ * a plane repeat the next slice's first rows).  The lower entries are
  * the mirrors: A[row, row - d_k] = A[row - d_k, row] = plane k at row - d_k, which an earlier
  * slice stored and read moments ago, so it comes from the cache while only the K upper planes
  * stream from HBM (half a 27-point stencil's bytes).  One workgroup per slice, two
@@ -628,19 +630,20 @@ __global__ void __launch_bounds__((256 / sym_rows_per_thread<T, E>())) symdia_ke
    * mirror rows lie below the last row) and takes its second value from the next slice. */
   typedef typename sym_raw<E>::type R;
   R ev[NK][H], mv[NK][H];
-  const unsigned KS = K * S, own = slice * KS + H * t;
+  constexpr unsigned PL = kSymPlane;
+  const unsigned KS = K * PL, own = slice * KS + H * t;
 #pragma unroll
   for (unsigned k = 0; k < NK; k++) {
 #pragma unroll
     for (unsigned e = 0; e < H; e++) ev[k][e] = mv[k][e] = R(0);
-    if (k < K) sym_rows<E, H>::get(el, own + k * S, ev[k]);
+    if (k < K) sym_rows<E, H>::get(el, own + k * PL, ev[k]);
     if (k >= 1 && k < K) {
       const int j = (int)row - a.d[k];
       if (j >= 0) {
-        sym_rows<E, H>::get_any(el, ((unsigned)j >> 8) * KS + k * S + ((unsigned)j & 255u), mv[k]);
-        if (H == 2 && ((unsigned)j & 255u) == 255u) mv[k][H - 1] = (R)el[(((unsigned)j + 1) >> 8) * KS + k * S];
+        /* a pair starting at a plane's last row reads the repeated next row */
+        sym_rows<E, H>::get_any(el, ((unsigned)j >> 8) * KS + k * PL + ((unsigned)j & 255u), mv[k]);
       } else if (H == 2 && j == -1) {
-        mv[k][H - 1] = (R)el[k * S];
+        mv[k][H - 1] = (R)el[k * PL];
       }
     }
   }
@@ -687,9 +690,152 @@ __global__ void __launch_bounds__((256 / sym_rows_per_thread<T, E>())) symdia_ke
   }
 }
 
+/* FP32 vectors and elements, four rows per lane.  The kernel above is bound by the rate at
+ * which the vector memory pipeline accepts load instructions (address unit busy ~85% of the
+ * time), not by bytes: here every load moves 16 bytes per lane, so a 256-row slice is one
+ * wave that issues 14 own-plane loads, 13 mirror loads (4 consecutive rows at 4-byte
+ * alignment, inside the plane thanks to its repeated rows) and about 10 window loads, against
+ * about 80 load instructions over two waves above.  Each wave has its own LDS windows, so no
+ * workgroup barrier; the planes go in two halves to bound the live registers.  x must be
+ * readable, and zero outside [0, n_cols), on every slice's window covers (the solvers keep
+ * zero guard zones around their vectors for this); mirror rows below 0 read as zero. */
+constexpr unsigned kR4Waves = 4, kR4Win = 10;
+
+/* the four rows of one lane (see symdia_r4_kernel); x must be readable (and 0) on the
+ * window covers of every slice, and mirror rows below 0 are read at row 0 and zeroed */
+template<unsigned NK>
+__device__ __forceinline__ float __attribute__((ext_vector_type(4)))
+symdia_r4_rows(const sym_args &a, const diag_pattern &pat, const float *__restrict__ el,
+               const float *__restrict__ xv, float *win, unsigned slice, int row0, unsigned lane)
+{
+  typedef float f4a __attribute__((ext_vector_type(4)));
+  typedef float f4u __attribute__((ext_vector_type(4))) __attribute__((aligned(4)));
+  constexpr unsigned PL = kSymPlane, K = NK, KS = K * PL, kR4Half = (NK + 1) / 2;
+  const int row = row0 + 4 * (int)lane;
+  f4a wx[kR4Win];
+#pragma unroll
+  for (unsigned q = 0; q < kR4Win; q++) {
+    const unsigned v0 = lane + 64 * q, v = v0 < pat.wtotal ? v0 : pat.wtotal - 1;
+    int dl = pat.wdelta[0];
+#pragma unroll
+    for (unsigned r = 1; r < 9; r++)
+      if (r < pat.nrun && v >= pat.wvbase[r]) dl = pat.wdelta[r];
+    wx[q] = *reinterpret_cast<const f4a *>(xv + (row0 + dl + (int)(4 * v)));
+  }
+  const float *own = el + slice * KS + 4 * lane;
+  f4a ev[kR4Half];
+  f4u mv[kR4Half];
+#pragma unroll
+  for (unsigned k = 0; k < kR4Half; k++) {
+    ev[k] = *reinterpret_cast<const f4a *>(own + k * PL);
+    mv[k] = f4u{0.f, 0.f, 0.f, 0.f};
+    if (k >= 1) {
+      const unsigned j = (unsigned)max(row - a.d[k], 0);
+      mv[k] = *reinterpret_cast<const f4u *>(el + (j >> 8) * KS + k * PL + (j & 255u));
+    }
+  }
+#pragma unroll
+  for (unsigned q = 0; q < kR4Win; q++)
+    if (lane + 64 * q < pat.wtotal) reinterpret_cast<f4a *>(win)[lane + 64 * q] = wx[q];
+  f4a ev2[NK - kR4Half];
+  f4u mv2[NK - kR4Half];
+#pragma unroll
+  for (unsigned k = kR4Half; k < NK; k++) {
+    ev2[k - kR4Half] = *reinterpret_cast<const f4a *>(own + k * PL);
+    const unsigned j = (unsigned)max(row - a.d[k], 0);
+    mv2[k - kR4Half] = *reinterpret_cast<const f4u *>(el + (j >> 8) * KS + k * PL + (j & 255u));
+  }
+  /* the wave's own window stores are complete before it reads them */
+  __builtin_amdgcn_s_waitcnt(0xc07f);
+  __builtin_amdgcn_wave_barrier();
+  f4a acc = f4a{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (unsigned k = 0; k < NK; k++) {
+    const f4a e = k < kR4Half ? ev[k] : ev2[k - kR4Half];
+    const float *xu = win + a.up_off[k] + 4 * lane;
+    acc.x += e.x * xu[0];
+    acc.y += e.y * xu[1];
+    acc.z += e.z * xu[2];
+    acc.w += e.w * xu[3];
+    if (k >= 1) {
+      f4u m = k < kR4Half ? mv[k] : mv2[k - kR4Half];
+      if (row0 < a.d[k]) {   /* only the first slices: uniform over the wave */
+        const int j = row - a.d[k];
+        if (j < 0) {
+          /* read rows 0..3 (the clamped start); row j + e is element j + e of them */
+          const float m0 = m.x, m1 = m.y, m2 = m.z;
+          m.x = 0.f;
+          m.y = j == -1 ? m0 : 0.f;
+          m.z = j == -2 ? m0 : j == -1 ? m1 : 0.f;
+          m.w = j == -3 ? m0 : j == -2 ? m1 : j == -1 ? m2 : 0.f;
+        }
+      }
+      const float *xl = win + a.lo_off[k] + 4 * lane;
+      acc.x += m.x * xl[0];
+      acc.y += m.y * xl[1];
+      acc.z += m.z * xl[2];
+      acc.w += m.w * xl[3];
+    }
+  }
+  return acc;
+}
+
+/* NK: the number of planes (compile time, so that no plane needs a branch); the first
+ * (NK + 1) / 2 planes are loaded with the windows, the rest after the windows went to LDS */
+template<unsigned NWAVES, unsigned NK>
+__global__ void __launch_bounds__(64 * NWAVES) symdia_r4_kernel(sym_args a, diag_pattern pat)
+{
+  extern __shared__ __align__(16) unsigned char sym_lds[];
+  typedef float f4a __attribute__((ext_vector_type(4)));
+  constexpr unsigned S = 256;
+  const unsigned lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  float *win = reinterpret_cast<float *>(sym_lds) + wv * pat.lds_len;
+  const unsigned xcd = blockIdx.x & 7u, i = (blockIdx.x >> 3) * kR4Waves + wv;
+  unsigned slice = xcd * a.per_xcd + i;
+  if (a.band) {
+    const unsigned q0 = xcd * a.P / 8, w = (xcd + 1) * a.P / 8 - q0;
+    slice = i < w * a.Z ? (i / w) * a.P + q0 + i % w : a.Z * a.P + (i - w * a.Z) * 8 + xcd;
+  }
+  if (a.skip && *a.skip) return;   /* uniform over the launch */
+  const bool active = slice < a.n_slices && i < a.per_xcd;
+  const float *__restrict__ el = static_cast<const float *>(a.elms);
+  const float *__restrict__ xv = static_cast<const float *>(a.x);
+  float *__restrict__ yv = static_cast<float *>(a.y);
+  const int row0 = (int)(slice * S), row = row0 + 4 * (int)lane;
+  f4a acc = f4a{0.f, 0.f, 0.f, 0.f};
+  if (active) acc = symdia_r4_rows<NK>(a, pat, el, xv, win, slice, row0, lane);
+  if (active) *reinterpret_cast<f4a *>(yv + row) = acc;
+  if (a.dparts) {
+    __shared__ double red[kR4Waves];
+    const float *__restrict__ dv = static_cast<const float *>(a.dotv);
+    double d[1] = {0.0};
+    if (active) {
+      /* CG's (p, A p) has x as the dot vector: its rows are in the window of offset 0 */
+      const bool dx = dv == xv;
+      const float *src = dx ? win + a.up_off[0] + 4 * lane : dv + row;
+      d[0] = (double)src[0] * (double)acc.x + (double)src[1] * (double)acc.y +
+             (double)src[2] * (double)acc.z + (double)src[3] * (double)acc.w;
+    }
+    dev::block_sum<64 * kR4Waves, 1>(d, red);
+    if (threadIdx.x == 0) a.dparts[blockIdx.x] = d[0];
+  }
+}
+
+/* zero elements x needs on each side so that every slice's FP32 window covers lie inside
+ * (the four-rows-per-lane kernel reads them without bounds checks) */
+inline unsigned sym_r4_guard(const diag_pattern &pat0, unsigned n_slices, unsigned n_cols)
+{
+  diag_pattern c4;
+  cover_pattern(pat0, 4, 64, 9, kR4Win, c4);
+  const long long lo = -c4.cover_lo, hi = (long long)(n_slices - 1) * 256 + c4.cover_hi - (long long)n_cols;
+  const long long g = std::max(0ll, std::max(lo, hi));
+  return (unsigned)((g + 63) / 64 * 64);
+}
+
 template<typename T, typename E>
 int launch_symdia(const sym_args &a0, const diag_pattern &pat0, const void *x, void *y,
-                  const unsigned *skip, const void *dotv, double *dparts, unsigned *dgrid)
+                  const unsigned *skip, const void *dotv, double *dparts, unsigned *dgrid,
+                  unsigned x_guard)
 {
   sym_args a = a0;
   a.x = x;
@@ -714,6 +860,30 @@ int launch_symdia(const sym_args &a0, const diag_pattern &pat0, const void *x, v
     a.dotv = dotv;
     a.dparts = dparts;
     *dgrid = a.per_xcd * 8;
+  }
+  if (std::is_same<T, float>::value && std::is_same<E, float>::value && x_guard > 0 &&
+      (reinterpret_cast<uintptr_t>(x) % 16) == 0 && x_guard >= sym_r4_guard(pat0, a.n_slices, a.n_cols)) {
+    diag_pattern c4;
+    cover_pattern(pat0, 4, 64, 9, kR4Win, c4);
+    const size_t lds4 = (size_t)c4.lds_len * sizeof(float) * kR4Waves;
+    /* the 7-, 19- and 27-point stencils (4, 10, 14 upper planes) */
+    if (c4.vec_runs && lds4 <= 65536 && (a.K == 4 || a.K == 10 || a.K == 14)) {
+      for (unsigned k = 0; k < a.K; k++) {
+        a.up_off[k] = c4.poff[(a.K - 1) + k];
+        a.lo_off[k] = k ? c4.poff[(a.K - 1) - k] : 0u;
+      }
+      const unsigned groups = (a.per_xcd + kR4Waves - 1) / kR4Waves;
+      if (dgrid && a.dparts) {
+        if (groups * 8 > kMaxDotParts) { a.dotv = nullptr; a.dparts = nullptr; *dgrid = 0; }
+        else *dgrid = groups * 8;
+      }
+      const dim3 g4(groups * 8), b4(64 * kR4Waves);
+      if (a.K == 4) hipLaunchKernelGGL((symdia_r4_kernel<kR4Waves, 4>), g4, b4, lds4, stream(), a, c4);
+      else if (a.K == 10) hipLaunchKernelGGL((symdia_r4_kernel<kR4Waves, 10>), g4, b4, lds4, stream(), a, c4);
+      else hipLaunchKernelGGL((symdia_r4_kernel<kR4Waves, 14>), g4, b4, lds4, stream(), a, c4);
+      ISPM_HIP(hipGetLastError());
+      return ISPM_OK;
+    }
   }
   constexpr unsigned NT = 256 / sym_rows_per_thread<T, E>(), VX = 16 / sizeof(T);
   diag_pattern c;

@@ -110,6 +110,8 @@ _SIGS = {
     "ispm_ppcg_update_scalars": (C.c_int, [C.c_int] + [C.c_void_p] * 5),
     "ispm_ppcg_update_vectors": (C.c_int, [C.c_int] + [C.c_void_p] * 15 + [C.c_int]),
     "ispm_ppcg_update_vectors_noalloc": (C.c_int, [C.c_int] + [C.c_void_p] * 15 + [C.c_int]),
+    "ispm_plan_execute_guarded": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint]),
+    "ispm_plan_x_guard": (C.c_uint, [C.c_void_p]),
     "ispm_csr_to_slell": (C.c_int, [C.POINTER(Csr), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_int, C.POINTER(SlellHost)]),
     "ispm_csr_to_slell_guess": (C.c_int, [C.POINTER(Csr), C.c_int, C.c_int, C.c_int,
@@ -464,6 +466,18 @@ class Plan:
         inf = self.info()
         self._check_vectors(inf.device.n_cols, self.n_rows_padded, x, y)
         check(lib().ispm_plan_execute(self.p, x.vp, y.vp))
+
+    def x_guard(self):
+        """Zero elements x needs on each side for ispm_plan_execute_guarded's fast kernel."""
+        return int(lib().ispm_plan_x_guard(self.p))
+
+    def execute_guarded(self, xg: DeviceArray, guard: int, y: DeviceArray):
+        """y = A x where xg holds `guard` zeros, x (n_cols entries, then the plan's padding)
+        and `guard` zeros again (ispm_plan_execute_guarded)."""
+        inf = self.info()
+        self._check_vectors(inf.device.n_cols + 2 * guard, self.n_rows_padded, xg, y)
+        ptr = C.c_void_p(xg.ptr + guard * np.dtype(xg.dtype).itemsize)
+        check(lib().ispm_plan_execute_guarded(self.p, ptr, y.vp, int(guard)))
 
     def execute_multi(self, k, X: DeviceArray, ldx, Y: DeviceArray, ldy):
         inf = self.info()

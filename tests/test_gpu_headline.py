@@ -88,6 +88,31 @@ def test_bf16_slell_full_size(work):
     spmv_check(A, pa, I.F32, I.BF16, 13)
 
 
+def test_guarded_symmetric_full_size(work):
+    """The FP32 product of A the mixed solve runs (x with zero guard zones, four rows per
+    lane): per entry against the oracle, and bit-identical to the checked kernel."""
+    A = work["A"]
+    p = I.Plan(A, I.F32, elm_type=I.F32, flags=I.SPMV_PLAN_SYMMETRIC)
+    g = p.x_guard()
+    n = p.n_rows_padded
+    x = G.uniform(14, A.n_rows)
+    xg = np.zeros(n + 2 * g, np.float32)
+    xg[g:g + A.n_rows] = x
+    yd = I.DeviceArray(n, np.float32)
+    p.execute_guarded(I.DeviceArray.from_numpy(xg), g, yd)
+    y = yd.numpy()
+    Ar = rounded(A, I.F32)
+    xr = x.astype(np.float32).astype(np.float64)
+    yref = O.csr_spmv(Ar, xr)
+    bad = np.abs(y[:A.n_rows].astype(np.float64) - yref) > 1e-5 * O.csr_spmv_abs(Ar, xr)
+    assert not bad.any(), int(bad.sum())
+    assert np.all(y[A.n_rows:] == 0)
+    x2 = I.DeviceArray.from_numpy(np.concatenate([x, np.zeros(n - A.n_rows)]).astype(np.float32))
+    y2 = I.DeviceArray(n, np.float32)
+    p.execute(x2, y2)
+    np.testing.assert_array_equal(y, y2.numpy())
+
+
 def headline_solver(work, precision):
     s = S.Solver(work["A"], M=work["M"], precision=precision, method=S.PCG, tol=TOL,
                  symmetric=True, fsai_product=True, split_diag=True)

@@ -357,6 +357,30 @@ def test_symmetric_plan(gpu, name, vt, et):
     check(A, None, x, run_plan(p, A, x, vt), vt, et)
 
 
+@pytest.mark.parametrize("name", ["poisson7", "stencil27_contrast", "stencil19_aniso", "small27",
+                                  "levels16", "levels131"])
+def test_symmetric_guarded_fp32(gpu, name):
+    """FP32 products of x held with zero guard zones (ispm_plan_execute_guarded) take the
+    four-rows-per-lane kernel: bit-identical to the checked kernel's product, and the CSR
+    product within the FP32 bar; a guard below what the plan needs runs the checked kernel."""
+    A = _sym_matrix(name)
+    p = I.Plan(A, I.F32, elm_type=I.F32, flags=I.SPMV_PLAN_SYMMETRIC)
+    g = p.x_guard()
+    assert g > 0 and g % 64 == 0
+    n = p.n_rows_padded
+    x = G.uniform(33, A.n_rows)
+    y_plain = run_plan(p, A, x, I.F32)
+    for guard in (g, g // 2):
+        xg = np.zeros(n + 2 * guard, np.float32)
+        xg[guard:guard + A.n_rows] = x
+        yd = I.DeviceArray(n, np.float32)
+        yd.upload(np.full(n, 7.0, np.float32))
+        p.execute_guarded(I.DeviceArray.from_numpy(xg), guard, yd)
+        y = yd.numpy()
+        np.testing.assert_array_equal(y, y_plain)
+        check(A, None, x, y, I.F32, I.F32)
+
+
 def test_symmetric_plan_integer_exact(gpu):
     A = G.stencil(12, 10, 9, 27, seed=5)
     Ai = I.CSR(A.n_rows, A.n_cols, A.row_ptr, A.cols, np.round(A.vals * 8))
