@@ -79,12 +79,23 @@ struct split_diag {
   int elm_type;
 };
 
+/* CG's residual update fused into a split-diagonal product: the product multiplies
+ * x - (*alpha) q (one fused multiply-add per entry), stores that vector to xout and leaves
+ * the partials of its squared norm in rrparts (as many as the dot product's) */
+struct split_update {
+  const void *q;
+  const double *alpha;
+  void *xout;
+  double *rrparts;
+};
+
 /* SpMV of a split-diagonal matrix (R given by its launch and descriptor; may be empty), with
- * the fused dot product as spmv_dot_internal (dparts/dgrid may be null). */
+ * the fused dot product as spmv_dot_internal (dparts/dgrid may be null); with up, the
+ * residual update in front (needs the fused dot). */
 int spmv_split(const split_diag &sd, const ispm_launch *p, const ispm_slell_desc *R, int vec_type,
                const void *x, void *y, const unsigned *skip, const void *dotv = nullptr,
                double *dparts = nullptr, unsigned *dgrid = nullptr,
-               const layout_hints *hints = nullptr);
+               const layout_hints *hints = nullptr, const split_update *up = nullptr);
 
 /* Symmetric diagonal storage of a symmetric matrix whose entries lie on at most 16 upper
  * diagonals (ISPM_SPMV_PLAN_SYMMETRIC): plane k of 256-row slice s holds A[row, row + d[k]]

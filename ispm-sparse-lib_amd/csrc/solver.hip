@@ -8,7 +8,11 @@
  *    (r,r) for the convergence test.
  *  - FP64 PCG (Hestenes-Stiefel): z = M r, rho = (r,z), p = z + beta p, q = A p, alpha,
  *    x += alpha p, r -= alpha q; x += alpha p runs in the next iteration's p update, which
- *    reads p anyway, and once more when the iterations stop.
+ *    reads p anyway, and once more when the iterations stop.  With a split-diagonal
+ *    preconditioner, r -= alpha q also moves into the next iteration: its product z = M r
+ *    reads r and q, multiplies r - alpha q and writes that vector to a second residual
+ *    buffer (the two swap every iteration, so no product reads an entry another workgroup
+ *    already updated), with (r, r) -- four launches per iteration instead of five.
  *  - single-reduction PCG (Chronopoulos-Gear): z = M r with (r,z), w = A z with (z,w), one
  *    finishing kernel (alpha, beta by the reference's formulas; the convergence test on the
  *    (r,r) of the previous update) and one fused update p = z + beta p, s = w + beta s,
@@ -457,6 +461,10 @@ struct ispm_solver {
   unsigned xguard = 0;
   size_t gbytes = 0;
   bool mixed = false, pipelined = false, single = false;
+  /* PCG with one split-diagonal preconditioner plan: r -= alpha q fused into z = M r, which
+   * writes the updated residual to r2; r and r2 swap every iteration (batches are even) */
+  bool fused_r = false;
+  void *r2 = nullptr;
   void *r = nullptr, *u = nullptr, *w = nullptr, *m = nullptr, *nv = nullptr, *z = nullptr,
        *q = nullptr, *sv = nullptr, *p = nullptr, *t = nullptr, *xs = nullptr;
   double *r64 = nullptr, *tmp64 = nullptr;
@@ -636,7 +644,19 @@ int iterate(ispm_solver *S)
   /* with a preconditioner, the (r, r) test of the previous iteration's update shares the
    * finishing kernel of (r, z) */
   double *pr = S->parts + 2 * kMaxDotParts;
-  if (S->M1) {
+  if (S->fused_r) {
+    /* r2 = r - alpha q and z = M r2 in one launch, with (r2, z) and (r2, r2); alpha is the
+     * previous iteration's (0 after a restart, and q is finite) */
+    ispm_plan *P = S->M1;
+    const split_diag sd{P->dev_diag, P->dev_rmap, P->dev_rmask, P->split_n, P->split_m, P->E};
+    const split_update up{S->q, &S->s->alpha, S->r2, pr};
+    if ((st = spmv_split(sd, &P->launch, &P->dev, S->T, S->r, S->z, sk, S->r, S->parts, &G, &P->hints, &up)))
+      return st;
+    if (G == 0) return fail(ISPM_ERR_RUNTIME, "solver: split product without its dot partials");
+    hipLaunchKernelGGL(k_fin_rz_rr, dim3(1), dim3(kFinThreads), 0, sm, (const double *)S->parts, G,
+                       (const double *)pr, G, S->s);
+    std::swap(S->r, S->r2);
+  } else if (S->M1) {
     /* z = M2 (M1 r), or M1 r for a one-matrix preconditioner, with (r, z) fused into the
      * last product where possible */
     ispm_plan *last = S->M2 ? S->M2 : S->M1;
@@ -664,6 +684,10 @@ int iterate(ispm_solver *S)
                         : spmv_dot<float>(S, S->A, S->p, S->q, S->p, sk, G);
   if (st) return st;
   finish<OP_PQ>(S, 1, G);
+  if (S->fused_r) {
+    ISPM_HIP(hipGetLastError());
+    return ISPM_OK;
+  }
   if (S->T == ISPM_F64)
     hipLaunchKernelGGL(k_r<double>, g, b, 0, sm, (double *)S->r, (const double *)S->q, S->n, S->s,
                        S->M1 ? pr : S->parts);
@@ -790,6 +814,12 @@ int ispm_solver_create(ispm_solver **out, ispm_plan *A, ispm_plan *M1, ispm_plan
     for (void **v : pv) if ((st = alloc_guarded(S, v, vb))) { ispm_solver_destroy(S); return st; }
   }
   if (S->single && (st = alloc_guarded(S, &S->sv, vb))) { ispm_solver_destroy(S); return st; }
+  S->fused_r = !S->pipelined && !S->single && M1 && !M2 && M1->split;
+  if (S->fused_r) {
+    if ((st = alloc_guarded(S, &S->r2, vb))) { ispm_solver_destroy(S); return st; }
+    /* iterations in pairs per batch, so that r names the same buffer after every batch */
+    S->o.check_every += S->o.check_every & 1;
+  }
   if ((st = alloc((void **)&S->r64, 8 * (size_t)S->n + 64)) || (st = alloc((void **)&S->tmp64, 8 * (size_t)S->n + 64)) ||
       (st = alloc((void **)&S->s, sizeof(scalars))) || (st = alloc((void **)&S->parts, sizeof(double) * (2 * (size_t)kMaxDotParts + 4 * (size_t)kMaxBlocks)))) {
     ispm_solver_destroy(S);
@@ -1071,7 +1101,7 @@ void ispm_solver_destroy(ispm_solver *S)
   if (!S) return;
   if (S->exec) (void)hipGraphExecDestroy(S->exec);
   if (S->graph) (void)hipGraphDestroy(S->graph);
-  for (void *v : {S->r, S->u, S->w, S->m, S->nv, S->z, S->q, S->sv, S->p, S->t, S->xs}) free_guarded(S, v);
+  for (void *v : {S->r, S->r2, S->u, S->w, S->m, S->nv, S->z, S->q, S->sv, S->p, S->t, S->xs}) free_guarded(S, v);
   ispm_free(S->r64);
   ispm_free(S->tmp64);
   ispm_free(S->sumA);

@@ -44,7 +44,7 @@ int spmv_dot_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_ty
 
 int spmv_split(const split_diag &sd, const ispm_launch *p, const ispm_slell_desc *R, int vec_type,
                const void *x, void *y, const unsigned *skip, const void *dotv, double *dparts,
-               unsigned *dgrid, const layout_hints *hints)
+               unsigned *dgrid, const layout_hints *hints, const split_update *up)
 {
   if (dgrid) *dgrid = 0;
   if (!x || !y || !sd.diag) return fail(ISPM_ERR_INVALID, "spmv: null argument");
@@ -55,13 +55,13 @@ int spmv_split(const split_diag &sd, const ispm_launch *p, const ispm_slell_desc
   if (rem && (!spmvk::use_stream(p, R) || R->elm_type != et || !sd.rmap || !sd.rmask))
     return fail(ISPM_ERR_INVALID, "spmv: split-diagonal remainder needs the streamed mapping");
   if (vec_type == ISPM_F64 && et == ISPM_F64)
-    return spmvk::launch_split<double, double>(R, sd.diag, sd.rmap, sd.rmask, sd.n, rem, x, y, skip, dotv, dparts, dgrid, hints);
+    return spmvk::launch_split<double, double>(R, sd.diag, sd.rmap, sd.rmask, sd.n, rem, x, y, skip, dotv, dparts, dgrid, hints, up);
   if (vec_type == ISPM_F64 && et == ISPM_F32)
-    return spmvk::launch_split<double, float>(R, sd.diag, sd.rmap, sd.rmask, sd.n, rem, x, y, skip, dotv, dparts, dgrid, hints);
+    return spmvk::launch_split<double, float>(R, sd.diag, sd.rmap, sd.rmask, sd.n, rem, x, y, skip, dotv, dparts, dgrid, hints, up);
   if (vec_type == ISPM_F32 && et == ISPM_F32)
-    return spmvk::launch_split<float, float>(R, sd.diag, sd.rmap, sd.rmask, sd.n, rem, x, y, skip, dotv, dparts, dgrid, hints);
+    return spmvk::launch_split<float, float>(R, sd.diag, sd.rmap, sd.rmask, sd.n, rem, x, y, skip, dotv, dparts, dgrid, hints, up);
   if (vec_type == ISPM_F32 && et == ISPM_BF16)
-    return spmvk::launch_split<float, spmvk::bf16_t>(R, sd.diag, sd.rmap, sd.rmask, sd.n, rem, x, y, skip, dotv, dparts, dgrid, hints);
+    return spmvk::launch_split<float, spmvk::bf16_t>(R, sd.diag, sd.rmap, sd.rmask, sd.n, rem, x, y, skip, dotv, dparts, dgrid, hints, up);
   return fail(ISPM_ERR_UNSUPPORTED, "spmv: unsupported (vector, element) types (%d, %d)", vec_type, et);
 }
 

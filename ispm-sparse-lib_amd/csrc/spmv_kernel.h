@@ -61,13 +61,36 @@ struct spmv_args {
    * y[r] = rdiag[r] * x[r] + the row's sum */
   const unsigned *rmap = nullptr;
   const void *rdiag = nullptr;
+  /* split-diagonal products with the CG residual update in front (AX kernels): the product
+   * multiplies x - alpha q instead of x, q = axq, alpha = *axa */
+  const void *axq = nullptr;
+  const double *axa = nullptr;
 };
+
+/* x - alpha q, one rounding (the CG residual update r - alpha q fused into the next product:
+ * every role that needs an entry computes it the same way) */
+__device__ __forceinline__ float ax_update(float x, float q, float a) { return __builtin_fmaf(-a, q, x); }
+__device__ __forceinline__ double ax_update(double x, double q, double a) { return __builtin_fma(-a, q, x); }
+
+
 
 template<typename T> __device__ __forceinline__ T xload(const T *x, unsigned i, unsigned n)
 {
   T v = x[i < n ? i : n - 1];
   return i < n ? v : (T)0;
 }
+
+/* entry i of x (bounds-checked), or of x - alpha q when AX */
+template<typename T, bool AX> struct xsrc {
+  const T *x, *q;
+  T a;
+  unsigned n;
+  __device__ __forceinline__ T at(unsigned i) const
+  {
+    const T v = xload(x, i, n);
+    return AX ? ax_update(v, xload(q, i, n), a) : v;
+  }
+};
 
 template<typename T, int H> struct vstore;
 template<> struct vstore<double, 2> {
@@ -695,11 +718,13 @@ __global__ void __launch_bounds__((256 / sym_rows_per_thread<T, E>())) symdia_ke
  * time), not by bytes: here every load moves 16 bytes per lane, so a 256-row slice is one
  * wave that issues 14 own-plane loads, 13 mirror loads (4 consecutive rows at 4-byte
  * alignment, inside the plane thanks to its repeated rows) and about 10 window loads, against
- * about 80 load instructions over two waves above.  Each wave has its own LDS windows, so no
- * workgroup barrier; the planes go in two halves to bound the live registers.  x must be
- * readable, and zero outside [0, n_cols), on every slice's window covers (the solvers keep
- * zero guard zones around their vectors for this); mirror rows below 0 read as zero. */
-constexpr unsigned kR4Waves = 4, kR4Win = 10;
+ * about 80 load instructions over two waves above.  A workgroup is one wave with its own LDS
+ * windows: no barrier, and LDS and registers are handed on as each wave ends (four waves per
+ * group ran 6% slower, 103.6 against 97 us at 27 points 192^3); the planes go in two halves to
+ * bound the live registers.  x must be readable, and zero outside [0, n_cols), on every
+ * slice's window covers (the solvers keep zero guard zones around their vectors for this);
+ * mirror rows below 0 read as zero. */
+constexpr unsigned kR4Waves = 1, kR4Win = 10;
 
 /* the four rows of one lane (see symdia_r4_kernel); x must be readable (and 0) on the
  * window covers of every slice, and mirror rows below 0 are read at row 0 and zeroed */
@@ -1024,7 +1049,7 @@ __global__ void __launch_bounds__(256) slell_packed_kernel(spmv_args a)
  * dependent gather per element.  Suited to irregular, short rows (approximate-inverse
  * factors): the work of a workgroup is three dependent memory round trips (offsets,
  * elements and columns, x). */
-template<typename T, typename E, int RPT>
+template<typename T, typename E, int RPT, bool AX = false>
 __device__ __forceinline__ void stream_body(const spmv_args &a, unsigned cap, unsigned bid)
 {
   constexpr unsigned ROWS = 256u * RPT, U = 4, VEC = dev::evec<E>::n;
@@ -1042,6 +1067,7 @@ __device__ __forceinline__ void stream_body(const spmv_args &a, unsigned cap, un
   T *__restrict__ yv = static_cast<T *>(a.y);
   const T *__restrict__ dv = static_cast<const T *>(a.dotv);
   const unsigned ncols = a.n_cols;
+  const xsrc<T, AX> xs{xv, static_cast<const T *>(a.axq), AX ? (T)*a.axa : (T)0, ncols};
   const unsigned group = (bid & 7u) * a.per_xcd + (bid >> 3);
   const unsigned s0 = group * spg;
   if (s0 >= a.n_slices) {   /* uniform over the workgroup */
@@ -1064,7 +1090,7 @@ __device__ __forceinline__ void stream_body(const spmv_args &a, unsigned cap, un
     dvv[k] = 0;
     if (dst[k] != ~0u) {
       /* the preconditioner's dot (r, M r) has x itself as the dot vector: one read of x */
-      const T xd = a.rmap || dv == xv ? xv[dst[k]] : (T)0;
+      const T xd = a.rmap || dv == xv ? xs.at(dst[k]) : (T)0;
       if (a.rmap) dterm[k] = elt<T, E>::get(static_cast<const E *>(a.rdiag), dst[k]) * xd;
       if (a.dparts) dvv[k] = dv == xv ? xd : dv[dst[k]];
     }
@@ -1087,11 +1113,11 @@ __device__ __forceinline__ void stream_body(const spmv_args &a, unsigned cap, un
     const unsigned hend = min(A0, E1);
     if (E0 + t < hend) {
       const unsigned e = E0 + t;
-      prod[e - base] = elt<T, E>::get(el, e) * xload(xv, dev::col_load(cl + e), ncols);
+      prod[e - base] = elt<T, E>::get(el, e) * xs.at(dev::col_load(cl + e));
     }
     if (A1 + t < E1 && A1 >= hend) {
       const unsigned e = A1 + t;
-      prod[e - base] = elt<T, E>::get(el, e) * xload(xv, dev::col_load(cl + e), ncols);
+      prod[e - base] = elt<T, E>::get(el, e) * xs.at(dev::col_load(cl + e));
     }
     const unsigned nv = (A1 - A0) / VEC;
     /* rounds of 256 vectors this workgroup needs (uniform), up to U per step: the element
@@ -1115,7 +1141,7 @@ __device__ __forceinline__ void stream_body(const spmv_args &a, unsigned cap, un
       for (unsigned u = 0; u < U; u++)
         if (r0 + u < rounds)
 #pragma unroll
-          for (unsigned q = 0; q < VEC; q++) xg[u][q] = xload(xv, cv[u][q], ncols);
+          for (unsigned q = 0; q < VEC; q++) xg[u][q] = xs.at(cv[u][q]);
 #pragma unroll
       for (unsigned u = 0; u < U; u++)
         if (r0 + u < rounds && v0 + u * 256 < nv) {
@@ -1146,7 +1172,7 @@ __device__ __forceinline__ void stream_body(const spmv_args &a, unsigned cap, un
            * whether the group fits */
 #pragma clang fp contract(off)
           const unsigned e = o + (q << log2S) + r;
-          const T pr = elt<T, E>::get(el, e) * xload(xv, dev::col_load(cl + e), ncols);
+          const T pr = elt<T, E>::get(el, e) * xs.at(dev::col_load(cl + e));
           acc[k] += pr;
         }
       }
@@ -1371,27 +1397,42 @@ inline int launch_stream_types(const ispm_slell_desc *A, int vt, const void *x, 
  * the bandwidth-bound diagonal stream.  Each diagonal workgroup thread takes VEC rows per
  * step: a vector load of d (nontemporal, read once), vector loads of x (and the dot
  * product's vector) and a vector store of y when VL (x, y, dotv aligned); fused dot
- * partials go to dparts[block] for both roles. */
+ * partials go to dparts[block] for both roles.
+ *
+ * AX: the product of x - alpha q (CG's residual update r - alpha q, then z = M r, in one
+ * pass).  The diagonal role computes the updated vector for every row, stores it to xout
+ * and sums its squares into rrparts[block]; the remainder role computes the entries it
+ * gathers the same way (one fused multiply-add each) and stores 0 there.  x is only read,
+ * so no workgroup sees another's update. */
 struct split_args {
   const void *d;
   const unsigned *rmask;
   unsigned n;        /* rows, a multiple of VEC */
   unsigned G2;       /* remainder workgroups (a multiple of 8, or 0) */
   double *dparts;    /* fused dot: G2 + G1 partials, or null */
+  void *xout;        /* AX: x - alpha q, n rows */
+  double *rrparts;   /* AX: G2 + G1 partials of (x - alpha q, x - alpha q) */
 };
 
-template<typename T, typename E, bool VL>
+template<typename T, typename E, bool VL, bool AX>
 __device__ __forceinline__ void diag_body(const spmv_args &a, const split_args &sa, unsigned bid, unsigned nb)
 {
-  __shared__ double red[4];
+  __shared__ double red[8];
   if (a.skip && *a.skip) return;
   constexpr unsigned V = dev::evec<E>::n;
+  constexpr int NS = AX ? 2 : 1;
   typedef T tv __attribute__((ext_vector_type(V)));
   const E *__restrict__ d = static_cast<const E *>(sa.d);
   const T *__restrict__ x = static_cast<const T *>(a.x);
   const T *__restrict__ dv = static_cast<const T *>(a.dotv);
+  const T *__restrict__ qv = static_cast<const T *>(a.axq);
+  T *__restrict__ xo = static_cast<T *>(sa.xout);
   T *__restrict__ y = static_cast<T *>(a.y);
-  double acc[1] = {0.0};
+  static_assert(VL || !AX, "the residual update runs on whole vectors");
+  const T al = AX ? (T)*a.axa : (T)0;
+  double acc[NS];
+#pragma unroll
+  for (int k = 0; k < NS; k++) acc[k] = 0.0;
   const unsigned nv = sa.n / V;
   for (unsigned i = bid * 256u + threadIdx.x; i < nv; i += nb * 256u) {
     T dd[V], xx[V], yy[V];
@@ -1401,6 +1442,17 @@ __device__ __forceinline__ void diag_body(const spmv_args &a, const split_args &
       const tv xv = *reinterpret_cast<const tv *>(x + (size_t)i * V);
 #pragma unroll
       for (unsigned q = 0; q < V; q++) xx[q] = xv[q];
+      if (AX) {
+        const tv qq = *reinterpret_cast<const tv *>(qv + (size_t)i * V);
+        tv xn;
+#pragma unroll
+        for (unsigned q = 0; q < V; q++) {
+          xx[q] = ax_update(xx[q], qq[q], al);
+          xn[q] = xx[q];
+          acc[NS - 1] += (double)xx[q] * (double)xx[q];
+        }
+        *reinterpret_cast<tv *>(xo + (size_t)i * V) = xn;
+      }
     } else {
 #pragma unroll
       for (unsigned q = 0; q < V; q++) xx[q] = x[(size_t)i * V + q];
@@ -1423,23 +1475,30 @@ __device__ __forceinline__ void diag_body(const spmv_args &a, const split_args &
     }
   }
   if (sa.dparts) {
-    dev::block_sum<256, 1>(acc, red);
-    if (threadIdx.x == 0) sa.dparts[sa.G2 + bid] = acc[0];
+    dev::block_sum<256, NS>(acc, red);
+    if (threadIdx.x == 0) {
+      sa.dparts[sa.G2 + bid] = acc[0];
+      if (AX) sa.rrparts[sa.G2 + bid] = acc[NS - 1];
+    }
   }
 }
 
-template<typename T, typename E, int RPT, bool VL>
+template<typename T, typename E, int RPT, bool VL, bool AX>
 __global__ void __launch_bounds__(256) split_kernel(spmv_args a, unsigned cap, split_args sa)
 {
-  if (blockIdx.x < sa.G2) stream_body<T, E, RPT>(a, cap, blockIdx.x);
-  else diag_body<T, E, VL>(a, sa, blockIdx.x - sa.G2, gridDim.x - sa.G2);
+  if (blockIdx.x < sa.G2) {
+    stream_body<T, E, RPT, AX>(a, cap, blockIdx.x);
+    if (AX && threadIdx.x == 0) sa.rrparts[blockIdx.x] = 0.0;
+  } else {
+    diag_body<T, E, VL, AX>(a, sa, blockIdx.x - sa.G2, gridDim.x - sa.G2);
+  }
 }
 
 template<typename T, typename E>
 int launch_split(const ispm_slell_desc *R, const void *diag, const unsigned *rmap,
                  const unsigned *rmask, unsigned n, bool has_rem, const void *x, void *y,
                  const unsigned *skip, const void *dotv, double *dparts, unsigned *dgrid,
-                 const layout_hints *hints)
+                 const layout_hints *hints, const split_update *up = nullptr)
 {
   constexpr unsigned V = dev::evec<E>::n;
   spmv_args a;
@@ -1474,7 +1533,7 @@ int launch_split(const ispm_slell_desc *R, const void *diag, const unsigned *rma
    * first: they are the longer chain (gathers), and measured slower after the diagonal ones
    * or interleaved with them. */
   const unsigned G1 = std::max(1u, std::min(8192u, (n / V + 255) / 256));
-  split_args sa{diag, rmask, n, G2, nullptr};
+  split_args sa{diag, rmask, n, G2, nullptr, nullptr, nullptr};
   if (dotv && dparts && dgrid && G1 + G2 <= kMaxDotParts) {
     a.dotv = dotv;
     a.dparts = dparts;
@@ -1482,14 +1541,28 @@ int launch_split(const ispm_slell_desc *R, const void *diag, const unsigned *rma
     *dgrid = G1 + G2;
   }
   const uintptr_t al = reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(y) |
-                       (sa.dparts ? reinterpret_cast<uintptr_t>(dotv) : 0);
+                       (sa.dparts ? reinterpret_cast<uintptr_t>(dotv) : 0) |
+                       (up ? reinterpret_cast<uintptr_t>(up->q) | reinterpret_cast<uintptr_t>(up->xout) : 0);
   const bool vl = al % (V * sizeof(T)) == 0;
+  if (up) {
+    /* the update needs the fused dot (its partials count is the caller's) and whole vectors */
+    if (!sa.dparts || !vl || !up->q || !up->alpha || !up->xout || !up->rrparts)
+      return fail(ISPM_ERR_INVALID, "spmv: split product with residual update needs aligned vectors and the fused dot");
+    a.axq = up->q;
+    a.axa = up->alpha;
+    sa.xout = up->xout;
+    sa.rrparts = up->rrparts;
+  }
   const dim3 g(G1 + G2), b(256);
   hipStream_t st = stream();
-#define ISPM_SPLIT_LAUNCH(R_, VL_) hipLaunchKernelGGL((split_kernel<T, E, R_, VL_>), g, b, lds, st, a, cap, sa)
-  if (rpt == 1) { if (vl) ISPM_SPLIT_LAUNCH(1, true); else ISPM_SPLIT_LAUNCH(1, false); }
-  else if (rpt == 2) { if (vl) ISPM_SPLIT_LAUNCH(2, true); else ISPM_SPLIT_LAUNCH(2, false); }
-  else { if (vl) ISPM_SPLIT_LAUNCH(4, true); else ISPM_SPLIT_LAUNCH(4, false); }
+#define ISPM_SPLIT_LAUNCH(R_, VL_, AX_) hipLaunchKernelGGL((split_kernel<T, E, R_, VL_, AX_>), g, b, lds, st, a, cap, sa)
+  if (up) {
+    if (rpt == 1) ISPM_SPLIT_LAUNCH(1, true, true);
+    else if (rpt == 2) ISPM_SPLIT_LAUNCH(2, true, true);
+    else ISPM_SPLIT_LAUNCH(4, true, true);
+  } else if (rpt == 1) { if (vl) ISPM_SPLIT_LAUNCH(1, true, false); else ISPM_SPLIT_LAUNCH(1, false, false); }
+  else if (rpt == 2) { if (vl) ISPM_SPLIT_LAUNCH(2, true, false); else ISPM_SPLIT_LAUNCH(2, false, false); }
+  else { if (vl) ISPM_SPLIT_LAUNCH(4, true, false); else ISPM_SPLIT_LAUNCH(4, false, false); }
 #undef ISPM_SPLIT_LAUNCH
   ISPM_HIP(hipGetLastError());
   return ISPM_OK;
