@@ -453,7 +453,7 @@ def main():
         "spmv": legs,
         # the dominant kernel of the timed (mixed) solve: its A product
         "roofline": (roofline(legs["fp32_sym"], "fp32_sym",
-                              "symdia_r4_kernel<4,14> (FP32 symmetric-storage 27-point 192^3 "
+                              "symdia_r4_kernel<1,14> (FP32 symmetric-storage 27-point 192^3 "
                               "SpMV on x with guard zones, four rows per lane; the largest "
                               "share of the mixed solve)")
                      if mixed["A_storage"] != "SlELL" else
