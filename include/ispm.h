@@ -288,11 +288,14 @@ int ispm_plan_execute_guarded(ispm_plan *p, const void *x, void *y, unsigned x_g
 unsigned ispm_plan_x_guard(const ispm_plan *p);
 int ispm_plan_execute_multi(ispm_plan *p, int k, const void *X, size_t ldx, void *Y, size_t ldy);
 /* change_host_matrix + update_device_matrix_coefs (plan/plan.h:39-56): new values, same
- * sparsity pattern; conversion reuses the plan's format parameters. */
+ * sparsity pattern; conversion reuses the plan's format parameters.  Symmetric plans take
+ * the upper entries and check every lower one against its mirror (ISPM_ERR_INVALID when
+ * they differ); split-diagonal plans are rebuilt instead (ISPM_ERR_UNSUPPORTED). */
 int ispm_plan_update_coefs(ispm_plan *p, const ispm_csr *A);
 /* On-device refresh: vals (device pointer, nnz values of val_type F32 or F64, in the CSR
  * order the plan was built from) are scattered into the element slots through the
- * conversion's entry -> slot map; no host re-conversion and no full re-upload. */
+ * conversion's entry -> slot map; no host re-conversion and no full re-upload.  Symmetric
+ * plans read the upper entries only (the caller keeps the matrix symmetric). */
 int ispm_plan_update_coefs_device(ispm_plan *p, const void *vals, int val_type);
 void ispm_plan_destroy(ispm_plan *p);
 
@@ -446,6 +449,9 @@ typedef struct ispm_batch ispm_batch;
 int ispm_batch_create(ispm_batch **out, int n_sys, const int *sys_rows, const int *row_ptr,
                       const int *cols, const float *vals);
 int ispm_batch_layout(const ispm_batch *b, long long *rows_padded, long long *row_offsets);
+/* bytes of the stored matrix (values, columns, slice offsets and widths): what one product
+ * over every system reads besides its vectors */
+int ispm_batch_matrix_bytes(const ispm_batch *b, long long *bytes);
 int ispm_batch_solve(ispm_batch *b, const float *rhs, float *x, double tol, int maxit,
                      int *iters, double *relres);
 void ispm_batch_destroy(ispm_batch *b);

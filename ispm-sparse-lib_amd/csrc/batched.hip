@@ -476,6 +476,7 @@ using namespace ispm;
 struct ispm_batch {
   int n_sys = 0, n_slices = 0, n_segs = 0;
   long long rows_padded = 0;
+  long long matrix_bytes = 0;   /* values, columns, slice offsets and widths */
   bool wide_cols = false;
   std::vector<long long> row_off;
   sys_desc *d_sys = nullptr;
@@ -693,6 +694,7 @@ int ispm_batch_create(ispm_batch **out, int n_sys, const int *sys_rows, const in
     colbytes = sizeof(unsigned short) * hc16.size();
   }
   const size_t vb = sizeof(float) * (size_t)prow;
+  B->matrix_bytes = (long long)(colbytes + sizeof(float) * hv.size() + sizeof(unsigned) * soff.size() + sw.size());
   void *hl = nullptr;
   if ((st = up((void **)&B->d_sys, sd.data(), sizeof(sys_desc) * n_sys)) ||
       (st = up((void **)&B->d_soff, soff.data(), sizeof(unsigned) * soff.size())) ||
@@ -730,6 +732,13 @@ int ispm_batch_layout(const ispm_batch *B, long long *rows_padded, long long *ro
   if (!B) return fail(ISPM_ERR_INVALID, "batch_layout: null batch");
   if (rows_padded) *rows_padded = B->rows_padded;
   if (row_offsets) std::copy(B->row_off.begin(), B->row_off.end(), row_offsets);
+  return ISPM_OK;
+}
+
+int ispm_batch_matrix_bytes(const ispm_batch *B, long long *bytes)
+{
+  if (!B || !bytes) return fail(ISPM_ERR_INVALID, "batch_matrix_bytes: null argument");
+  *bytes = B->matrix_bytes;
   return ISPM_OK;
 }
 

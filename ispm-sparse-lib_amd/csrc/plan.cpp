@@ -727,6 +727,19 @@ static int create_sym(const ispm_csr *A, int et, unsigned flags, ispm_plan *p)
       nlo++;
     }
   if (nup != nlo) return fail_plain(ISPM_ERR_INVALID, "symmetric plan: the matrix is not symmetric");
+  /* the CSR entry -> slot map of the refreshes: an upper entry (i, c >= i) -> its slot
+   * (s K + k) kSymPlane + r of row i = 256 s + r; a lower entry -> -(slot of its mirror) - 2 */
+  {
+    const long long nnz = A->row_ptr[n];
+    p->L.map.assign((size_t)nnz, -1);
+    for (int i = 0; i < n; i++)
+      for (int k = A->row_ptr[i]; k < A->row_ptr[i + 1]; k++) {
+        const int c = A->cols[k];
+        const int lo = std::min(i, c), hi = std::max(i, c);
+        const long long slot = ((long long)(lo / 256) * K + idx(hi - lo)) * kSymPlane + lo % 256;
+        p->L.map[k] = c >= i ? slot : -slot - 2;
+      }
+  }
   /* x-window pattern over the offsets -d[K-1] .. d[K-1] (positions relative to the slice's
    * first row + 256, runs of offsets no more than 8 apart, as build_pattern makes them) */
   std::vector<long long> R;
@@ -790,10 +803,45 @@ static int create_sym(const ispm_csr *A, int et, unsigned flags, ispm_plan *p)
   p->L.S = 256;
   p->L.D = true;
   p->L.n_slices = (unsigned)ns;
+  p->L.n_elms = ns * K * kSymPlane;
   p->launch = default_launch(p->L);
   p->stored = tsize(et) * ns * K * 256 + tsize(p->T) * ((size_t)n + n_pad);
   p->csr = csr_bytes(A, p->E, p->T);
   return ISPM_OK;
+}
+}  // namespace ispm
+
+namespace ispm {
+/* New values for a symmetric plan's pattern: upper entries into their slots (and into the
+ * repeated rows 256 .. 259 of the previous slice's plane), then every lower entry must equal
+ * its mirror, as at creation. */
+static int sym_refresh_host(ispm_plan *p, const ispm_csr *A)
+{
+  const unsigned K = p->symd.K, et = (unsigned)p->E;
+  const size_t ks = (size_t)K * kSymPlane;
+  std::vector<unsigned char> eh(p->L.n_elms * tsize(et), 0);
+  const long long nnz = A->row_ptr[A->n_rows];
+  for (long long k = 0; k < nnz; k++) {
+    const long long s = p->L.map[k];
+    if (s < 0) continue;
+    const double v = csr_value(A, k);
+    store_value(eh.data(), et, (size_t)s, v);
+    if ((size_t)s % kSymPlane < 4 && (size_t)s >= ks) store_value(eh.data(), et, (size_t)s - ks + 256, v);
+  }
+  for (int i = 0; i < A->n_rows; i++)
+    for (int k = A->row_ptr[i]; k < A->row_ptr[i + 1]; k++) {
+      const long long s = p->L.map[k];
+      if (s >= 0) continue;
+      /* the mirror's stored value, compared in the plan's element type */
+      std::vector<unsigned char> one(tsize(et));
+      store_value(one.data(), et, 0, csr_value(A, k));
+      if (std::memcmp(one.data(), eh.data() + (size_t)(-s - 2) * tsize(et), tsize(et))) {
+        char m[96];
+        std::snprintf(m, sizeof m, "plan_update_coefs: entry (%d, %d) differs from its mirror", i, A->cols[k]);
+        return fail_plain(ISPM_ERR_INVALID, m);
+      }
+    }
+  return ispm_memcpy(const_cast<void *>(p->symd.elms), eh.data(), eh.size(), ISPM_H2D);
 }
 }  // namespace ispm
 
@@ -1047,11 +1095,12 @@ int ispm_plan_execute_multi(ispm_plan *p, int k, const void *X, size_t ldx, void
 int ispm_plan_update_coefs(ispm_plan *p, const ispm_csr *A)
 {
   if (!p) return fail_plain(ISPM_ERR_INVALID, "plan_update_coefs: null plan");
-  if (plan_single_vector(p)) return fail_plain(ISPM_ERR_UNSUPPORTED, "plan_update_coefs: split-diagonal and symmetric plans are rebuilt");
+  if (p->split) return fail_plain(ISPM_ERR_UNSUPPORTED, "plan_update_coefs: split-diagonal plans are rebuilt");
   int st = check_csr(A);
   if (st) return st;
   if ((long long)A->row_ptr[A->n_rows] != (long long)p->L.map.size())
     return fail_plain(ISPM_ERR_INVALID, "plan_update_coefs: pattern size differs from the plan's");
+  if (p->sym) return sym_refresh_host(p, A);
   if (fill_values(p->L, A, p->E, p->elms.data())) return fail_plain(ISPM_ERR_INVALID, "value map incomplete");
   if (p->has_dev && p->L.n_elms)
     return ispm_memcpy((void *)p->dev.elms, p->elms.data(), tsize(p->E) * p->L.n_elms, ISPM_H2D);

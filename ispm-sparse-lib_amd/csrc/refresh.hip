@@ -34,6 +34,25 @@ __global__ void __launch_bounds__(256) scatter_kernel(E *__restrict__ elms, cons
   }
 }
 
+/* symmetric plans: slot ~0u (a lower entry, the mirror of a stored one) is skipped; a slot
+ * in a plane's first four rows is also written to the previous slice's repeated rows */
+template<typename V, typename E>
+__global__ void __launch_bounds__(256) sym_scatter_kernel(E *__restrict__ elms, const V *__restrict__ vals,
+                                                          const unsigned *__restrict__ map, long long nnz,
+                                                          unsigned ks)
+{
+  const long long stride = (long long)gridDim.x * 256;
+  for (long long k = (long long)blockIdx.x * 256 + threadIdx.x; k < nnz; k += stride) {
+    const unsigned s = map[k];
+    if (s == ~0u) continue;
+    E e;
+    if constexpr (sizeof(E) == 2) e = f32_bits_to_bf16((float)vals[k]);
+    else e = (E)vals[k];
+    elms[s] = e;
+    if (s % kSymPlane < 4 && s >= ks) elms[s - ks + 256] = e;
+  }
+}
+
 template<typename V>
 int scatter(ispm_plan *p, const V *vals, long long nnz)
 {
@@ -43,6 +62,16 @@ int scatter(ispm_plan *p, const V *vals, long long nnz)
   if (blocks < 1) blocks = 1;
   dim3 g((unsigned)blocks), b(256);
   void *e = const_cast<void *>(p->dev.elms);
+  if (p->sym) {
+    const unsigned ks = p->symd.K * kSymPlane;
+    switch (p->E) {
+      case ISPM_F64: hipLaunchKernelGGL((sym_scatter_kernel<V, double>), g, b, 0, stream(), (double *)e, vals, map, nnz, ks); break;
+      case ISPM_F32: hipLaunchKernelGGL((sym_scatter_kernel<V, float>), g, b, 0, stream(), (float *)e, vals, map, nnz, ks); break;
+      default: hipLaunchKernelGGL((sym_scatter_kernel<V, unsigned short>), g, b, 0, stream(), (unsigned short *)e, vals, map, nnz, ks); break;
+    }
+    ISPM_HIP(hipGetLastError());
+    return ISPM_OK;
+  }
   switch (p->E) {
     case ISPM_F64: hipLaunchKernelGGL((scatter_kernel<V, double>), g, b, 0, stream(), (double *)e, vals, map, nnz); break;
     case ISPM_F32: hipLaunchKernelGGL((scatter_kernel<V, float>), g, b, 0, stream(), (float *)e, vals, map, nnz); break;
@@ -92,13 +121,16 @@ extern "C" int ispm_plan_update_coefs_device(ispm_plan *p, const void *vals, int
 {
   if (!p || !vals) return fail(ISPM_ERR_INVALID, "plan_update_coefs_device: null argument");
   if (!p->has_dev) return fail(ISPM_ERR_INVALID, "plan_update_coefs_device: plan has no device matrix");
-  if (plan_single_vector(p)) return fail(ISPM_ERR_UNSUPPORTED, "plan_update_coefs_device: split-diagonal and symmetric plans are rebuilt");
+  if (p->split) return fail(ISPM_ERR_UNSUPPORTED, "plan_update_coefs_device: split-diagonal plans are rebuilt");
   int st = ensure_init();
   if (st) return st;
   const long long nnz = (long long)p->L.map.size();
   if (!p->dev_map) {
-    /* slot indices fit 32 bits: conversion refuses layouts beyond 2^32 slots */
-    std::vector<unsigned> m(p->L.map.begin(), p->L.map.end());
+    /* slot indices fit 32 bits: conversion refuses layouts beyond 2^32 slots; a symmetric
+     * plan's lower entries (negative: mirrors) are skipped on the device (the caller keeps
+     * the matrix symmetric, as at creation) */
+    std::vector<unsigned> m(p->L.map.size());
+    for (size_t k = 0; k < m.size(); k++) m[k] = p->L.map[k] < 0 ? ~0u : (unsigned)p->L.map[k];
     void *d = nullptr;
     if ((st = ispm_malloc(&d, sizeof(unsigned) * (nnz ? nnz : 1)))) return st;
     if (nnz && (st = ispm_memcpy(d, m.data(), sizeof(unsigned) * nnz, ISPM_H2D))) { ispm_free(d); return st; }

@@ -17,10 +17,13 @@
  *    finishing kernel (alpha, beta by the reference's formulas; the convergence test on the
  *    (r,r) of the previous update) and one fused update p = z + beta p, s = w + beta s,
  *    x += alpha p, r -= alpha s: four launches per iteration instead of seven.
- *  - mixed precision (PCG or single-reduction PCG): FP32 vectors, FP32 A and FP32/BF16
- *    factors; x is accumulated in FP64 and the residual is replaced by the FP64 true
- *    residual (b - A x with the FP32-stored values read into FP64 accumulators) whenever the
- *    FP32 recursive residual has dropped by reliable_delta ("reliable updates").
+ *  - mixed precision (PCG, single-reduction PCG or pipelined PCG): FP32 vectors, FP32 A and
+ *    FP32/BF16 factors; x is accumulated in FP32 between residual replacements and added to
+ *    the FP64 x there, and the residual is replaced by the FP64 true residual (b - A x with
+ *    the FP32-stored values read into FP64 accumulators) whenever the FP32 recursive
+ *    residual has dropped by reliable_delta ("reliable updates").  PCG and PCG1 keep their
+ *    search direction across a replacement (rescaled); the pipelined method restarts its
+ *    recurrences from the replaced residual, as the FP64 solve does when it tightens.
  * All scalars live in device memory in FP64; every reduction is two-phase (block partials,
  * then a one-block finishing kernel) and deterministic.  Iterations are launched in
  * hipGraph-captured batches; once the recursive residual reaches its target a device flag
@@ -593,10 +596,16 @@ int iterate(ispm_solver *S)
   if (S->pipelined) {
     if ((st = apply_M(S, S->w, S->m, sk))) return st;
     if ((st = spmv(S->A, S->T, S->m, S->nv, sk, S->xguard))) return st;
-    hipLaunchKernelGGL(k_ppcg<double>, g, b, 0, sm, (const double *)S->nv, (const double *)S->m,
-                       (double *)S->p, (double *)S->sv, (double *)S->q, (double *)S->z,
-                       (double *)S->xs, (double *)S->r, (double *)S->u, (double *)S->w, S->n, S->s,
-                       S->parts);
+    if (S->T == ISPM_F64)
+      hipLaunchKernelGGL(k_ppcg<double>, g, b, 0, sm, (const double *)S->nv, (const double *)S->m,
+                         (double *)S->p, (double *)S->sv, (double *)S->q, (double *)S->z,
+                         (double *)S->xs, (double *)S->r, (double *)S->u, (double *)S->w, S->n, S->s,
+                         S->parts);
+    else
+      hipLaunchKernelGGL(k_ppcg<float>, g, b, 0, sm, (const float *)S->nv, (const float *)S->m,
+                         (float *)S->p, (float *)S->sv, (float *)S->q, (float *)S->z,
+                         (float *)S->xs, (float *)S->r, (float *)S->u, (float *)S->w, S->n, S->s,
+                         S->parts);
     finish<OP_PPCG>(S, 1);
     ISPM_HIP(hipGetLastError());
     return ISPM_OK;
@@ -768,8 +777,6 @@ int ispm_solver_create(ispm_solver **out, ispm_plan *A, ispm_plan *M1, ispm_plan
   if (!M1 && M2) return fail(ISPM_ERR_INVALID, "solver_create: second factor without the first");
   if (!A->has_dev) return fail(ISPM_ERR_INVALID, "solver_create: A plan has no device matrix");
   const bool mixed = o->precision == ISPM_PREC_MIXED;
-  if (mixed && o->method != ISPM_SOLVER_PCG && o->method != ISPM_SOLVER_PCG1)
-    return fail(ISPM_ERR_UNSUPPORTED, "solver_create: mixed precision uses method PCG or PCG1");
   if (o->method != ISPM_SOLVER_PPCG && o->method != ISPM_SOLVER_PCG && o->method != ISPM_SOLVER_PCG1)
     return fail(ISPM_ERR_INVALID, "solver_create: unknown method %d", o->method);
   if (!mixed && A->E != ISPM_F64) return fail(ISPM_ERR_INVALID, "solver_create: FP64 solve needs an FP64 A plan");
@@ -917,6 +924,25 @@ int solve_core(ispm_solver *S, const double *b, double *x, measure &mc, ispm_sol
   auto setup_recurrence = [&](double tol2) -> int {
     int e;
     scalars &h = *S->hs;
+    if (S->mixed && S->pipelined) {
+      /* r = r64 / ||r64||; u = M r; w = A u; gamma = (r, u); delta = (w, u); the
+       * recurrences restart (z = q = p = s = 0) */
+      hipLaunchKernelGGL(k_cast_scaled<float>, g, bl, 0, sm, (float *)S->r, (const double *)S->r64, n, S->s);
+      ISPM_HIP(hipGetLastError());
+      if ((e = apply_M(S, S->r, S->u, nullptr))) return e;
+      if ((e = spmv(S->A, ISPM_F32, S->u, S->w, nullptr, S->xguard))) return e;
+      if ((e = launch_dot<float>(S, OP_GAMMA, S->r, S->u, 0))) return e;
+      if ((e = launch_dot<float>(S, OP_DELTA, S->w, S->u, 0))) return e;
+      for (void *v : {S->z, S->q, S->p, S->sv, S->xs}) if ((e = zero(v, vb))) return e;
+      const double r64 = h.r64;
+      if ((e = pull(S))) return e;
+      h.rscale = std::sqrt(r64);
+      h.gamma_old = INFINITY;
+      h.alpha_old = 1.0;
+      h.done = 0;
+      h.tol2 = tol2;
+      return push(S);
+    }
     if (S->mixed) {
       /* r = r64 / ||r64||; rho restarts the Krylov sequence (p = 0) */
       hipLaunchKernelGGL(k_cast_scaled<float>, g, bl, 0, sm, (float *)S->r, (const double *)S->r64, n, S->s);
@@ -1022,7 +1048,11 @@ int solve_core(ispm_solver *S, const double *b, double *x, measure &mc, ispm_sol
     if (met(rel)) { res.converged = 1; break; }
     if (out_of_iters || ++guard > 10000) break;
     res.replacements++;
-    if (S->mixed) {
+    if (S->mixed && S->pipelined) {
+      /* restart the recurrences from the replaced residual, at its own scale */
+      rec_target = std::max(S->o.reliable_delta, margin * abs2 / std::sqrt(S->hs->r64));
+      if ((st = setup_recurrence(rec_target * rec_target))) break;
+    } else if (S->mixed) {
       const double rmax_old = S->hs->rscale, rmax = std::sqrt(S->hs->r64);
       /* keep the search direction: rescale p and rho to the new residual scale */
       const double c = rmax_old / rmax;

@@ -180,6 +180,7 @@ _SIGS = {
     "ispm_block_solver_destroy": (None, [C.c_void_p]),
     "ispm_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "ispm_batch_layout": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.c_void_p]),
+    "ispm_batch_matrix_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong)]),
     "ispm_batch_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_void_p, C.c_void_p]),
     "ispm_batch_destroy": (None, [C.c_void_p]),
 }

@@ -65,6 +65,9 @@ class Batch:
         self.row_off = np.zeros(len(systems), np.int64)
         check(lib().ispm_batch_layout(b, C.byref(total), self.row_off.ctypes.data))
         self.rows_padded = total.value
+        mb = C.c_longlong()
+        check(lib().ispm_batch_matrix_bytes(b, C.byref(mb)))
+        self.matrix_bytes = mb.value   # stored values, columns, slice offsets and widths
 
     def pack(self, vecs):
         out = np.zeros(self.rows_padded, np.float32)

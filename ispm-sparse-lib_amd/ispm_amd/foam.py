@@ -16,8 +16,8 @@ from typing import Optional
 
 import numpy as np
 
-from . import (CSR, F32, F64, CsrOut, DeviceArray, Plan, _csr_from_out, check, fastainv_sym, lib,
-               synchronize)
+from . import (CSR, F32, F64, SPMV_PLAN_SYMMETRIC, CsrOut, DeviceArray, IspmError, Plan, _csr_from_out,
+               check, fastainv_sym, lib, synchronize)
 from .solver import PCG, PPCG, Solver
 
 
@@ -121,7 +121,9 @@ class SolverPerformance:
 class FoamPCG:
     """FSAI-PCG for a sequence of lduMatrix systems with one addressing (time steps).
 
-    The A plan is built once (a SlELL plan, whose values the device refresh can replace);
+    The A plan is built once and its values are replaced by the device refresh: a symmetric
+    diagonal plan when the lduMatrix is symmetric and its faces lie on at most 16 diagonals
+    (structured meshes: half the bytes of the general layout), else a SlELL plan;
     `update(ldu)` scatters a new step's coefficients into it on the device (and into the host
     CSR the approximate inverse is computed from).  `solve(source, psi, tolerance, rel_tol)`
     computes the approximate inverse at the tuner's drop tolerance, solves on the device with
@@ -129,7 +131,7 @@ class FoamPCG:
     precision "fp64" or "mixed" (FP32/BF16 iterations, FP64 residuals)."""
 
     def __init__(self, ldu: LduMatrix, precision="fp64", method=None, droptol=None,
-                 log2_range=(-10.0, -3.0), max_elts_per_row=64, maxit=1000):
+                 log2_range=(-10.0, -3.0), max_elts_per_row=64, maxit=1000, symmetric=None):
         self.precision = precision
         self.mixed = precision == "mixed"
         self.method = method if method is not None else (PCG if self.mixed else PPCG)
@@ -138,9 +140,23 @@ class FoamPCG:
         self.csr, self.pos = ldu_to_csr(ldu)
         self.n, self.n_faces = ldu.n, ldu.n_faces
         vt = F32 if self.mixed else F64
-        self.A_plan = Plan(self.csr, vt, elm_type=F32 if self.mixed else F64)
+        et = F32 if self.mixed else F64
+        # symmetric=None: symmetric storage where the matrix allows it
+        if symmetric is None:
+            symmetric = ldu.lower is None
+        elif symmetric and ldu.lower is not None:
+            raise ValueError("symmetric storage needs a symmetric lduMatrix (lower None)")
+        self.A_plan = None
+        if symmetric:
+            try:
+                self.A_plan = Plan(self.csr, vt, elm_type=et, flags=SPMV_PLAN_SYMMETRIC)
+            except IspmError:
+                pass   # more than 16 diagonals: the general layout
+        if self.A_plan is None:
+            self.A_plan = Plan(self.csr, vt, elm_type=et)
+        flags = self.A_plan.flags & SPMV_PLAN_SYMMETRIC
         # the FP64 true residual reads A in FP64 (FP32 storage rounds the coefficients)
-        self.res_plan = Plan(self.csr, F64, elm_type=F64) if self.mixed else None
+        self.res_plan = Plan(self.csr, F64, elm_type=F64, flags=flags) if self.mixed else None
         self.n_pad = self.A_plan.n_rows_padded
         self.d_pos = DeviceArray.from_numpy(self.pos)
         self.d_vals = DeviceArray(self.csr.nnz, np.float64)

@@ -72,9 +72,17 @@ def test_initial_field_already_converged(gpu):
     assert perf2.final_residual == perf2.initial_residual < 1e-6
 
 
-def test_time_steps_reuse_plan_and_probe_droptol(gpu):
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("symmetric", [True, False], ids=["symmetric", "slell"])
+def test_time_steps_reuse_plan_and_probe_droptol(gpu, symmetric, precision):
+    """Three time steps on one addressing: the device refresh of the A plan (symmetric
+    diagonal storage or SlELL) and of the mixed solve's FP64 residual plan, the solves to the
+    normalised tolerance by the oracle's measure, and the tuner's first probes."""
     L = _ldu(m=16, seed=4010)
-    drv = F.FoamPCG(L, precision="fp64")
+    drv = F.FoamPCG(L, precision=precision, symmetric=symmetric)
+    assert bool(drv.A_plan.flags & I.SPMV_PLAN_SYMMETRIC) == symmetric
+    flags = I.SPMV_PLAN_SYMMETRIC if symmetric else 0
+    vt = I.F32 if precision == "mixed" else I.F64
     psi = np.zeros(L.n)
     probes = []
     for step in range(3):
@@ -87,13 +95,20 @@ def test_time_steps_reuse_plan_and_probe_droptol(gpu):
         _, r = O.foam_residual(A, b, psi0, psi)
         assert perf.converged and r < 1e-9
         probes.append(perf.droptol)
-        # the refreshed plan multiplies exactly like a plan built from this step's matrix
+        # the refreshed plans multiply exactly like plans built from this step's matrix
         n = drv.n_pad
-        x = I.DeviceArray.from_numpy(np.concatenate([G.uniform(30 + step, L.n), np.zeros(n - L.n)]))
-        y1, y2 = I.DeviceArray(n), I.DeviceArray(n)
+        dt = np.float32 if vt == I.F32 else np.float64
+        x = I.DeviceArray.from_numpy(np.concatenate([G.uniform(30 + step, L.n), np.zeros(n - L.n)]).astype(dt))
+        y1, y2 = I.DeviceArray(n, dt), I.DeviceArray(n, dt)
         drv.A_plan.execute(x, y1)
-        I.Plan(A, I.F64, elm_type=I.F64).execute(x, y2)
+        I.Plan(A, vt, elm_type=vt, flags=flags).execute(x, y2)
         np.testing.assert_array_equal(y1.numpy(), y2.numpy())
+        if drv.res_plan is not None:
+            x64 = I.DeviceArray.from_numpy(np.concatenate([G.uniform(40 + step, L.n), np.zeros(n - L.n)]))
+            y3, y4 = I.DeviceArray(n), I.DeviceArray(n)
+            drv.res_plan.execute(x64, y3)
+            I.Plan(A, I.F64, elm_type=I.F64, flags=flags).execute(x64, y4)
+            np.testing.assert_array_equal(y3.numpy(), y4.numpy())
     # the first two probes of the golden-section search on [2^-10, 2^-3]
     assert abs(np.log2(probes[0]) - (-10 + 7 * (3 - 5 ** 0.5) / 2)) < 1e-12
     assert probes[1] != probes[0]

@@ -11,7 +11,8 @@ from ispm_amd import solver as S
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = [("fp64", S.PPCG), ("fp64", S.PCG), ("fp64", S.PCG1), ("mixed", S.PCG), ("mixed", S.PCG1)]
+VARIANTS = [("fp64", S.PPCG), ("fp64", S.PCG), ("fp64", S.PCG1), ("mixed", S.PCG), ("mixed", S.PCG1),
+            ("mixed", S.PPCG)]
 
 
 def oracle_relres(A, b, x):
@@ -125,7 +126,7 @@ def test_single_reduction_matches_pcg(gpu, prec):
     assert abs(r_cg.iterations - r_hs.iterations) <= max(3, r_hs.iterations // 20)
 
 
-@pytest.mark.parametrize("method", [S.PCG, S.PCG1])
+@pytest.mark.parametrize("method", [S.PCG, S.PCG1, S.PPCG])
 def test_repeated_solves_reuse_graph(gpu, method):
     """Graph replays across solves and eager work in between give identical results."""
     A = G.poisson7(16)

@@ -357,6 +357,60 @@ def test_symmetric_plan(gpu, name, vt, et):
     check(A, None, x, run_plan(p, A, x, vt), vt, et)
 
 
+def _sym_revalued(A):
+    """A's pattern with new symmetric values: entry (i, j) scaled by a function of
+    (min(i, j), max(i, j))."""
+    rows = np.repeat(np.arange(A.n_rows), np.diff(A.row_ptr))
+    lo, hi = np.minimum(rows, A.cols), np.maximum(rows, A.cols)
+    w = 1.0 + 0.25 * np.cos(0.37 * lo + 0.11 * hi)
+    return I.CSR(A.n_rows, A.n_cols, A.row_ptr, A.cols, A.vals * w)
+
+
+def _products_equal(p, q, n_rows, vt, seed):
+    """p and q give the same bits (x without and, for FP32, with guard zones)."""
+    n = p.n_rows_padded
+    dt = np.float64 if vt == I.F64 else np.float32
+    x = np.concatenate([G.uniform(seed, n_rows), np.zeros(n - n_rows)]).astype(dt)
+    ys = []
+    for pl in (p, q):
+        y = I.DeviceArray(n, dt)
+        pl.execute(I.DeviceArray.from_numpy(x), y)
+        ys.append(y.numpy())
+        if vt == I.F32:
+            g = pl.x_guard()
+            xg = np.zeros(n + 2 * g, dt)
+            xg[g:g + n] = x
+            yg = I.DeviceArray(n, dt)
+            pl.execute_guarded(I.DeviceArray.from_numpy(xg), g, yg)
+            ys.append(yg.numpy())
+    for y in ys[1:]:
+        np.testing.assert_array_equal(y, ys[0])
+
+
+@pytest.mark.parametrize("vt,et", TYPES)
+@pytest.mark.parametrize("name", ["stencil27_contrast", "levels16", "small27"])
+def test_symmetric_refresh(gpu, name, vt, et):
+    """New symmetric values for a symmetric plan's pattern, from the host (ispm_plan_update_coefs)
+    and on the device (ispm_plan_update_coefs_device), multiply bit for bit like a plan built
+    from the new matrix (the planes' repeated rows included: the FP32 guarded kernel reads
+    them); host values whose lower entry differs from its mirror are refused."""
+    A = _sym_matrix(name)
+    A2 = _sym_revalued(A)
+    fresh = I.Plan(A2, vt, elm_type=et, flags=I.SPMV_PLAN_SYMMETRIC)
+    ph = I.Plan(A, vt, elm_type=et, flags=I.SPMV_PLAN_SYMMETRIC)
+    ph.update_coefs(A2)
+    _products_equal(ph, fresh, A.n_rows, vt, 51)
+    pd = I.Plan(A, vt, elm_type=et, flags=I.SPMV_PLAN_SYMMETRIC)
+    pd.update_coefs_device(I.DeviceArray.from_numpy(np.ascontiguousarray(A2.vals, np.float64)))
+    _products_equal(pd, fresh, A.n_rows, vt, 52)
+    rows = np.repeat(np.arange(A.n_rows), np.diff(A.row_ptr))
+    k = int(np.nonzero(A.cols < rows)[0][len(rows) // 3])
+    v3 = A2.vals.copy()
+    v3[k] *= 1.5
+    with pytest.raises(I.IspmError):
+        ph.update_coefs(I.CSR(A.n_rows, A.n_cols, A.row_ptr, A.cols, v3))
+
+
 @pytest.mark.parametrize("name", ["poisson7", "stencil27_contrast", "stencil19_aniso", "small27",
                                   "levels16", "levels131"])
 def test_symmetric_guarded_fp32(gpu, name):
