@@ -58,6 +58,8 @@ struct layout_hints {
   /* streamed mapping: the largest element range (from its first element rounded down to a
    * multiple of 8) of any group of 256*RPT consecutive rows, RPT = 1, 2, 4; 0 = unknown */
   unsigned stream_span[3];
+  /* the same for groups of 64 and 128 consecutive rows */
+  unsigned stream_span_small[2];
 };
 
 /* SpMV with the checks of ispm_spmv, an optional device skip flag (nothing is done while

@@ -300,15 +300,17 @@ void compute_hints(const conv::layout &L, layout_hints &h)
   std::memset(&h, 0, sizeof h);
   if (L.V || L.W != 1 || L.D || L.S == 0 || L.S > 256 || !L.n_slices || L.sp.size() < L.n_slices + 1)
     return;
-  for (unsigned k = 0; k < 3; k++) {
-    const unsigned spg = (256u << k) / L.S;
+  auto span_of = [&](unsigned rows) {
+    const unsigned spg = std::max(1u, rows / L.S);
     unsigned span = 0;
     for (unsigned s0 = 0; s0 < L.n_slices; s0 += spg) {
       const unsigned s1 = std::min(L.n_slices, s0 + spg);
       span = std::max(span, L.sp[s1] - (L.sp[s0] & ~7u));
     }
-    h.stream_span[k] = span;
-  }
+    return span;
+  };
+  for (unsigned k = 0; k < 3; k++) h.stream_span[k] = span_of(256u << k);
+  for (unsigned k = 0; k < 2; k++) h.stream_span_small[k] = L.S <= (64u << k) ? span_of(64u << k) : 0u;
 }
 
 static int finish_plan(ispm_plan *p, const ispm_csr *A, unsigned flags)

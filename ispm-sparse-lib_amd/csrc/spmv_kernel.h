@@ -1207,13 +1207,13 @@ template<typename T> constexpr size_t stream_lds_bytes(unsigned cap, unsigned rp
   return ((size_t)cap * sizeof(T) + 4 * (size_t)(((256u * rpt) >> log2S) + 1) + 15) / 16 * 16;
 }
 
-/* Rows per thread (one for FP64 vectors, two for FP32; ISPM_STREAM_RPT overrides) and the
- * product capacity: the layout's largest group span when known (64 KiB of LDS at most),
- * else 8 (FP64) or 4 (FP32) products per row. */
+/* Rows per thread (one for FP64 vectors, two for FP32, or rpt0 when given; ISPM_STREAM_RPT
+ * overrides) and the product capacity: the layout's largest group span when known (64 KiB
+ * of LDS at most), else 8 (FP64) or 4 (FP32) products per row. */
 template<typename T>
-void stream_shape(const layout_hints *h, unsigned log2S, unsigned &rpt, unsigned &cap)
+void stream_shape(const layout_hints *h, unsigned log2S, unsigned &rpt, unsigned &cap, unsigned rpt0 = 0)
 {
-  rpt = sizeof(T) == 8 ? 1u : 2u;
+  rpt = rpt0 ? rpt0 : sizeof(T) == 8 ? 1u : 2u;
   if (const char *e = std::getenv("ISPM_STREAM_RPT")) {
     const unsigned r = (unsigned)std::atoi(e);
     if ((r == 1 || r == 2 || r == 4) && (256u * r) >= (1u << log2S)) rpt = r;
@@ -1518,7 +1518,10 @@ int launch_split(const ispm_slell_desc *R, const void *diag, const unsigned *rma
     a.n_slices = R->n_slices;
     a.log2S = (unsigned)(31 - __builtin_clz(R->slice_height));
     a.n_cols = (unsigned)R->n_cols;
-    stream_shape<T>(hints, a.log2S, rpt, cap);
+    /* 256-row remainder groups: shorter chains per workgroup than 512 rows, which the
+     * streamed kernel takes for FP32 alone (192^3 FSAI product with the residual update:
+     * 41.8 against 43.5 us per launch; 128 rows 41.1, 64 rows 45.6) */
+    stream_shape<T>(hints, a.log2S, rpt, cap, 1);
     lds = stream_lds_bytes<T>(cap, rpt, a.log2S);
     const unsigned spg = (256u * rpt) >> a.log2S;
     const unsigned groups = (R->n_slices + spg - 1) / spg;
