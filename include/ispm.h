@@ -273,6 +273,13 @@ int ispm_plan_estimate(int vec_type, int elm_type, const ispm_csr *A, unsigned *
 int ispm_plan_time_formats(int vec_type, int elm_type, const ispm_csr *A, int count,
                            const unsigned *S, const unsigned *hblock, const int *V, const int *D,
                            double *seconds);
+/* No reference counterpart (labelling processes that share one GPU): ispm_plan_time_formats
+ * with the layouts converted on `threads` host threads (0: up to 8) and the device held
+ * through an advisory lock, flock(lock_fd, LOCK_EX), while each batch of converted layouts is
+ * uploaded and timed (lock_fd < 0: no lock).  Host conversion runs outside the lock. */
+int ispm_plan_time_formats_shared(int vec_type, int elm_type, const ispm_csr *A, int count,
+                                  const unsigned *S, const unsigned *hblock, const int *V,
+                                  const int *D, double *seconds, int threads, int lock_fd);
 /* The PLAN_EXHAUSTIVE search (plan/plan.h:97-144): times every (S, V, D, hblock) format and
  * launch shape on the device and returns the fastest. */
 int ispm_plan_tune(int vec_type, int elm_type, const ispm_csr *A, unsigned *S, unsigned *hblock,

@@ -16,6 +16,7 @@
  *    plan.h:39-56, without re-converting).
  */
 #include <algorithm>
+#include <cerrno>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -23,6 +24,7 @@
 #include <map>
 #include <thread>
 #include <vector>
+#include <sys/file.h>
 #include "convert.h"
 #include "internal.h"
 #include "ispm.h"
@@ -969,8 +971,25 @@ int ispm_plan_estimate(int vt, int et, const ispm_csr *A, unsigned *S, unsigned 
   return ISPM_OK;
 }
 
+/* Exclusive use of the device for timing, shared between processes by an advisory file
+ * lock (flock on lock_fd); lock_fd < 0 takes nothing. */
+namespace {
+struct device_lock {
+  int fd;
+  explicit device_lock(int f) : fd(f) { if (fd >= 0) while (flock(fd, LOCK_EX) != 0 && errno == EINTR) {} }
+  ~device_lock() { if (fd >= 0) flock(fd, LOCK_UN); }
+};
+}
+
 int ispm_plan_time_formats(int vt, int et, const ispm_csr *A, int count, const unsigned *S,
                            const unsigned *hblock, const int *V, const int *D, double *seconds)
+{
+  return ispm_plan_time_formats_shared(vt, et, A, count, S, hblock, V, D, seconds, 0, -1);
+}
+
+int ispm_plan_time_formats_shared(int vt, int et, const ispm_csr *A, int count, const unsigned *S,
+                                  const unsigned *hblock, const int *V, const int *D, double *seconds,
+                                  int threads, int lock_fd)
 {
   int st = check_csr(A);
   if (st) return st;
@@ -985,25 +1004,33 @@ int ispm_plan_time_formats(int vt, int et, const ispm_csr *A, int count, const u
     ispm_free(x); ispm_free(y); ispm_free(x2); ispm_free(y2);
     return st;
   }
-  /* layouts are converted on host threads, a batch at a time, then uploaded and timed */
-  const int nt = (int)std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+  /* layouts are converted and their values placed on host threads, a batch at a time, then
+   * uploaded and timed (only that part holds the device lock) */
+  const int nt = threads > 0 ? std::min(threads, 64)
+                             : (int)std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
   for (int c0 = 0; c0 < count && !st; c0 += nt) {
     const int nb = std::min(nt, count - c0);
     std::vector<conv::layout> Ls(nb);
+    std::vector<std::vector<unsigned char>> els(nb);
     std::vector<int> ok(nb, 0);
     std::vector<std::thread> th;
     for (int k = 0; k < nb; k++)
-      th.emplace_back([&, k] { ok[k] = !make_layout(A, et, S[c0 + k], hblock[c0 + k], V[c0 + k], D[c0 + k], Ls[k]); });
+      th.emplace_back([&, k] {
+        ok[k] = !make_layout(A, et, S[c0 + k], hblock[c0 + k], V[c0 + k], D[c0 + k], Ls[k]);
+        if (!ok[k]) return;
+        els[k].resize(Ls[k].n_elms * tsize(et) + 1);
+        fill_values(Ls[k], A, et, els[k].data());
+        std::vector<long long>().swap(Ls[k].map);   /* not needed for timing */
+      });
     for (auto &t : th) t.join();
+    device_lock held(lock_fd);   /* uploads and timings of this batch */
     for (int k = 0; k < nb && !st; k++) {
       const int c = c0 + k;
       seconds[c] = INFINITY;
       if (!ok[k]) continue;   /* format not buildable */
       const conv::layout &L = Ls[k];
-      std::vector<unsigned char> el(L.n_elms * tsize(et) + 1);
-      fill_values(L, A, et, el.data());
       ispm_slell_desc d;
-      if ((st = upload_layout(L, el.data(), et, &d))) break;
+      if ((st = upload_layout(L, els[k].data(), et, &d))) break;
       diag_pattern pat;
       unsigned *uni = nullptr;
       bool has_pat = false;

@@ -720,11 +720,21 @@ __global__ void __launch_bounds__((256 / sym_rows_per_thread<T, E>())) symdia_ke
  * alignment, inside the plane thanks to its repeated rows) and about 10 window loads, against
  * about 80 load instructions over two waves above.  A workgroup is one wave with its own LDS
  * windows: no barrier, and LDS and registers are handed on as each wave ends (four waves per
- * group ran 6% slower, 103.6 against 97 us at 27 points 192^3); the planes go in two halves to
- * bound the live registers.  x must be readable, and zero outside [0, n_cols), on every
+ * group ran 6% slower, 103.6 against 97 us at 27 points 192^3).  The windows go from memory
+ * to LDS directly, so all 37 loads are issued before the one wait, and the products are
+ * kept in plane order (a compiler barrier per plane): left free, the compiler read every
+ * plane's x values from LDS before the first product and needed 221 registers (two waves
+ * per SIMD); this way 120 (four).  x must be readable, and zero outside [0, n_cols), on every
  * slice's window covers (the solvers keep zero guard zones around their vectors for this);
  * mirror rows below 0 read as zero. */
 constexpr unsigned kR4Waves = 1, kR4Win = 10;
+
+/* 16 bytes per lane from global memory straight to LDS: lane l writes lds + 16 l (lds is
+ * uniform over the wave); completion is counted with the wave's vector memory loads */
+__device__ __forceinline__ void lds_dma16(const void *g, void *lds)
+{
+  __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void *)lds, 16, 0, 0);
+}
 
 /* the four rows of one lane (see symdia_r4_kernel); x must be readable (and 0) on the
  * window covers of every slice, and mirror rows below 0 are read at row 0 and zeroed */
@@ -735,78 +745,75 @@ symdia_r4_rows(const sym_args &a, const diag_pattern &pat, const float *__restri
 {
   typedef float f4a __attribute__((ext_vector_type(4)));
   typedef float f4u __attribute__((ext_vector_type(4))) __attribute__((aligned(4)));
-  constexpr unsigned PL = kSymPlane, K = NK, KS = K * PL, kR4Half = (NK + 1) / 2;
+  constexpr unsigned PL = kSymPlane, K = NK, KS = K * PL;
   const int row = row0 + 4 * (int)lane;
-  f4a wx[kR4Win];
+  /* windows go straight from memory to LDS (16 bytes per lane, lane l of round q at vector
+   * 64 q + l), so every plane load can be issued before the first wait */
 #pragma unroll
   for (unsigned q = 0; q < kR4Win; q++) {
-    const unsigned v0 = lane + 64 * q, v = v0 < pat.wtotal ? v0 : pat.wtotal - 1;
-    int dl = pat.wdelta[0];
+    const unsigned v = lane + 64 * q;
+    if (v < pat.wtotal) {
+      int dl = pat.wdelta[0];
 #pragma unroll
-    for (unsigned r = 1; r < 9; r++)
-      if (r < pat.nrun && v >= pat.wvbase[r]) dl = pat.wdelta[r];
-    wx[q] = *reinterpret_cast<const f4a *>(xv + (row0 + dl + (int)(4 * v)));
+      for (unsigned r = 1; r < 9; r++)
+        if (r < pat.nrun && v >= pat.wvbase[r]) dl = pat.wdelta[r];
+      lds_dma16(xv + (row0 + dl + (int)(4 * v)), win + 256 * q);
+    }
   }
-  const float *own = el + slice * KS + 4 * lane;
-  f4a ev[kR4Half];
-  f4u mv[kR4Half];
+  /* 32-bit byte offsets from the uniform base (the stored planes are below 4 GiB): one
+   * address register per load */
+  const char *eb = reinterpret_cast<const char *>(el);
+  const unsigned ownb = (slice * KS + 4 * lane) * 4u;
+  f4a ev[NK];
+  f4u mv[NK];
 #pragma unroll
-  for (unsigned k = 0; k < kR4Half; k++) {
-    ev[k] = *reinterpret_cast<const f4a *>(own + k * PL);
+  for (unsigned k = 0; k < NK; k++) {
+    ev[k] = *reinterpret_cast<const f4a *>(eb + (ownb + k * PL * 4u));
     mv[k] = f4u{0.f, 0.f, 0.f, 0.f};
     if (k >= 1) {
       const unsigned j = (unsigned)max(row - a.d[k], 0);
-      mv[k] = *reinterpret_cast<const f4u *>(el + (j >> 8) * KS + k * PL + (j & 255u));
+      f4u m = *reinterpret_cast<const f4u *>(eb + ((j >> 8) * KS + k * PL + (j & 255u)) * 4u);
+      if (row0 < a.d[k]) {   /* only the first slices: uniform over the wave */
+        const int jn = row - a.d[k];
+        if (jn < 0) {
+          /* read rows 0..3 (the clamped start); row jn + e is element jn + e of them */
+          const float m0 = m.x, m1 = m.y, m2 = m.z;
+          m.x = 0.f;
+          m.y = jn == -1 ? m0 : 0.f;
+          m.z = jn == -2 ? m0 : jn == -1 ? m1 : 0.f;
+          m.w = jn == -3 ? m0 : jn == -2 ? m1 : jn == -1 ? m2 : 0.f;
+        }
+      }
+      mv[k] = m;
     }
   }
-#pragma unroll
-  for (unsigned q = 0; q < kR4Win; q++)
-    if (lane + 64 * q < pat.wtotal) reinterpret_cast<f4a *>(win)[lane + 64 * q] = wx[q];
-  f4a ev2[NK - kR4Half];
-  f4u mv2[NK - kR4Half];
-#pragma unroll
-  for (unsigned k = kR4Half; k < NK; k++) {
-    ev2[k - kR4Half] = *reinterpret_cast<const f4a *>(own + k * PL);
-    const unsigned j = (unsigned)max(row - a.d[k], 0);
-    mv2[k - kR4Half] = *reinterpret_cast<const f4u *>(el + (j >> 8) * KS + k * PL + (j & 255u));
-  }
-  /* the wave's own window stores are complete before it reads them */
-  __builtin_amdgcn_s_waitcnt(0xc07f);
+  /* every load, the windows' included, is complete before LDS is read */
+  __builtin_amdgcn_s_waitcnt(0x0f70);
   __builtin_amdgcn_wave_barrier();
   f4a acc = f4a{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (unsigned k = 0; k < NK; k++) {
-    const f4a e = k < kR4Half ? ev[k] : ev2[k - kR4Half];
+    const f4a e = ev[k];
     const float *xu = win + a.up_off[k] + 4 * lane;
     acc.x += e.x * xu[0];
     acc.y += e.y * xu[1];
     acc.z += e.z * xu[2];
     acc.w += e.w * xu[3];
     if (k >= 1) {
-      f4u m = k < kR4Half ? mv[k] : mv2[k - kR4Half];
-      if (row0 < a.d[k]) {   /* only the first slices: uniform over the wave */
-        const int j = row - a.d[k];
-        if (j < 0) {
-          /* read rows 0..3 (the clamped start); row j + e is element j + e of them */
-          const float m0 = m.x, m1 = m.y, m2 = m.z;
-          m.x = 0.f;
-          m.y = j == -1 ? m0 : 0.f;
-          m.z = j == -2 ? m0 : j == -1 ? m1 : 0.f;
-          m.w = j == -3 ? m0 : j == -2 ? m1 : j == -1 ? m2 : 0.f;
-        }
-      }
+      const f4u m = mv[k];
       const float *xl = win + a.lo_off[k] + 4 * lane;
       acc.x += m.x * xl[0];
       acc.y += m.y * xl[1];
       acc.z += m.z * xl[2];
       acc.w += m.w * xl[3];
     }
+    /* keep each plane's products here: later planes' LDS reads are not hoisted above */
+    asm volatile("" : "+v"(acc.x), "+v"(acc.y), "+v"(acc.z), "+v"(acc.w) : : "memory");
   }
   return acc;
 }
 
-/* NK: the number of planes (compile time, so that no plane needs a branch); the first
- * (NK + 1) / 2 planes are loaded with the windows, the rest after the windows went to LDS */
+/* NK: the number of planes (compile time, so that no plane needs a branch) */
 template<unsigned NWAVES, unsigned NK>
 __global__ void __launch_bounds__(64 * NWAVES) symdia_r4_kernel(sym_args a, diag_pattern pat)
 {

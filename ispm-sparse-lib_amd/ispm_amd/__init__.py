@@ -142,6 +142,9 @@ _SIGS = {
     "ispm_csr_multiply": (C.c_int, [C.POINTER(Csr), C.POINTER(Csr), C.c_int, C.POINTER(CsrOut)]),
     "ispm_plan_time_formats": (C.c_int, [C.c_int, C.c_int, C.POINTER(Csr), C.c_int, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ispm_plan_time_formats_shared": (C.c_int, [C.c_int, C.c_int, C.POINTER(Csr), C.c_int, C.c_void_p,
+                                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                                C.c_int]),
     "ispm_mlp_param_count": (C.c_int, [C.c_int]),
     "ispm_mlp_forward": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "ispm_mlp_gradient": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,

@@ -16,16 +16,23 @@ droptol prediction leaves the trained range; the same two procedures produce the
 labels and judge the model on held-out matrices (evaluate()).
 
 Command line (each step resumable; corpus and model files are small .npz):
-    python -m ispm_amd.planner label --first 0 --count 250 --out corpus_a.npz
-    python -m ispm_amd.planner train --corpus corpus_a.npz corpus_b.npz --holdout 60 --out model.npz
-    python -m ispm_amd.planner eval --corpus ... --model model.npz --holdout 60
+    python -m ispm_amd.planner label --first 0 --count 700 --workers 12 --out corpus_a.npz
+    python -m ispm_amd.planner train --corpus corpus_a.npz corpus_b.npz --holdout 200 --out model.npz
+    python -m ispm_amd.planner eval --corpus ... --model model.npz --holdout 200
+Labelling runs `--workers` processes that share the one GPU: matrix generation, features,
+layout conversions and the host factorisations run in parallel, while every upload, timed
+launch and timed solve holds an advisory file lock, so no two timings overlap on the device.
 """
 from __future__ import annotations
 
 import argparse
+import contextlib
 import ctypes as C
+import fcntl
+import functools
 import os
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -36,7 +43,11 @@ from . import gen as G
 N_FEATURES = 32
 HIDDEN = 64
 DEFAULT_DROPTOL = 0.125
-DROPTOL_GRID = (-3.0, -4.0, -5.0, -6.0, -7.0)    # log2 drop tolerances timed for labels
+# log2 drop tolerances timed for labels: the DroptolTuner's bracket [-10, -3]
+# (fastainv/droptol-tuning.h:23), largest first; grid[0] is the default 0.125
+DROPTOL_GRID = tuple(float(-k) for k in range(3, 11))
+CORPUS_SIZE = 2000
+HOLDOUT = 200
 
 
 def _classes():
@@ -113,17 +124,33 @@ def features(A: CSR) -> np.ndarray:
 
 
 # ----------------------------------------------------------------------- GPU labels --
-def time_formats(A: CSR, vt=F64, et=F64) -> np.ndarray:
+def time_formats(A: CSR, vt=F64, et=F64, threads=0, lock_fd=-1) -> np.ndarray:
     """Measured SpMV seconds of every class (default launch of each format; +inf where a
-    format cannot be built): the exhaustive autotune's table over CLASSES."""
+    format cannot be built): the exhaustive autotune's table over CLASSES.  lock_fd >= 0: an
+    open file whose flock() lock is held while layouts are uploaded and timed (processes
+    sharing the GPU); threads: host conversion threads (0: up to 8)."""
     S = np.array([c[0] for c in CLASSES], np.uint32)
     V = np.array([c[1] for c in CLASSES], np.int32)
     D = np.array([c[2] for c in CLASSES], np.int32)
     W = np.array([c[3] for c in CLASSES], np.uint32)
     out = np.full(N_CLASSES, np.inf)
-    check(lib().ispm_plan_time_formats(vt, et, C.byref(A.c()), N_CLASSES, S.ctypes.data, W.ctypes.data,
-                                       V.ctypes.data, D.ctypes.data, out.ctypes.data))
+    check(lib().ispm_plan_time_formats_shared(vt, et, C.byref(A.c()), N_CLASSES, S.ctypes.data,
+                                              W.ctypes.data, V.ctypes.data, D.ctypes.data,
+                                              out.ctypes.data, int(threads), int(lock_fd)))
     return out
+
+
+@contextlib.contextmanager
+def device_lock(fd):
+    """Exclusive use of the GPU among the processes locking the same file (fd < 0: none)."""
+    if fd is None or fd < 0:
+        yield
+        return
+    fcntl.flock(fd, fcntl.LOCK_EX)
+    try:
+        yield
+    finally:
+        fcntl.flock(fd, fcntl.LOCK_UN)
 
 
 def plan_target(times, tau: float = 8.0) -> np.ndarray:
@@ -136,47 +163,77 @@ def plan_target(times, tau: float = 8.0) -> np.ndarray:
     return (e / e.sum()).astype(np.float32)
 
 
-def droptol_times(A: CSR, grid=DROPTOL_GRID, tol: float = 1e-8, seed: int = 77):
+def droptol_times(A: CSR, grid=DROPTOL_GRID, tol: float = 1e-8, seed: int = 77, lock_fd=-1,
+                  max_setup: float = 120.0):
     """Seconds of factor set-up (fastainv_sym on the host) plus FP64 FSAI-PCG solve on the GPU
-    for each log2 drop tolerance of the grid (solve timed after one warm-up solve)."""
+    (timed after one warm-up solve) at each log2 drop tolerance of the grid, largest first.
+
+    Returns (total, setup, iters), one entry per grid point.  Set-up time grows as the drop
+    tolerance falls (more fill), so once a factorisation alone takes at least the best total so
+    far, no smaller drop tolerance can win: the search stops there (also past `max_setup`
+    seconds).  Entries not solved have total = +inf; their `setup` holds the last measured
+    set-up (a lower bound of their own) and iters = 0.  A solve that does not converge has
+    total = +inf and iters = -1."""
     from .solver import PCG, Solver
     xt = G.uniform(seed, A.n_rows)
     b = np.asarray(A.to_scipy() @ xt)
-    out = np.full(len(grid), np.inf)
+    total = np.full(len(grid), np.inf)
+    setup = np.full(len(grid), np.nan)
+    iters = np.zeros(len(grid), np.int32)
+    best, A_plan, bound = np.inf, None, np.nan
     for k, lg in enumerate(grid):
         t0 = time.perf_counter()
         M = fastainv_sym(A, F64, max_elts=64 * A.n_rows, droptol=2.0 ** lg)
-        setup = time.perf_counter() - t0
-        sol = Solver(A, M=M, precision="fp64", method=PCG, tol=tol, maxit=20000, fsai_product=False)
-        n = sol.n
-        bd = DeviceArray.from_numpy(np.concatenate([b, np.zeros(n - A.n_rows)]))
-        xd = DeviceArray(n)
-        sol.solve_device(bd, xd, allow_fail=True)
-        xd.zero()
-        synchronize()
-        t0 = time.perf_counter()
-        st = sol.solve_device(bd, xd, allow_fail=True)
-        synchronize()
+        setup[k] = bound = time.perf_counter() - t0
+        if setup[k] >= best:
+            break
+        with device_lock(lock_fd):
+            sol = Solver(A, M=M, precision="fp64", method=PCG, tol=tol, maxit=20000, fsai_product=False,
+                         A_plan=A_plan)
+            A_plan = sol.A_plan
+            n = sol.n
+            bd = DeviceArray.from_numpy(np.concatenate([b, np.zeros(n - A.n_rows)]))
+            xd = DeviceArray(n)
+            sol.solve_device(bd, xd, allow_fail=True)
+            xd.zero()
+            synchronize()
+            t0 = time.perf_counter()
+            st = sol.solve_device(bd, xd, allow_fail=True)
+            synchronize()
+            ts = time.perf_counter() - t0
+            del sol, bd, xd
         if st.converged:
-            out[k] = setup + time.perf_counter() - t0
-    return out
+            total[k] = setup[k] + ts
+            iters[k] = st.iterations
+            best = min(best, total[k])
+        else:
+            iters[k] = -1
+        if setup[k] > max_setup:
+            break
+    setup[np.isnan(setup)] = bound
+    return total, setup, iters
 
 
 # ----------------------------------------------------------------------------- corpus --
 def corpus_spec(i: int) -> dict:
-    """Deterministic description of corpus matrix i: 7/19/27-point stencils (5e4..1e6 rows,
-    z anisotropy 1..100 and coefficient contrast 1..1e6, all log-uniform) and random
-    geometric graph Laplacians (3e4..3e5 rows, 10..26 neighbours)."""
+    """Deterministic description of corpus matrix i: rows log-uniform in [1e4, 2e6]; 3 in 4
+    are 7/19/27-point stencils on boxes of random aspect (z anisotropy 1..100 and coefficient
+    contrast 1..1e6, log-uniform), the rest random geometric kNN graph Laplacians with 10..26
+    neighbours."""
     u = G.uniform01(900001 + i, 8)
+    n = int(10 ** (4 + u[2] * np.log10(200.0)))
     if u[0] < 0.75:
         pts = (7, 19, 27)[min(2, int(u[1] * 3))]
-        n = int(10 ** (4.7 + u[2] * 1.3))
         nx = max(8, int(round(n ** (1 / 3) * (0.7 + 0.6 * u[3]))))
         ny = max(8, int(round(n ** (1 / 3) * (0.7 + 0.6 * u[4]))))
         nz = max(8, n // (nx * ny))
         return dict(kind="stencil", nx=nx, ny=ny, nz=nz, points=pts, anisotropy=float(10 ** (2 * u[5])),
                     contrast=float(10 ** (6 * u[6])), seed=1000 + i)
-    return dict(kind="rgg", n=int(10 ** (4.5 + u[2])), k=10 + min(16, int(u[3] * 17)), seed=2000 + i)
+    return dict(kind="rgg", n=n, k=10 + min(16, int(u[3] * 17)), seed=2000 + i)
+
+
+def spec_rows(spec: dict) -> int:
+    return spec["nx"] * spec["ny"] * spec["nz"] if spec["kind"] == "stencil" else spec["n"]
 
 
 def make_matrix(spec: dict) -> CSR:
@@ -186,42 +243,110 @@ def make_matrix(spec: dict) -> CSR:
     return G.rgg_laplacian(spec["n"], spec["k"], spec["seed"])
 
 
-def label(first: int, count: int, out: str, log=print, droptol=True):
+_WORKER = {}
+
+
+def _worker_init(lock_path, threads):
+    from . import init
+    init()
+    _WORKER["fd"] = os.open(lock_path, os.O_RDWR | os.O_CREAT, 0o600) if lock_path else -1
+    _WORKER["threads"] = threads
+
+
+def label_one(i: int, droptol=True, lock_fd=None, threads=None):
+    """(i, features, format times, droptol totals, set-up seconds, iterations, log line) of
+    corpus matrix i."""
+    lock_fd = _WORKER.get("fd", -1) if lock_fd is None else lock_fd
+    threads = _WORKER.get("threads", 0) if threads is None else threads
+    t0 = time.perf_counter()
+    A = make_matrix(corpus_spec(i))
+    t1 = time.perf_counter()
+    x = features(A)
+    t2 = time.perf_counter()
+    T = time_formats(A, threads=threads, lock_fd=lock_fd)
+    t3 = time.perf_counter()
+    if droptol:
+        DT, DS, DI = droptol_times(A, lock_fd=lock_fd)
+    else:
+        DT = np.full(len(DROPTOL_GRID), np.inf)
+        DS = np.full(len(DROPTOL_GRID), np.nan)
+        DI = np.zeros(len(DROPTOL_GRID), np.int32)
+    dbest = f"2^{DROPTOL_GRID[int(np.argmin(DT))]:.0f}" if np.isfinite(DT).any() else "n/a"
+    line = (f"matrix {i}: n={A.n_rows} nnz={A.nnz} best {CLASSES[int(np.argmin(T))]} "
+            f"{np.min(T) * 1e6:.1f} us, droptol {dbest} ({int((DI != 0).sum())} solved) "
+            f"[{time.perf_counter() - t0:.1f}s: matrix {t1 - t0:.1f}, features {t2 - t1:.1f}, "
+            f"formats {t3 - t2:.1f}, droptol {time.perf_counter() - t3:.1f}]")
+    return i, x, T, DT, DS, DI, line
+
+
+def _save(out, rows):
+    rows = sorted(rows, key=lambda r: r[0])
+    tmp = out + ".tmp.npz"
+    np.savez(tmp, X=np.array([r[1] for r in rows], np.float32), T=np.array([r[2] for r in rows]),
+             DT=np.array([r[3] for r in rows]), DS=np.array([r[4] for r in rows]),
+             DI=np.array([r[5] for r in rows], np.int32), idx=np.array([r[0] for r in rows]),
+             droptol_grid=np.array(DROPTOL_GRID))
+    os.replace(tmp, out)      # a run stopped mid-write leaves the previous file whole
+
+
+def label(first: int, count: int, out: str, log=print, droptol=True, workers=1, threads=0,
+          deadline=None):
     """Features and measured labels of corpus matrices first .. first+count-1, saved to `out`
-    after every matrix (a partial file is resumed)."""
-    X, T, DT, idx = [], [], [], []
+    after every matrix (a partial file is resumed).  workers > 1: that many processes share
+    the GPU through a file lock (largest matrices first, so the tail is short).  deadline:
+    seconds after which no further result is awaited (unfinished matrices stay unlabelled)."""
+    rows = []
     if os.path.exists(out):
         z = np.load(out, allow_pickle=False)
-        X, T, DT, idx = list(z["X"]), list(z["T"]), list(z["DT"]), list(z["idx"])
-    done = set(int(i) for i in idx)
-    for i in range(first, first + count):
-        if i in done:
-            continue
-        t0 = time.perf_counter()
-        A = make_matrix(corpus_spec(i))
-        X.append(features(A))
-        T.append(time_formats(A))
-        DT.append(droptol_times(A) if droptol else np.full(len(DROPTOL_GRID), np.nan))
-        idx.append(i)
-        np.savez(out, X=np.array(X, np.float32), T=np.array(T), DT=np.array(DT), idx=np.array(idx))
-        dt = DT[-1]
-        dbest = f"2^{DROPTOL_GRID[int(np.nanargmin(dt))]:.0f}" if np.isfinite(dt).any() else "n/a"
-        log(f"matrix {i}: n={A.n_rows} nnz={A.nnz} best {CLASSES[int(np.argmin(T[-1]))]} "
-            f"{np.min(T[-1]) * 1e6:.1f} us, droptol {dbest} [{time.perf_counter() - t0:.1f}s]")
+        rows = [(int(z["idx"][k]), z["X"][k], z["T"][k], z["DT"][k], z["DS"][k], z["DI"][k])
+                for k in range(z["idx"].size)]
+    done = set(r[0] for r in rows)
+    todo = [i for i in range(first, first + count) if i not in done]
+    todo.sort(key=lambda i: -spec_rows(corpus_spec(i)))
+    t0 = time.perf_counter()
+    if workers <= 1:
+        from . import init
+        init()
+        results = (label_one(i, droptol, -1, threads) for i in todo)
+        pool = None
+    else:
+        import multiprocessing as mp
+        lock_path = os.path.join(tempfile.gettempdir(), f"ispm_label_{os.getpid()}.lock")
+        pool = mp.get_context("spawn").Pool(workers, initializer=_worker_init,
+                                             initargs=(lock_path, threads))
+        results = pool.imap_unordered(functools.partial(label_one, droptol=droptol), todo)
+    stopped = False
+    try:
+        for r in results:
+            rows.append(r[:6])
+            _save(out, rows)
+            log(f"{r[6]} ({len(rows)} labelled, {time.perf_counter() - t0:.0f}s)")
+            if deadline is not None and time.perf_counter() - t0 > deadline:
+                stopped = True
+                log(f"deadline reached: {len(rows)} labelled, the rest left for a later run")
+                break
+    finally:
+        if pool is not None:
+            if stopped:
+                pool.terminate()
+            else:
+                pool.close()
+            pool.join()
     return out
 
 
 def load_corpus(paths):
-    X, T, DT, idx = [], [], [], []
+    """(X, T, DT, DS, DI, idx) of the corpus files, ordered by matrix index."""
+    parts = {k: [] for k in ("X", "T", "DT", "DS", "DI", "idx")}
     for p in paths:
         z = np.load(p, allow_pickle=False)
-        X.append(z["X"])
-        T.append(z["T"])
-        DT.append(z["DT"])
-        idx.append(z["idx"])
-    idx = np.concatenate(idx)
+        if "droptol_grid" not in z.files or not np.array_equal(z["droptol_grid"], np.array(DROPTOL_GRID)):
+            raise ValueError(f"{p}: labelled over another drop-tolerance grid")
+        for k in parts:
+            parts[k].append(z[k])
+    idx = np.concatenate(parts["idx"])
     order = np.argsort(idx)
-    return (np.concatenate(X)[order], np.concatenate(T)[order], np.concatenate(DT)[order], idx[order])
+    return tuple(np.concatenate(parts[k])[order] for k in ("X", "T", "DT", "DS", "DI")) + (idx[order],)
 
 
 def targets(T, DT, tau: float = 8.0):
@@ -375,11 +500,20 @@ def best_fixed_class(T) -> int:
     return int(np.argmin(np.exp(np.log(T / T.min(1, keepdims=True)).mean(0))))
 
 
-def evaluate(model, X, T, DT, confidence=None, fixed_class=None):
+def droptol_cost(DT, DS, DI):
+    """Per grid point: the measured total where solved; the set-up lower bound where the
+    search stopped before solving (the total there is at least that, and above the best);
+    +inf where the solve did not converge."""
+    return np.where(np.isfinite(DT), DT, np.where(DI == 0, DS, np.inf))
+
+
+def evaluate(model, X, T, DT=None, DS=None, DI=None, confidence=None, fixed_class=None):
     """Held-out quality: for the plan head, the time of the chosen class (model when
     confident, else exhaustive) over the exhaustive best, and of the model's top class alone
     (and of one fixed class, when given); for the droptol head, set-up + solve time at the grid
-    point nearest the prediction over that at the default 0.125 and over the best grid point."""
+    point nearest the prediction over that at the default 0.125 and over the best grid point,
+    over all matrices and over the slice whose best grid point is not the default (a
+    prediction on a point the label search skipped counts its set-up lower bound)."""
     p, lg = model.predict(X)
     confidence = model.confidence if confidence is None else confidence
     res = {"n": int(X.shape[0]), "confidence_threshold": confidence}
@@ -398,16 +532,34 @@ def evaluate(model, X, T, DT, confidence=None, fixed_class=None):
     res["plan_choice_ratio_gmean"] = float(np.exp(np.mean(np.log(t_choice / best))))
     res["plan_choice_within_5pct"] = float(np.mean(t_choice <= 1.05 * best))
     res["plan_confident_fraction"] = float(np.mean(conf))
+    if DT is None:
+        return res
+    if DS is None:
+        DS = np.where(np.isfinite(DT), 0.0, np.inf)
+    if DI is None:
+        DI = np.where(np.isfinite(DT), 1, -1)
     grid = np.array(DROPTOL_GRID)
-    ok = np.isfinite(DT).all(1)
+    cost = droptol_cost(DT, DS, DI)
     k = np.abs(np.clip(lg, grid.min(), grid.max())[:, None] - grid[None, :]).argmin(1)
-    t_pred = DT[np.arange(len(k)), k]
-    t_def = DT[:, 0]                 # grid[0] = log2(0.125), the default
+    rows = np.arange(len(k))
+    t_pred = cost[rows, k]
+    t_def = cost[:, 0]                 # grid[0] = log2(0.125), the default
+    t_best = np.min(np.where(np.isfinite(DT), DT, np.inf), 1)
+    ok = np.isfinite(t_def) & np.isfinite(t_best) & np.isfinite(t_pred)
+    res["droptol_n"] = int(ok.sum())
+    res["droptol_pred_not_converged"] = int((np.isfinite(t_best) & ~np.isfinite(t_pred)).sum())
+    res["droptol_pred_exact_grid_point"] = float(np.mean(k == np.argmin(np.where(np.isfinite(DT), DT, np.inf), 1)))
     if ok.any():
-        t_best = DT[ok].min(1)
-        res["droptol_pred_over_default_gmean"] = float(np.exp(np.mean(np.log(t_pred[ok] / t_def[ok]))))
-        res["droptol_pred_over_best_gmean"] = float(np.exp(np.mean(np.log(t_pred[ok] / t_best))))
-        res["droptol_default_over_best_gmean"] = float(np.exp(np.mean(np.log(t_def[ok] / t_best))))
+        gm = lambda a: float(np.exp(np.mean(np.log(a))))
+        res["droptol_pred_over_default_gmean"] = gm(t_pred[ok] / t_def[ok])
+        res["droptol_pred_over_best_gmean"] = gm(t_pred[ok] / t_best[ok])
+        res["droptol_default_over_best_gmean"] = gm(t_def[ok] / t_best[ok])
+        sl = ok & (np.argmin(np.where(np.isfinite(DT), DT, np.inf), 1) != 0)
+        res["droptol_slice_n"] = int(sl.sum())
+        if sl.any():
+            res["droptol_slice_pred_over_default_gmean"] = gm(t_pred[sl] / t_def[sl])
+            res["droptol_slice_default_over_best_gmean"] = gm(t_def[sl] / t_best[sl])
+            res["droptol_slice_pred_over_default_max"] = float(np.max(t_pred[sl] / t_def[sl]))
     return res
 
 
@@ -420,27 +572,32 @@ def main(argv=None):
     a.add_argument("--count", type=int, default=100)
     a.add_argument("--out", required=True)
     a.add_argument("--no-droptol", action="store_true")
+    a.add_argument("--workers", type=int, default=1)
+    a.add_argument("--threads", type=int, default=0, help="host conversion threads per process")
+    a.add_argument("--deadline", type=float, default=None, help="seconds; stop awaiting results after")
     t = sub.add_parser("train")
     t.add_argument("--corpus", nargs="+", required=True)
-    t.add_argument("--holdout", type=int, default=60)
+    t.add_argument("--holdout", type=int, default=HOLDOUT)
     t.add_argument("--epochs", type=int, default=400)
     t.add_argument("--tau", type=float, default=8.0)
     t.add_argument("--lr", type=float, default=3e-3)
     t.add_argument("--out", default=MODEL_PATH)
     e = sub.add_parser("eval")
     e.add_argument("--corpus", nargs="+", required=True)
-    e.add_argument("--holdout", type=int, default=60)
+    e.add_argument("--holdout", type=int, default=HOLDOUT)
     e.add_argument("--model", default=MODEL_PATH)
     args = ap.parse_args(argv)
-    from . import init
-    init()
 
     def log(*x):
         print(*x, flush=True)
     if args.cmd == "label":
-        label(args.first, args.count, args.out, log=log, droptol=not args.no_droptol)
+        # the processes that time on the GPU initialise it themselves; this one stays off it
+        label(args.first, args.count, args.out, log=log, droptol=not args.no_droptol,
+              workers=args.workers, threads=args.threads, deadline=args.deadline)
         return 0
-    X, T, DT, idx = load_corpus(args.corpus)
+    from . import init
+    init()
+    X, T, DT, DS, DI, idx = load_corpus(args.corpus)
     ntr = X.shape[0] - args.holdout
     if args.cmd == "train":
         Tp, Mp, Td, Md = targets(T, DT, args.tau)
@@ -449,11 +606,12 @@ def main(argv=None):
         model.calibrate(X[:ntr])
         model.save(args.out)
         fixed = best_fixed_class(T[:ntr])
-        log("train", evaluate(model, X[:ntr], T[:ntr], DT[:ntr], fixed_class=fixed))
-        log("holdout", evaluate(model, X[ntr:], T[ntr:], DT[ntr:], fixed_class=fixed))
+        log("train", evaluate(model, X[:ntr], T[:ntr], DT[:ntr], DS[:ntr], DI[:ntr], fixed_class=fixed))
+        log("holdout", evaluate(model, X[ntr:], T[ntr:], DT[ntr:], DS[ntr:], DI[ntr:], fixed_class=fixed))
         return 0
     model = Model.load(args.model)
-    log("holdout", evaluate(model, X[ntr:], T[ntr:], DT[ntr:], fixed_class=best_fixed_class(T[:ntr])))
+    log("holdout", evaluate(model, X[ntr:], T[ntr:], DT[ntr:], DS[ntr:], DI[ntr:],
+                            fixed_class=best_fixed_class(T[:ntr])))
     return 0
 
 

@@ -68,11 +68,62 @@ def test_plan_target_and_targets():
     q = PL.plan_target(t)
     assert q.sum() == pytest.approx(1.0, rel=1e-6)
     assert q.argmax() == 1 and q[2] == 0
-    DT = np.array([[1.0, 0.5, 0.7, np.inf, 0.9]])
+    G_ = len(PL.DROPTOL_GRID)
+    DT = np.array([[1.0, 0.5, 0.7, np.inf, 0.9] + [np.inf] * (G_ - 5)])
     Tp, Mp, Td, Md = PL.targets(t[None, :], DT)
     assert Mp[0] == 1 and Md[0] == 1 and Td[0] == PL.DROPTOL_GRID[1]
-    Tp, Mp, Td, Md = PL.targets(t[None, :], np.full((1, 5), np.inf))
+    Tp, Mp, Td, Md = PL.targets(t[None, :], np.full((1, G_), np.inf))
     assert Md[0] == 0
+
+
+def test_droptol_grid_is_the_tuner_bracket():
+    # DroptolTuner searches log2 droptol in [-10, -3] (fastainv/droptol-tuning.h:23); the
+    # default 0.125 (fastainv/fastainv.h:14) is the first grid point
+    assert PL.DROPTOL_GRID[0] == -3.0 and PL.DROPTOL_GRID[-1] == -10.0
+    assert 2.0 ** PL.DROPTOL_GRID[0] == PL.DEFAULT_DROPTOL
+    assert all(a > b for a, b in zip(PL.DROPTOL_GRID, PL.DROPTOL_GRID[1:]))
+
+
+class _FixedModel:
+    """predict() stand-in: class probabilities and log2 droptol given per matrix."""
+    confidence = 0.5
+
+    def __init__(self, cls, lg):
+        self.cls, self.lg = np.asarray(cls), np.asarray(lg, float)
+
+    def predict(self, X):
+        p = np.full((len(self.cls), C_), 0.1 / (C_ - 1))
+        p[np.arange(len(self.cls)), self.cls] = 0.9
+        return p, self.lg
+
+
+def test_evaluate_hand_case_with_pruned_and_failed_points():
+    G_ = len(PL.DROPTOL_GRID)
+    inf = np.inf
+    # matrix 0: default best (1.0); search stopped after 2^-4 (its set-up 2.0 >= best 1.0)
+    # matrix 1: 2^-4 best (2.0 vs 4.0 at the default); stopped at 2^-5 (set-up 2.5)
+    # matrix 2: default did not converge; 2^-4 best (3.0)
+    DT = np.full((3, G_), inf)
+    DS = np.zeros((3, G_))
+    DI = np.zeros((3, G_), np.int32)
+    DT[0, 0], DS[0, :] = 1.0, 2.0
+    DS[0, 0], DI[0, 0] = 0.5, 10
+    DT[1, :2], DS[1, :] = (4.0, 2.0), 2.5
+    DS[1, :2], DI[1, :2] = (0.5, 1.0), (100, 20)
+    DT[2, 1], DS[2, :2], DI[2, :2] = 3.0, (0.5, 1.5), (-1, 30)
+    cost = PL.droptol_cost(DT, DS, DI)
+    assert cost[0, 1] == 2.0 and cost[1, 2] == 2.5 and cost[2, 0] == inf
+    T = np.ones((3, C_))
+    T[:, 0] = 0.5
+    # predictions: matrix 0 -> 2^-4 (pruned: lower bound 2.0), matrix 1 -> 2^-4, matrix 2 -> 2^-4
+    res = PL.evaluate(_FixedModel([0, 0, 1], [-4.1, -3.9, -4.0]), np.zeros((3, 32)), T, DT, DS, DI)
+    assert res["plan_top_ratio_gmean"] == pytest.approx(2.0 ** (1 / 3))
+    assert res["droptol_n"] == 2                      # matrix 2 has no finite default
+    assert res["droptol_pred_over_default_gmean"] == pytest.approx(np.sqrt(2.0 * 0.5))
+    assert res["droptol_default_over_best_gmean"] == pytest.approx(np.sqrt(1.0 * 2.0))
+    assert res["droptol_slice_n"] == 1
+    assert res["droptol_slice_pred_over_default_gmean"] == pytest.approx(0.5)
+    assert res["droptol_pred_exact_grid_point"] == pytest.approx(2 / 3)
 
 
 def test_features_known_values():
@@ -91,6 +142,21 @@ def test_corpus_specs_deterministic():
     assert a == b
     kinds = {s["kind"] for s in a}
     assert kinds == {"stencil", "rgg"}
+
+
+def test_corpus_spans_the_stated_ranges():
+    specs = [PL.corpus_spec(i) for i in range(PL.CORPUS_SIZE)]
+    rows = np.array([PL.spec_rows(s) for s in specs])
+    assert rows.min() >= 0.9e4 and rows.max() <= 2e6
+    assert rows.max() > 1.5e6 and rows.min() < 1.5e4
+    st = [s for s in specs if s["kind"] == "stencil"]
+    assert {s["points"] for s in st} == {7, 19, 27}
+    an = np.array([s["anisotropy"] for s in st])
+    co = np.array([s["contrast"] for s in st])
+    assert an.min() >= 1 and an.max() <= 100 and an.max() > 50
+    assert co.min() >= 1 and co.max() <= 1e6 and co.max() > 1e5
+    k = [s["k"] for s in specs if s["kind"] == "rgg"]
+    assert min(k) == 10 and max(k) == 26
 
 
 # ------------------------------------------------------------------------------ GPU --
@@ -187,6 +253,42 @@ def test_gpu_time_formats_table(gpu):
 
 
 @pytest.mark.gpu
+def test_gpu_droptol_labels_stop_when_setup_exceeds_best(gpu):
+    A = G.stencil(24, 24, 24, 7, anisotropy=4, contrast=100, seed=12)
+    DT, DS, DI = PL.droptol_times(A)
+    G_ = len(PL.DROPTOL_GRID)
+    assert DT.shape == DS.shape == DI.shape == (G_,)
+    assert np.isfinite(DT[0]) and DI[0] > 0                 # the default converges
+    solved = DI != 0
+    last = int(np.nonzero(solved)[0].max())
+    assert np.all(solved[:last + 1])                        # solved in grid order, then stopped
+    best = np.min(DT[solved])
+    if last + 1 < G_:
+        # the stopping factorisation took at least the best total; later points carry it
+        assert DS[last + 1] >= best and np.all(DS[last + 1:] == DS[last + 1])
+    assert np.all(DT[solved] >= DS[solved])
+
+
+@pytest.mark.gpu
+def test_gpu_label_one_and_shared_time_formats(gpu, tmp_path):
+    import fcntl
+    fd = os.open(str(tmp_path / "lock"), os.O_RDWR | os.O_CREAT)
+    try:
+        A = G.stencil(16, 16, 16, 19, seed=4)
+        t = PL.time_formats(A, threads=2, lock_fd=fd)
+        ok = np.isfinite(t)
+        assert ok.sum() >= C_ - 3 and (t[ok] > 0).all()
+        fcntl.flock(fd, fcntl.LOCK_EX | fcntl.LOCK_NB)     # released after the timings
+        fcntl.flock(fd, fcntl.LOCK_UN)
+    finally:
+        os.close(fd)
+    small = min(range(PL.CORPUS_SIZE), key=lambda i: PL.spec_rows(PL.corpus_spec(i)))
+    i, x, T, DT, DS, DI, line = PL.label_one(small, lock_fd=-1, threads=2)
+    assert i == small and x.shape == (32,) and T.shape == (C_,) and np.isfinite(T).any()
+    assert np.isfinite(DT).any() and f"matrix {small}:" in line
+
+
+@pytest.mark.gpu
 @pytest.mark.skipif(not os.path.exists(PL.MODEL_PATH), reason="no trained model in the package")
 def test_gpu_committed_model_predicts(gpu):
     model = PL.load_default()
@@ -204,11 +306,14 @@ def test_gpu_committed_model_beats_fixed_format_on_holdout(gpu):
     network picks one format for all matrices; the exhaustive search pays for 39 conversions
     and timings per matrix)."""
     data = os.path.join(os.path.dirname(PL.MODEL_PATH), "planner_data")
-    paths = sorted(os.path.join(data, f) for f in os.listdir(data) if f.endswith(".npz"))
-    X, T, DT, idx = PL.load_corpus(paths)
-    hold = 50
+    paths = sorted(os.path.join(data, f) for f in os.listdir(data) if f.endswith(".npz")) \
+        if os.path.isdir(data) else []
+    if not paths:
+        pytest.skip("no labelled corpus in the package")
+    X, T, DT, DS, DI, idx = PL.load_corpus(paths)
+    hold = PL.HOLDOUT
     model = PL.load_default()
-    res = PL.evaluate(model, X[-hold:], T[-hold:], DT[-hold:],
+    res = PL.evaluate(model, X[-hold:], T[-hold:], DT[-hold:], DS[-hold:], DI[-hold:],
                       fixed_class=PL.best_fixed_class(T[:-hold]))
     assert res["plan_top_ratio_gmean"] < res["plan_fixed_ratio_gmean"]
     assert res["plan_top_ratio_gmean"] <= 1.05
