@@ -375,8 +375,9 @@ def _dp(a: DeviceArray, off_elems: int = 0):
 class Model:
     """Parameters (flat FP32, the layout of ispm_mlp_*) and the feature normalisation."""
 
-    def __init__(self, params=None, mean=None, std=None, seed=0, confidence=0.3):
+    def __init__(self, params=None, mean=None, std=None, seed=0, confidence=0.3, droptol_offset=0.0):
         self.confidence = float(confidence)
+        self.droptol_offset = float(droptol_offset)
         self.n_params = lib().ispm_mlp_param_count(N_CLASSES)
         if params is None:
             rng = np.random.default_rng(seed)
@@ -405,12 +406,13 @@ class Model:
         return out.numpy().reshape(B, N_CLASSES + 1)
 
     def predict(self, X):
-        """(class probabilities B x C, log2 droptol B)."""
+        """(class probabilities B x C, log2 droptol B: the regression output plus the
+        calibrated offset)."""
         z = self.forward(X)
         zc = z[:, :N_CLASSES].astype(np.float64)
         p = np.exp(zc - zc.max(1, keepdims=True))
         p /= p.sum(1, keepdims=True)
-        return p, z[:, N_CLASSES]
+        return p, z[:, N_CLASSES] + self.droptol_offset
 
     def fit(self, X, Tp, Mp, Td, Md, epochs=400, batch=64, lr=3e-3, drop_weight=0.5, seed=0, log=None):
         """Adam on the GPU (ispm_mlp_train_step); the shuffled set is uploaded once per epoch
@@ -451,17 +453,42 @@ class Model:
         self.confidence = float(np.quantile(p.max(1), quantile))
         return self.confidence
 
+    def calibrate_droptol(self, X, DT, DS, DI, offsets=np.arange(-1.0, 1.0001, 0.05)):
+        """Offset added to the regression output before it is rounded to a grid point: the
+        one (smallest in size among equals) that minimises the geometric mean of set-up +
+        solve time over the best grid point on X.  The regression is trained towards grid
+        labels that are mostly the default, so its output for the rest lies between grid
+        points; the offset places the rounding threshold where the measured cost says."""
+        self.droptol_offset = 0.0
+        _, lg = self.predict(X)
+        grid = np.array(DROPTOL_GRID)
+        cost = droptol_cost(DT, DS, DI)
+        best = np.min(np.where(np.isfinite(DT), DT, np.inf), 1)
+        ok = np.isfinite(best)
+        rows = np.arange(len(lg))
+        res = []
+        for off in offsets:
+            k = np.abs(np.clip(lg + off, grid.min(), grid.max())[:, None] - grid[None, :]).argmin(1)
+            r = np.minimum(cost[rows, k] / best, 1e3)
+            res.append((float(np.exp(np.mean(np.log(r[ok])))), abs(float(off)), float(off)))
+        self.droptol_offset = min(res)[2]
+        return self.droptol_offset
+
     def save(self, path):
         np.savez(path, params=self.params, mean=self.mean, std=self.std, confidence=self.confidence,
-                 classes=np.array(CLASSES, np.int32), droptol_grid=np.array(DROPTOL_GRID))
+                 droptol_offset=self.droptol_offset, classes=np.array(CLASSES, np.int32),
+                 droptol_grid=np.array(DROPTOL_GRID))
 
     @classmethod
     def load(cls, path):
         z = np.load(path, allow_pickle=False)
         if not np.array_equal(z["classes"], np.array(CLASSES, np.int32)):
             raise ValueError("the model was trained on another class list")
+        if "droptol_grid" in z.files and not np.array_equal(z["droptol_grid"], np.array(DROPTOL_GRID)):
+            raise ValueError("the model was trained on another drop-tolerance grid")
         conf = float(z["confidence"]) if "confidence" in z.files else 0.3
-        return cls(z["params"], z["mean"], z["std"], confidence=conf)
+        off = float(z["droptol_offset"]) if "droptol_offset" in z.files else 0.0
+        return cls(z["params"], z["mean"], z["std"], confidence=conf, droptol_offset=off)
 
 
 def load_default():
@@ -484,13 +511,15 @@ def choose_plan(A: CSR, vt=F64, et=F64, model=None, confidence=None):
 
 
 def choose_droptol(A: CSR, model=None):
-    """(droptol, source): the model's prediction inside the trained range, else 0.125."""
+    """(droptol, source): the grid point nearest the model's prediction when that lies inside
+    the trained range (half a step of slack), else the default 0.125."""
     model = model or load_default()
     if model is not None:
         _, lg = model.predict(features(A))
         lg = float(lg[0])
-        if min(DROPTOL_GRID) - 0.5 <= lg <= max(DROPTOL_GRID) + 0.5:
-            return 2.0 ** float(np.clip(lg, min(DROPTOL_GRID), max(DROPTOL_GRID))), "model"
+        grid = np.array(DROPTOL_GRID)
+        if grid.min() - 0.5 <= lg <= grid.max() + 0.5:
+            return 2.0 ** float(grid[np.abs(grid - lg).argmin()]), "model"
     return DEFAULT_DROPTOL, "default"
 
 
@@ -604,6 +633,7 @@ def main(argv=None):
         model = Model(seed=0)
         model.fit(X[:ntr], Tp[:ntr], Mp[:ntr], Td[:ntr], Md[:ntr], epochs=args.epochs, lr=args.lr, log=log)
         model.calibrate(X[:ntr])
+        log("droptol offset", model.calibrate_droptol(X[:ntr], DT[:ntr], DS[:ntr], DI[:ntr]))
         model.save(args.out)
         fixed = best_fixed_class(T[:ntr])
         log("train", evaluate(model, X[:ntr], T[:ntr], DT[:ntr], DS[:ntr], DI[:ntr], fixed_class=fixed))

@@ -136,6 +136,32 @@ def test_features_known_values():
     assert f[31] == 1.0
 
 
+def test_droptol_offset_calibration_hand_case():
+    # raw predictions -3.3 where 2^-4 is best (cost 1 vs 2 at the default) and -3.1 where the
+    # default is best (1 vs 3): the rounding threshold must fall between them; the offsets
+    # -0.25 .. -0.35 all place it there and the smallest in size is taken
+    G_ = len(PL.DROPTOL_GRID)
+    DT = np.full((4, G_), np.inf)
+    DT[:, 0] = (2.0, 2.0, 1.0, 1.0)
+    DT[:, 1] = (1.0, 1.0, 3.0, 3.0)
+    DS = np.where(np.isfinite(DT), 0.1, 5.0)
+    DI = np.where(np.isfinite(DT), 10, 0).astype(np.int32)
+
+    class Raw:
+        droptol_offset = 0.0
+
+        def predict(self, X):
+            return np.full((4, C_), 1.0 / C_), np.array([-3.3, -3.3, -3.1, -3.1]) + self.droptol_offset
+
+    m = Raw()
+    off = PL.Model.calibrate_droptol(m, np.zeros((4, 32)), DT, DS, DI)
+    assert off == pytest.approx(-0.25) and m.droptol_offset == off
+    res = PL.evaluate(_FixedModel([0] * 4, m.predict(None)[1]), np.zeros((4, 32)), np.ones((4, C_)),
+                      DT, DS, DI)
+    assert res["droptol_pred_over_best_gmean"] == pytest.approx(1.0)
+    assert res["droptol_slice_n"] == 2 and res["droptol_slice_pred_over_default_gmean"] == pytest.approx(0.5)
+
+
 def test_corpus_specs_deterministic():
     a = [PL.corpus_spec(i) for i in range(20)]
     b = [PL.corpus_spec(i) for i in range(20)]
@@ -318,3 +344,9 @@ def test_gpu_committed_model_beats_fixed_format_on_holdout(gpu):
     assert res["plan_top_ratio_gmean"] < res["plan_fixed_ratio_gmean"]
     assert res["plan_top_ratio_gmean"] <= 1.05
     assert res["plan_choice_ratio_gmean"] <= res["plan_top_ratio_gmean"] + 1e-12
+    # drop tolerance: on the held-out matrices whose best grid point is not the default, the
+    # predicted one sets up and solves faster than the default; over all of them no worse
+    # than 1% (what the reference's DroptolTuner searches for, without its probe solves)
+    assert res["droptol_slice_n"] >= 10
+    assert res["droptol_slice_pred_over_default_gmean"] < 0.95
+    assert res["droptol_pred_over_default_gmean"] <= 1.01
