@@ -62,26 +62,41 @@ static_assert(kMaxBlocks % kFinThreads == 0, "partials per finishing thread");
  * NS * kFinThreads/64 doubles).  All loads are issued before the adds; the order of the adds
  * is fixed, so the result is reproducible run to run.  out[] is valid in every thread. */
 template<int NS, unsigned GMAX = kMaxBlocks>
+struct partials {
+  static_assert(GMAX % kFinThreads == 0, "partials per finishing thread");
+  static constexpr int K = GMAX / kFinThreads;
+  double t[NS][K];
+  /* requests this thread's share of the G partials of each sum */
+  __device__ __forceinline__ void load(const double *parts, unsigned G)
+  {
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const unsigned i = threadIdx.x + k * kFinThreads;
+        t[s][k] = i < G ? parts[s * G + i] : 0.0;
+      }
+  }
+  /* the sums, valid in every thread (all threads of the block take part) */
+  __device__ __forceinline__ void sum(double *lds, double (&out)[NS])
+  {
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      double a = 0;
+#pragma unroll
+      for (int k = 0; k < K; k++) a += t[s][k];
+      out[s] = a;
+    }
+    dev::block_sum<kFinThreads, NS>(out, lds);
+  }
+};
+
+template<int NS, unsigned GMAX = kMaxBlocks>
 __device__ __forceinline__ void sum_partials(const double *parts, unsigned G, double *lds, double (&out)[NS])
 {
-  static_assert(GMAX % kFinThreads == 0, "partials per finishing thread");
-  constexpr int K = GMAX / kFinThreads;
-  double t[NS][K];
-#pragma unroll
-  for (int s = 0; s < NS; s++)
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-      const unsigned i = threadIdx.x + k * kFinThreads;
-      t[s][k] = i < G ? parts[s * G + i] : 0.0;
-    }
-#pragma unroll
-  for (int s = 0; s < NS; s++) {
-    double a = 0;
-#pragma unroll
-    for (int k = 0; k < K; k++) a += t[s][k];
-    out[s] = a;
-  }
-  dev::block_sum<kFinThreads, NS>(out, lds);
+  partials<NS, GMAX> p;
+  p.load(parts, G);
+  p.sum(lds, out);
 }
 
 /* Fused recurrence update, one pass over 10 input and 8 output vectors:

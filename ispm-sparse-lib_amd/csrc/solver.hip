@@ -88,9 +88,14 @@ __global__ void __launch_bounds__(kFinThreads) k_fin(const double *parts, int G,
 {
   constexpr int NS = OP == OP_PPCG ? 4 : OP == OP_NORMF ? 2 : 1;
   __shared__ double lds[NS * kFinThreads / 64];
-  if (skip_if_done && s->done) return;
+  /* the partials are requested together with the done flag, not after it (one memory round
+   * trip less per finishing kernel); the flag is the same for the whole block */
+  const unsigned done = skip_if_done ? s->done : 0u;
+  partials<NS, GMAX> pt;
+  pt.load(parts, G);
+  if (done) return;
   double out[NS];
-  sum_partials<NS, GMAX>(parts, G, lds, out);
+  pt.sum(lds, out);
   if (threadIdx.x) return;
   const double r = out[0];
   if (OP == OP_PQ) { s->pq = r; s->alpha = s->rho / r; }
@@ -211,7 +216,8 @@ __global__ void __launch_bounds__(kFinThreads)
 k_fin_rz_rr(const double *pg, unsigned Gg, const double *pr, unsigned Gr, scalars *s)
 {
   __shared__ double lds[2 * kFinThreads / 64];
-  if (s->done) return;
+  /* the done flag is read with the first partials, not before them */
+  const unsigned done = s->done;
   double out[2] = {0.0, 0.0};
   const unsigned t = threadIdx.x;
   /* eight loads of each sum in flight per step; a fixed order keeps the sums reproducible */
@@ -225,9 +231,11 @@ k_fin_rz_rr(const double *pg, unsigned Gg, const double *pr, unsigned Gr, scalar
       a[k] = i < Gg ? pg[i] : 0.0;
       c[k] = i < Gr ? pr[i] : 0.0;
     }
+    if (done) return;   /* the same for the whole block */
 #pragma unroll
     for (int k = 0; k < 8; k++) { out[0] += a[k]; out[1] += c[k]; }
   }
+  if (done) return;
   dev::block_sum<kFinThreads, 2>(out, lds);
   if (t) return;
   if (s->rr_valid) {
