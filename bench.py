@@ -41,22 +41,33 @@ class CacheFlush:
 
 
 def time_spmv(I, run, steps, warmup, flush=None):
-    """Average launch duration of one SpMV (`run()` launches it), timed with HIP events
-    recorded on the library stream (the stream the kernel runs on); with `flush`, the cache is
-    flushed before every timed launch (for matrices that fit in the Infinity Cache)."""
+    """(average, median) launch duration of one SpMV (`run()` launches it), timed with HIP
+    events recorded on the library stream (the stream the kernel runs on).  Without `flush`,
+    five batches of `steps` back-to-back launches, each batch between two events (the
+    launches' own time, not the event overhead of single launches); with `flush`, the cache is
+    flushed before every timed launch (for matrices that fit in the Infinity Cache), each
+    launch between its own two events."""
     for _ in range(warmup):
         run()
     I.synchronize()
     e0, e1 = I.Event(), I.Event()
     ts = []
-    for _ in range(steps):
-        if flush:
+    if flush:
+        for _ in range(steps):
             flush()
-        e0.record()
-        run()
-        e1.record()
-        e1.synchronize()
-        ts.append(e0.elapsed_ms(e1) * 1e-3)
+            e0.record()
+            run()
+            e1.record()
+            e1.synchronize()
+            ts.append(e0.elapsed_ms(e1) * 1e-3)
+    else:
+        for _ in range(5):
+            e0.record()
+            for _ in range(steps):
+                run()
+            e1.record()
+            e1.synchronize()
+            ts.append(e0.elapsed_ms(e1) * 1e-3 / steps)
     return float(np.mean(ts)), float(np.median(ts))
 
 
@@ -390,6 +401,9 @@ def main():
     ap.add_argument("--foam", action="store_true",
                     help="time the foam-extend call pattern (lduMatrix input, device coefficient refresh, "
                          "normalised-residual solves, drop tolerance tuned across time steps) on a 128^3 mesh")
+    ap.add_argument("--droptol-sweep", action="store_true",
+                    help="also time the FP64 and mixed PCG solves with the factors of drop tolerances "
+                         "2^-4 and 2^-5 (set-up on the host + solve, one line of its own)")
     ap.add_argument("--calibrate", action="store_true",
                     help="first copy 1 GiB device-to-device 3 times (a plain copy of known size, "
                          "the reference for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -454,6 +468,24 @@ def main():
         v = solves[name]
         log(f"solve {name}: {v['seconds'] * 1e3:.1f} ms, {v['iterations']} it, "
             f"{v['replacements']} repl, oracle relres {v['oracle_relres']:.2e} [{time.perf_counter() - t:.1f}s]")
+    if args.droptol_sweep:
+        # set-up (host fastainv_sym) + solve at smaller drop tolerances: denser factors, fewer
+        # iterations, more off-diagonal preconditioner work per iteration
+        sweep = {"2^-3": {"setup_s": setup_s, "fsai_nnz_per_row": factors[0].nnz / A.n_rows,
+                          "fp64_pcg": solves["fp64_pcg"], "mixed_pcg": solves["mixed_pcg"]}}
+        for lg in (-4, -5):
+            t = time.perf_counter()
+            fac = I.fastainv_sym(A, I.F64, max_elts=64 * A.n_rows, droptol=2.0 ** lg)
+            ent = {"setup_s": time.perf_counter() - t, "fsai_nnz_per_row": fac[0].nnz / A.n_rows}
+            for name, prec in (("fp64_pcg", "fp64"), ("mixed_pcg", "mixed")):
+                ent[name], _ = solve_leg(I, S, O, A, b, fac, prec, S.PCG, min(steps, 3), 1, tol)
+            sweep[f"2^{lg}"] = ent
+            log(f"droptol 2^{lg}: set-up {ent['setup_s']:.1f} s, nnz(Ainv1)/n {ent['fsai_nnz_per_row']:.2f}; "
+                f"fp64 {ent['fp64_pcg']['seconds'] * 1e3:.1f} ms / {ent['fp64_pcg']['iterations']} it, "
+                f"mixed {ent['mixed_pcg']['seconds'] * 1e3:.1f} ms / {ent['mixed_pcg']['iterations']} it")
+            del fac
+        print(json.dumps({"metric": "FSAI-PCG over drop tolerances", "sweep": sweep}))
+        return
     # the fastest solve of each precision: the speed-up compares best with best
     fp64_best = min(v["seconds"] for k, v in solves.items() if k.startswith("fp64"))
     mixed = min((v for k, v in solves.items() if k.startswith("mixed")), key=lambda v: v["seconds"])

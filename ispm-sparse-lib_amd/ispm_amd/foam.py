@@ -16,8 +16,8 @@ from typing import Optional
 
 import numpy as np
 
-from . import (CSR, F32, F64, SPMV_PLAN_SYMMETRIC, CsrOut, DeviceArray, IspmError, Plan, _csr_from_out,
-               check, fastainv_sym, lib, synchronize)
+from . import (CSR, F32, F64, SPMV_PLAN_SPLIT_DIAG, SPMV_PLAN_SYMMETRIC, CsrOut, DeviceArray, IspmError,
+               Plan, _csr_from_out, check, fastainv_sym, lib, synchronize)
 from .solver import PCG, PPCG, Solver
 
 
@@ -128,10 +128,16 @@ class FoamPCG:
     CSR the approximate inverse is computed from).  `solve(source, psi, tolerance, rel_tol)`
     computes the approximate inverse at the tuner's drop tolerance, solves on the device with
     OpenFOAM's normalised residual and reports the set-up + solve time to the tuner.
-    precision "fp64" or "mixed" (FP32/BF16 iterations, FP64 residuals)."""
+    precision "fp64" or "mixed" (FP32/BF16 iterations, FP64 residuals).
+
+    The preconditioner's storage (the two factors or their product, each as SlELL or
+    split-diagonal plans) is chosen by timing the candidates at the first solve and kept for
+    the following ones (the addressing is fixed); `redecide_every` > 0 times them again every
+    that many solves."""
 
     def __init__(self, ldu: LduMatrix, precision="fp64", method=None, droptol=None,
-                 log2_range=(-10.0, -3.0), max_elts_per_row=64, maxit=1000, symmetric=None):
+                 log2_range=(-10.0, -3.0), max_elts_per_row=64, maxit=1000, symmetric=None,
+                 redecide_every=0):
         self.precision = precision
         self.mixed = precision == "mixed"
         self.method = method if method is not None else (PCG if self.mixed else PPCG)
@@ -163,6 +169,8 @@ class FoamPCG:
         self.fixed_droptol = droptol
         self.tuner = None if droptol is not None else DroptolTuner(*log2_range)
         self.steps = 0
+        self.redecide_every = int(redecide_every)
+        self.storage = None      # (fsai_product, split_diag) chosen at the last decision
 
     def update(self, ldu: LduMatrix):
         """New coefficients, same addressing: refresh the plan(s) of A on the device."""
@@ -195,8 +203,13 @@ class FoamPCG:
         ft = F32 if self.mixed else F64
         factors = fastainv_sym(self.csr, ft, max_elts=self.max_elts_per_row * self.n, droptol=dt)
         tf = time.perf_counter()
+        decide = self.storage is None or (self.redecide_every > 0 and self.steps % self.redecide_every == 0)
+        product, split = (None, None) if decide else self.storage
         solver = Solver(self.csr, M=factors, precision=self.precision, method=self.method,
-                        A_plan=self.A_plan, maxit=self.maxit)
+                        A_plan=self.A_plan, maxit=self.maxit, fsai_product=product, split_diag=split)
+        if decide and solver.M_plans:
+            self.storage = (len(solver.M_plans) == 1,
+                            bool(solver.M_plans[0].flags & SPMV_PLAN_SPLIT_DIAG))
         if self.res_plan is not None:
             check(lib().ispm_solver_set_residual_plan(solver.p, self.res_plan.p))
         pad = np.zeros(self.n_pad - self.n)

@@ -95,6 +95,11 @@ def test_time_steps_reuse_plan_and_probe_droptol(gpu, symmetric, precision):
         _, r = O.foam_residual(A, b, psi0, psi)
         assert perf.converged and r < 1e-9
         probes.append(perf.droptol)
+        # the preconditioner storage chosen at the first solve is kept for the later ones
+        if step == 0:
+            storage = drv.storage
+            assert storage is not None
+        assert drv.storage == storage
         # the refreshed plans multiply exactly like plans built from this step's matrix
         n = drv.n_pad
         dt = np.float32 if vt == I.F32 else np.float64
