@@ -247,6 +247,44 @@ def foam_legs(I, G, O, m=128, steps=4, tol=1e-8):
     return {"mesh": f"{m}^3", "cells": L.n, "faces": L.n_faces, "tolerance": tol, "steps": legs}
 
 
+def hblock_legs(I, G, O, steps, warmup):
+    """SpMV of a banded two-unknowns-per-node matrix (1,048,576 nodes, 2,097,152 rows, every
+    row aligned column pairs within 40 nodes: the format guess takes hblock pairs) with the
+    default plan, whose slices read x from LDS windows, against the same device layout through
+    the generic slice kernel that gathers x from global memory (the reference's mapping)."""
+    import ctypes as C
+    A = G.banded_pairs(1 << 20, seed=5)
+    x = G.uniform(6, A.n_rows)
+    out = {"n_rows": A.n_rows, "nnz": A.nnz}
+    for name, vt, et in (("fp64", I.F64, I.F64), ("fp32", I.F32, I.F32)):
+        plan = I.Plan(A, vt, elm_type=et)
+        inf = plan.info()
+        n = plan.n_rows_padded
+        dt = np.float64 if vt == I.F64 else np.float32
+        xd = I.DeviceArray.from_numpy(np.concatenate([x, np.zeros(n - A.n_rows)]).astype(dt))
+        y1, y2 = I.DeviceArray(n, dt), I.DeviceArray(n, dt)
+        lp = inf.launch
+        t_win, _ = time_spmv(I, lambda: plan.execute(xd, y1), steps, warmup)
+        t_glob, _ = time_spmv(I, lambda: I.check(I.lib().ispm_spmv(C.byref(lp), C.byref(inf.device), vt,
+                                                                   xd.vp, y2.vp)), steps, warmup)
+        I.synchronize()
+        same = bool(np.array_equal(y1.numpy(), y2.numpy()))
+        ref = O.csr_spmv(A, x.astype(dt).astype(np.float64))
+        err = float(np.max(np.abs(y1.numpy()[:A.n_rows] - ref) / np.maximum(O.csr_spmv_abs(A, x), 1e-300)))
+        stored = inf.stored_bytes
+        out[name] = {"format": dict(S=inf.S, hblock=inf.hblock, V=inf.V, D=inf.D, N=lp.N, H=lp.H),
+                     "window_slices": inf.window_slices, "slices": inf.device.n_slices,
+                     "stored_bytes": stored, "windows_s": t_win, "global_gathers_s": t_glob,
+                     "windows_GBps": stored / t_win * 1e-9, "global_GBps": stored / t_glob * 1e-9,
+                     "frac_hbm": stored / t_win * 1e-9 / HBM_PEAK_GBS,
+                     "bit_identical_to_global": same, "max_rel_err_vs_oracle": err}
+        log(f"hblock {name}: S={inf.S} windows {inf.window_slices}/{inf.device.n_slices} slices, "
+            f"{t_win * 1e6:.1f} us ({stored / t_win * 1e-9:.0f} GB/s) vs global gathers "
+            f"{t_glob * 1e6:.1f} us ({stored / t_glob * 1e-9:.0f} GB/s); same {same}, rel err {err:.1e}")
+        del plan, xd, y1, y2
+    return out
+
+
 def batching_legs(I, G, O, S):
     """The other workload configurations, timed on the device with inputs resident: the 128^3
     finite-volume operator (FP64 SpMV and FSAI-PCG to 1e-8), its 3 and 8 right-hand sides as
@@ -404,6 +442,9 @@ def main():
     ap.add_argument("--droptol-sweep", action="store_true",
                     help="also time the FP64 and mixed PCG solves with the factors of drop tolerances "
                          "2^-4 and 2^-5 (set-up on the host + solve, one line of its own)")
+    ap.add_argument("--hblock", action="store_true",
+                    help="time the SpMV of a banded two-unknowns-per-node matrix stored as hblock pairs, "
+                         "LDS x windows against global gathers (one JSON line of its own)")
     ap.add_argument("--calibrate", action="store_true",
                     help="first copy 1 GiB device-to-device 3 times (a plain copy of known size, "
                          "the reference for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -427,6 +468,10 @@ def main():
         log("calibration: 3 device-to-device copies of 1 GiB")
     if args.foam:
         print(json.dumps({"metric": "foam-extend call pattern", "legs": foam_legs(I, G, O)}))
+        return
+    if args.hblock:
+        print(json.dumps({"metric": "hblock SpMV, LDS x windows vs global gathers",
+                          "legs": hblock_legs(I, G, O, max(args.steps, 10), args.warmup)}))
         return
     if args.batching:
         from ispm_amd import solver as S

@@ -246,6 +246,7 @@ typedef struct ispm_plan_info {
   ispm_slell_desc device;      /* device matrix (valid unless NO_DEVICE) */
   unsigned pattern_planes;     /* diagonal planes of the pattern most slices share (0: none) */
   unsigned pattern_slices;     /* slices that have exactly that pattern */
+  unsigned window_slices;      /* hblock slices whose x window is staged in LDS */
 } ispm_plan_info;
 
 /* Replaces spmv_plan<T,E>::spmv_plan(csr, kind, flags) (plan/plan.h:64-146):

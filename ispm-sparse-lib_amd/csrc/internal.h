@@ -60,6 +60,12 @@ struct layout_hints {
   unsigned stream_span[3];
   /* the same for groups of 64 and 128 consecutive rows */
   unsigned stream_span_small[2];
+  /* hblock slices (fixed height 64..512): per slice (first column, length) of the x window
+   * its entries read, staged in LDS (length 0: the slice gathers from global memory);
+   * device array of 2 n_slices words, and the longest window */
+  const unsigned *xwin;
+  unsigned xwin_max;
+  unsigned xwin_slices;
 };
 
 /* SpMV with the checks of ispm_spmv, an optional device skip flag (nothing is done while

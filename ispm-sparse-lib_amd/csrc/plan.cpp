@@ -315,6 +315,52 @@ void compute_hints(const conv::layout &L, layout_hints &h)
   for (unsigned k = 0; k < 2; k++) h.stream_span_small[k] = L.S <= (64u << k) ? span_of(64u << k) : 0u;
 }
 
+/* x windows of hblock slices: the columns a slice's entries read span [lo, hi]; a slice is
+ * windowed when the span fits 32 KiB of LDS in the plan's vector type and is at most twice
+ * the slice's stored slots (staging it costs no more than the gathers it replaces).  Padding
+ * slots (value 0) may name columns outside the span: the kernel clamps their index. */
+int upload_xwin(const conv::layout &L, const ispm_csr *A, int vt, unsigned *&dev, layout_hints &h)
+{
+  h.xwin = nullptr;
+  h.xwin_max = h.xwin_slices = 0;
+  if (L.W != 2 || L.V || L.D || L.S < 64 || L.S > 512 || !L.n_slices) return ISPM_OK;
+  /* records of 2 words: element offset, column offset */
+  const unsigned cap = 32768u / (unsigned)tsize(vt), S = L.S, P = 2;
+  std::vector<unsigned> w(2 * (size_t)L.n_slices, 0u);
+  unsigned mx = 0, nw = 0;
+  for (unsigned s = 0; s < L.n_slices; s++) {
+    const unsigned r0 = s * S, r1 = std::min<unsigned>((unsigned)A->n_rows, r0 + S);
+    if (r0 >= r1) continue;
+    const int k0 = A->row_ptr[r0], k1 = A->row_ptr[r1];
+    if (k1 <= k0) continue;
+    int lo = A->cols[k0], hi = A->cols[k0];
+    for (int k = k0; k < k1; k++) { lo = std::min(lo, A->cols[k]); hi = std::max(hi, A->cols[k]); }
+    /* a pair plane reads columns 2c and 2c + 1: widen to whole pairs */
+    lo &= ~1;
+    hi = std::min(hi | 1, A->n_cols - 1);
+    const unsigned len = (unsigned)(hi - lo + 1);
+    const unsigned slots = L.sp[P * (s + 1)] - L.sp[P * s];
+    if (len <= cap && len <= 2 * slots) {
+      w[2 * (size_t)s] = (unsigned)lo;
+      w[2 * (size_t)s + 1] = len;
+      mx = std::max(mx, len);
+      nw++;
+    }
+  }
+  if (!nw) return ISPM_OK;
+  int st = ispm_malloc(reinterpret_cast<void **>(&dev), w.size() * sizeof(unsigned));
+  if (st) return st;
+  if ((st = ispm_memcpy(dev, w.data(), w.size() * sizeof(unsigned), ISPM_H2D))) {
+    ispm_free(dev);
+    dev = nullptr;
+    return st;
+  }
+  h.xwin = dev;
+  h.xwin_max = mx;
+  h.xwin_slices = nw;
+  return ISPM_OK;
+}
+
 static int finish_plan(ispm_plan *p, const ispm_csr *A, unsigned flags)
 {
   compute_hints(p->L, p->hints);
@@ -327,6 +373,7 @@ static int finish_plan(ispm_plan *p, const ispm_csr *A, unsigned flags)
     if (st) return st;
     p->has_dev = true;
     if ((st = upload_pattern(p->L, p->pat, p->dev_uni, p->has_pat, &p->pat_slices))) return st;
+    if ((st = upload_xwin(p->L, A, p->T, p->dev_xwin, p->hints))) return st;
   } else {
     std::memset(&p->dev, 0, sizeof p->dev);
     p->dev.n_rows = p->L.n_rows;
@@ -1080,6 +1127,7 @@ int ispm_plan_info_get(const ispm_plan *p, ispm_plan_info *info)
   info->device = p->dev;
   info->pattern_planes = p->has_pat ? p->pat.n : 0;
   info->pattern_slices = p->has_pat ? p->pat_slices : 0;
+  info->window_slices = p->hints.xwin_slices;
   return ISPM_OK;
 }
 
@@ -1145,6 +1193,7 @@ void ispm_plan_destroy(ispm_plan *p)
   if (p->dev_rmask) ispm_free(p->dev_rmask);
   if (p->dev_map) ispm_free(p->dev_map);
   if (p->dev_uni) ispm_free(p->dev_uni);
+  if (p->dev_xwin) ispm_free(p->dev_xwin);
   delete p;
 }
 

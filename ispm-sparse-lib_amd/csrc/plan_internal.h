@@ -29,6 +29,7 @@ struct ispm_plan {
   unsigned *dev_rmask = nullptr;          /* bit per matrix row: a remainder row */
   unsigned split_n = 0, split_m = 0;      /* padded rows of the matrix; remainder rows */
   ispm::layout_hints hints{};             /* launch facts of the layout (streamed mapping) */
+  unsigned *dev_xwin = nullptr;           /* hblock x windows (hints.xwin) */
   /* symmetric diagonal plan (ISPM_SPMV_PLAN_SYMMETRIC): dev.elms holds the upper planes */
   bool sym = false;
   ispm::sym_desc symd{};
@@ -47,6 +48,7 @@ int upload_layout(const conv::layout &L, const void *elms, int et, ispm_slell_de
 ispm_launch default_launch(const conv::layout &L);
 bool build_pattern(const conv::layout &L, diag_pattern &pat, std::vector<unsigned> &mask);
 void compute_hints(const conv::layout &L, layout_hints &h);
+int upload_xwin(const conv::layout &L, const ispm_csr *A, int vt, unsigned *&dev, layout_hints &h);
 }
 
 #endif

@@ -21,6 +21,7 @@ int spmv_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, 
   if (spmvk::use_diagpat(p, A, vec_type, H, pat, uni)) return spmvk::launch_diagpat_types(A, vec_type, *pat, uni, x, y, skip);
   if (spmvk::use_stream(p, A))
     return spmvk::launch_stream_types(A, vec_type, x, y, skip, nullptr, nullptr, nullptr, hints);
+  if (spmvk::use_hwin(p, A, vec_type, H, hints)) return spmvk::launch_hwin_types(A, vec_type, x, y, skip, hints);
   if (spmvk::use_packed(p, A)) return spmvk::launch_packed_types(p, A, vec_type, x, y, skip);
   return spmvk::launch_types<1>(p, A, vec_type, x, 0, y, 0, H, skip);
 }

@@ -960,6 +960,118 @@ inline int launch_diagpat_types(const ispm_slell_desc *A, int vt, const diag_pat
   return fail(ISPM_ERR_UNSUPPORTED, "spmv: unsupported (vector, element) types (%d, %d)", vt, et);
 }
 
+/* hblock slices of fixed height S (64..512), one workgroup of S threads per slice, one row
+ * per thread (the texture-pair fetches of cuda/spmv/kernel.h:93-119 become LDS reads).  A
+ * slice with an x window (xwin[2 s] = first column, xwin[2 s + 1] = length, staged by the
+ * plan when the slice's columns are close together) loads that stretch of x into LDS with
+ * 16-byte vectors and reads every single and pair entry from there; other slices gather
+ * from global memory.  Each row adds its single planes, then its pair planes, in storage
+ * order (as slell_kernel does); a padding slot's column may lie outside the window, its
+ * index is clamped (the value there is 0). */
+template<typename T, typename E>
+__global__ void __launch_bounds__(512) slell_hwin_kernel(spmv_args a, const unsigned *__restrict__ xwin)
+{
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *win = reinterpret_cast<T *>(lds_raw);
+  if (a.skip && *a.skip) return;
+  const unsigned slice = (blockIdx.x & 7u) * a.per_xcd + (blockIdx.x >> 3);
+  if (slice >= a.n_slices) return;   /* uniform over the workgroup */
+  const unsigned log2S = a.log2S, S = 1u << log2S, t = threadIdx.x;
+  /* records of 2 words (element offset, column offset) for hblock without variable heights
+   * or diagonals */
+  const unsigned *rec = a.sp + 2 * slice;
+  const unsigned e0 = rec[0], c0 = rec[1], e1 = rec[2], c1 = rec[3];
+  const unsigned nse = 2 * (c1 - c0) - (e1 - e0);
+  const unsigned nell = nse >> log2S, nsec = ((c1 - c0) >> log2S) - nell;
+  const E *__restrict__ el = static_cast<const E *>(a.elms);
+  const unsigned *__restrict__ cl = a.cols;
+  const T *__restrict__ xv = static_cast<const T *>(a.x);
+  T *__restrict__ yv = static_cast<T *>(a.y);
+  const unsigned ncols = a.n_cols;
+  const unsigned wb = xwin[2 * slice], wl = xwin[2 * slice + 1];
+  if (wl) {
+    /* the window as aligned 16-byte vectors where x allows it, element by element at its
+     * ends (and throughout when x itself is not 16-byte aligned) */
+    constexpr unsigned VX = 16 / sizeof(T);
+    const bool xal = (reinterpret_cast<uintptr_t>(xv) & 15u) == 0;
+    const unsigned a0 = xal ? min((wb + VX - 1) & ~(VX - 1), wb + wl) : wb + wl;
+    const unsigned a1 = max(a0, xal ? (wb + wl) & ~(VX - 1) : wb + wl);
+    for (unsigned q = t; q < a0 - wb; q += S) win[q] = xv[wb + q];
+    for (unsigned q = a1 - wb + t; q < wl; q += S) win[q] = xv[wb + q];
+    const unsigned nv = (a1 - a0) / VX;
+    for (unsigned v = t; v < nv; v += S) {
+      const vecx<T> xx = *reinterpret_cast<const vecx<T> *>(xv + a0 + v * VX);
+#pragma unroll
+      for (unsigned e = 0; e < VX; e++) win[a0 - wb + v * VX + e] = xx[e];
+    }
+    __syncthreads();
+  }
+  auto X = [&](unsigned c) -> T {
+    if (wl) {
+      const int i = (int)c - (int)wb;
+      return win[i < 0 ? 0 : (i >= (int)wl ? (int)wl - 1 : i)];
+    }
+    return xload(xv, c, ncols);
+  };
+  T acc = 0;
+  for (unsigned k = 0; k < nell; k++) {
+    const unsigned off = (k << log2S) + t;
+    acc += elt<T, E>::get(el, e0 + off) * X(dev::col_load(cl + c0 + off));
+  }
+  const unsigned boff = nell << log2S;
+  for (unsigned d = 0; d < nsec; d++) {
+    const unsigned xc = 2 * dev::col_load(cl + c0 + ((nell + d) << log2S) + t);
+    const unsigned eo = e0 + boff + ((2 * d) << log2S) + t;
+    acc += elt<T, E>::get(el, eo) * X(xc) + elt<T, E>::get(el, eo + S) * X(xc + 1);
+  }
+  yv[(slice << log2S) + t] = acc;
+}
+
+template<typename T, typename E>
+int launch_hwin(const ispm_slell_desc *A, const void *x, void *y, const unsigned *skip,
+                const layout_hints *h)
+{
+  spmv_args a;
+  a.skip = skip;
+  a.sp = A->slice_ptr;
+  a.cols = A->cols;
+  a.elms = A->elms;
+  a.x = x;
+  a.y = y;
+  a.ldx = a.ldy = 0;
+  a.spb = 1;
+  a.n_slices = A->n_slices;
+  a.log2S = (unsigned)(31 - __builtin_clz(A->slice_height));
+  a.n_cols = (unsigned)A->n_cols;
+  a.per_xcd = (A->n_slices + 7) / 8;
+  const size_t lds = (size_t)h->xwin_max * sizeof(T);
+  hipLaunchKernelGGL((slell_hwin_kernel<T, E>), dim3(a.per_xcd * 8), dim3(A->slice_height), lds, stream(),
+                     a, h->xwin);
+  ISPM_HIP(hipGetLastError());
+  return ISPM_OK;
+}
+
+/* windowed hblock path: a plan's hblock layout with fixed slices of 64..512 rows, one row per
+ * thread (the default launch), and x windows on record */
+inline bool use_hwin(const ispm_launch *p, const ispm_slell_desc *A, int vt, unsigned H,
+                     const layout_hints *h)
+{
+  return h && h->xwin && h->xwin_max && p->mode == 0 && A->hblock == 2 && !A->var_height && !A->diags &&
+         H == 1 && A->slice_height >= 64 && A->slice_height <= 512 && p->N == A->slice_height &&
+         (size_t)h->xwin_max * type_size(vt) <= 32768;
+}
+
+inline int launch_hwin_types(const ispm_slell_desc *A, int vt, const void *x, void *y,
+                             const unsigned *skip, const layout_hints *h)
+{
+  const int et = A->elm_type;
+  if (vt == ISPM_F64 && et == ISPM_F64) return launch_hwin<double, double>(A, x, y, skip, h);
+  if (vt == ISPM_F64 && et == ISPM_F32) return launch_hwin<double, float>(A, x, y, skip, h);
+  if (vt == ISPM_F32 && et == ISPM_F32) return launch_hwin<float, float>(A, x, y, skip, h);
+  if (vt == ISPM_F32 && et == ISPM_BF16) return launch_hwin<float, bf16_t>(A, x, y, skip, h);
+  return fail(ISPM_ERR_UNSUPPORTED, "spmv: unsupported (vector, element) types (%d, %d)", vt, et);
+}
+
 /* Packed mapping for fixed slice heights S < 64: each wave covers 64/S consecutive slices,
  * one row per lane, so every lane works even when a slice has only one or two planes (the
  * slice-per-workgroup mapping would give each row 64/S lanes and leave most of them idle).

@@ -69,7 +69,8 @@ class PlanInfo(C.Structure):
     _fields_ = [("S", C.c_uint), ("hblock", C.c_uint), ("V", C.c_uint), ("D", C.c_uint),
                 ("launch", Launch), ("tuned_seconds", C.c_double),
                 ("stored_bytes", C.c_size_t), ("csr_bytes", C.c_size_t),
-                ("device", SlellDesc), ("pattern_planes", C.c_uint), ("pattern_slices", C.c_uint)]
+                ("device", SlellDesc), ("pattern_planes", C.c_uint), ("pattern_slices", C.c_uint),
+                ("window_slices", C.c_uint)]
 
 
 class CsrOut(C.Structure):

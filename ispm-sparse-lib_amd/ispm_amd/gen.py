@@ -301,3 +301,31 @@ def random_sparse(n: int, seed: int, min_len=1, max_len=40, n_cols=None, integer
     M = M.tocsr()
     M.sort_indices()
     return CSR.from_scipy(M)
+
+
+def banded_pairs(nodes: int, bw: int = 40, per: int = 5, seed: int = 0, far_every: int = 0) -> CSR:
+    """Two unknowns per node (rows 2i, 2i+1; a 2-D elasticity-like coupling), each node
+    coupled by full 2x2 blocks to itself and to `per` nodes drawn uniformly within `bw` of
+    it: every row consists of aligned column pairs (2j, 2j+1), the structure the hblock
+    encoder stores as pairs, and a slice's columns lie close to its rows.  far_every > 0:
+    every such node also couples to the node nodes/2 away."""
+    import scipy.sparse as sp
+    u = uniform01(seed, nodes * per)
+    i = np.repeat(np.arange(nodes, dtype=np.int64), per)
+    j = np.clip(i + np.floor(u * (2 * bw + 1)).astype(np.int64) - bw, 0, nodes - 1)
+    ii = [np.arange(nodes, dtype=np.int64), i]
+    jj = [np.arange(nodes, dtype=np.int64), j]
+    if far_every:
+        f = np.arange(0, nodes, far_every, dtype=np.int64)
+        ii.append(f)
+        jj.append((f + nodes // 2) % nodes)
+    ni, nj = np.concatenate(ii), np.concatenate(jj)
+    # each node pair once (the blocks of repeated draws merge), then the 2x2 block entries
+    key = np.unique(ni * nodes + nj)
+    ni, nj = key // nodes, key % nodes
+    r = np.concatenate([2 * ni + a for a in (0, 0, 1, 1)])
+    c = np.concatenate([2 * nj + b for b in (0, 1, 0, 1)])
+    v = uniform(seed + 1, r.size)
+    M = sp.csr_matrix((v, (r, c)), shape=(2 * nodes, 2 * nodes))
+    M.sort_indices()
+    return CSR.from_scipy(M)

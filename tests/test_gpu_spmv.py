@@ -530,3 +530,65 @@ def _big():
     if "A" not in _BIG:
         _BIG["A"] = G.stencil27_contrast(192)
     return _BIG["A"]
+
+
+def banded_pairs(nodes=4096, bw=40, per=5, seed=21, far_every=0):
+    return G.banded_pairs(nodes, bw, per, seed, far_every)
+
+
+@pytest.mark.parametrize("vt,et", TYPES)
+@pytest.mark.parametrize("S", [64, 256, 512])
+@pytest.mark.parametrize("far", [0, 37])
+def test_hblock_x_windows(gpu, vt, et, S, far):
+    """hblock slices whose columns are close together read x from LDS windows (the plan
+    reports how many); the product matches the oracle, with and without far entries that
+    leave some slices on the global gathers."""
+    A = banded_pairs(far_every=far)
+    plan = I.Plan(A, vt, elm_type=et, explicit=dict(S=S, hblock=2))
+    inf = plan.info()
+    assert inf.hblock == 2 and inf.launch.N == S
+    n_slices = inf.device.n_slices
+    if far == 0:
+        assert inf.window_slices == n_slices
+    else:
+        # slices holding a far entry keep global gathers where their span is too wide for
+        # the vector type's LDS budget or for their own stored slots
+        assert inf.window_slices <= n_slices
+        if S == 64:
+            assert 0 < inf.window_slices < n_slices
+    n = plan.n_rows_padded
+    dt = np.float64 if vt == I.F64 else np.float32
+    x = G.uniform(3, A.n_rows)
+    xd = I.DeviceArray.from_numpy(np.concatenate([x, np.zeros(n - A.n_rows)]).astype(dt))
+    yd = I.DeviceArray(n, dt)
+    plan.execute(xd, yd)
+    check(A, None, x, yd.numpy(), vt, et)
+
+
+def test_hblock_x_windows_integer_exact(gpu):
+    A = banded_pairs(nodes=3000, seed=22, far_every=97)
+    A.vals = np.round(A.vals * 8)
+    plan = I.Plan(A, I.F64, explicit=dict(S=128, hblock=2))
+    assert plan.info().window_slices > 0
+    n = plan.n_rows_padded
+    x = np.round(G.uniform(4, A.n_rows) * 16)
+    xd = I.DeviceArray.from_numpy(np.concatenate([x, np.zeros(n - A.n_rows)]))
+    yd = I.DeviceArray(n)
+    plan.execute(xd, yd)
+    np.testing.assert_array_equal(yd.numpy()[:A.n_rows], O.csr_spmv(A, x))
+
+
+def test_hblock_x_windows_unaligned_x(gpu):
+    """x at an address that is not 16-byte aligned: the windows are staged element by
+    element; results equal the aligned product bit for bit."""
+    A = banded_pairs(nodes=2048, seed=23)
+    plan = I.Plan(A, I.F32, elm_type=I.F32, explicit=dict(S=256, hblock=2))
+    n = plan.n_rows_padded
+    x = G.uniform(5, A.n_rows).astype(np.float32)
+    buf = I.DeviceArray.from_numpy(np.concatenate([[0.0], x, np.zeros(n - A.n_rows + 3)]).astype(np.float32))
+    xs = I.DeviceArray.from_numpy(np.concatenate([x, np.zeros(n - A.n_rows)]).astype(np.float32))
+    y1, y2 = I.DeviceArray(n, np.float32), I.DeviceArray(n, np.float32)
+    plan.execute(xs, y1)
+    I.check(I.lib().ispm_plan_execute(plan.p, I.C.c_void_p(buf.ptr + 4), y2.vp))
+    I.synchronize()
+    np.testing.assert_array_equal(y1.numpy(), y2.numpy())
