@@ -152,3 +152,30 @@ def test_maxit_reports_not_converged(gpu):
     for method in (S.PCG, S.PCG1):
         res = S.solve(A, b, M=None, tol=1e-12, precision="fp64", method=method, maxit=5, check_every=4)
         assert not res.converged and res.iterations >= 5
+
+
+@pytest.mark.parametrize("use_graph", [True, False], ids=["graph", "eager"])
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_maxit_stop_flushes_x_once(gpu, prec, use_graph):
+    """PCG with an FSAI preconditioner stopped by maxit: x += alpha p is deferred into the next
+    iteration's p update and flushed when the iterations stop.  The returned x must be the
+    oracle's PCG iterate after the same number of updates (a missing or doubled flush is one
+    whole step away from it)."""
+    A = G.stencil(12, 12, 12, 7, anisotropy=10, contrast=100, seed=31)
+    b = O.csr_spmv(A, G.uniform(9, A.n_rows))
+    M = I.fastainv_sym(A, I.F64, droptol=0.05)
+    K = 12
+    res = S.solve(A, b, M=M, tol=1e-30, precision=prec, method=S.PCG, maxit=K, check_every=4,
+                  use_graph=use_graph, fsai_product=False)
+    assert not res.converged and res.iterations >= K
+    dist = {}
+    for k in range(max(1, res.iterations - 2), res.iterations + 3):
+        xk, _, _ = O.pcg(A, b, M1=M[0], M2=M[1], tol=1e-30, maxit=k)
+        dist[k] = float(np.linalg.norm(res.x - xk) / np.linalg.norm(xk))
+    best = min(dist, key=dist.get)
+    # FP64: rounding only; mixed: FP32 recurrences with FP64 residual replacements
+    bound = 1e-10 if prec == "fp64" else 1e-3
+    assert dist[best] <= bound, dist
+    # the neighbouring iterates are far away: the match is not an accident of slow progress
+    others = [v for k, v in dist.items() if k != best]
+    assert min(others) > 10 * max(dist[best], bound / 10), dist
