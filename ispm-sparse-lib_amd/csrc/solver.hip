@@ -1033,7 +1033,10 @@ int solve_core(ispm_solver *S, const double *b, double *x, measure &mc, ispm_sol
     const bool out_of_iters = (int)h.iters >= S->o.maxit;
     if (!h.done && !out_of_iters) continue;
     if (!S->pipelined && !S->single) {
-      /* the x update k_px defers to the next iteration */
+      /* the x update k_px defers to the next iteration; stopped by maxit, the update it
+       * applies here belongs to a residual not yet tested, and counts as an iteration (x
+       * then is the iterate after res.iterations updates, as when the solve converges) */
+      if (!h.done && h.alpha != 0.0) res.iterations = (int)h.iters + 1;
       if (S->T == ISPM_F64)
         hipLaunchKernelGGL(k_xflush<double>, g, bl, 0, sm, (double *)S->xs, (const double *)S->p, n, S->s);
       else
