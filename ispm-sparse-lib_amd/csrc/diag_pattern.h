@@ -4,7 +4,8 @@
  * In a diagonal slice (fixed height S) plane d reads x[pos_d - S + r] for its rows r < S,
  * pos_d being the plane's stored diagonal position.  For a stencil matrix the positions
  * relative to the slice's first row are the same in almost every slice, and they come in
- * runs of nearby values (three per (dy, dz) line of a 27-point stencil).  The plan records
+ * runs of values less than S apart, whose windows overlap (one per z-plane of a 27-point
+ * stencil when a grid line is shorter than S, else one per (y, z) line).  The plan records
  * that shared list once, with one LDS window per run -- x[first row + rrel[r] + i] for
  * i < rlen[r], staged at rbase[r] -- and marks the slices whose positions equal it, so the
  * SpMV reads those slices' x from LDS without loading any position. */
@@ -17,7 +18,7 @@ constexpr int kPatPlanes = 32;
 
 struct diag_pattern {
   unsigned n;                    /* diagonal planes of the shared list */
-  unsigned nrun;                 /* runs of nearby diagonals */
+  unsigned nrun;                 /* runs of diagonals less than S apart */
   unsigned poff[kPatPlanes];     /* LDS index of plane d's x entry for row 0 */
   int rrel[kPatPlanes];          /* run r's window starts at x[first row + rrel[r]] */
   unsigned rbase[kPatPlanes];    /* LDS index of run r's window */

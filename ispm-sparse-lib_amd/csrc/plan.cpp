@@ -220,8 +220,10 @@ static int make_layout(const ispm_csr *A, int et, unsigned S, unsigned W, int V,
 
 /* The diagonal list (positions relative to the slice's first row) that most diagonal slices
  * share, when it covers at least a quarter of them and its windows fit 64 KiB of LDS for
- * FP64 vectors, with one window per run of positions no more than 8 apart; mask gets one
- * bit per slice that has exactly that list and no ELL planes. */
+ * FP64 vectors, with one window per run of increasing positions less than S apart (the
+ * windows of such neighbours overlap, and one window over both is shorter than two: the
+ * near diagonals of a 27-point stencil, -n-1 .. n+1, share one); mask gets one bit per
+ * slice that has exactly that list and no ELL planes. */
 bool build_pattern(const conv::layout &L, diag_pattern &pat, std::vector<unsigned> &mask)
 {
   if (!L.D || L.V || L.W != 1 || L.S < 64 || L.S > 512 || !L.n_slices) return false;
@@ -251,7 +253,7 @@ bool build_pattern(const conv::layout &L, diag_pattern &pat, std::vector<unsigne
   unsigned used = 0;
   for (unsigned f = 0; f < pat.n;) {
     unsigned l = f;
-    while (l + 1 < pat.n && R[l + 1] >= R[l] && R[l + 1] - R[l] <= 8) l++;
+    while (l + 1 < pat.n && R[l + 1] >= R[l] && R[l + 1] - R[l] < (long long)S) l++;
     const unsigned span = (unsigned)(R[l] - R[f]);
     pat.rrel[pat.nrun] = (int)(R[f] - (long long)S);
     pat.rlen[pat.nrun] = S + span;
@@ -792,7 +794,7 @@ static int create_sym(const ispm_csr *A, int et, unsigned flags, ispm_plan *p)
       }
   }
   /* x-window pattern over the offsets -d[K-1] .. d[K-1] (positions relative to the slice's
-   * first row + 256, runs of offsets no more than 8 apart, as build_pattern makes them) */
+   * first row + 256, runs of offsets less than 256 apart, as build_pattern makes them) */
   std::vector<long long> R;
   for (unsigned k = K; k-- > 1;) R.push_back(-d[k] + 256);
   for (unsigned k = 0; k < K; k++) R.push_back(d[k] + 256);
@@ -803,7 +805,7 @@ static int create_sym(const ispm_csr *A, int et, unsigned flags, ispm_plan *p)
   unsigned used = 0;
   for (unsigned f = 0; f < pat.n;) {
     unsigned l = f;
-    while (l + 1 < pat.n && R[l + 1] - R[l] <= 8) l++;
+    while (l + 1 < pat.n && R[l + 1] - R[l] < 256) l++;
     const unsigned span = (unsigned)(R[l] - R[f]);
     pat.rrel[pat.nrun] = (int)(R[f] - 256);
     pat.rlen[pat.nrun] = 256 + span;

@@ -736,9 +736,18 @@ __device__ __forceinline__ void lds_dma16(const void *g, void *lds)
   __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void *)lds, 16, 0, 0);
 }
 
+/* lane l - 1's v in lane l (lane 0: 0) */
+__device__ __forceinline__ float lane_up(float v)
+{
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138 /* wave_shr:1 */, 0xf, 0xf, false));
+}
+
 /* the four rows of one lane (see symdia_r4_kernel); x must be readable (and 0) on the
- * window covers of every slice, and mirror rows below 0 are read at row 0 and zeroed */
-template<unsigned NK>
+ * window covers of every slice, and mirror rows below 0 are read at row 0 and zeroed.
+ * D1: d[1] == 1, whose mirror A[row, row - 1] is plane 1 one row up: the lane's own rows
+ * but the first, which is the previous lane's last (lane 0: the previous slice's row 255),
+ * so it takes no load. */
+template<unsigned NK, bool D1>
 __device__ __forceinline__ float __attribute__((ext_vector_type(4)))
 symdia_r4_rows(const sym_args &a, const diag_pattern &pat, const float *__restrict__ el,
                const float *__restrict__ xv, float *win, unsigned slice, int row0, unsigned lane)
@@ -747,18 +756,14 @@ symdia_r4_rows(const sym_args &a, const diag_pattern &pat, const float *__restri
   typedef float f4u __attribute__((ext_vector_type(4))) __attribute__((aligned(4)));
   constexpr unsigned PL = kSymPlane, K = NK, KS = K * PL;
   const int row = row0 + 4 * (int)lane;
-  /* windows go straight from memory to LDS (16 bytes per lane, lane l of round q at vector
-   * 64 q + l), so every plane load can be issued before the first wait */
-#pragma unroll
-  for (unsigned q = 0; q < kR4Win; q++) {
-    const unsigned v = lane + 64 * q;
-    if (v < pat.wtotal) {
-      int dl = pat.wdelta[0];
-#pragma unroll
-      for (unsigned r = 1; r < 9; r++)
-        if (r < pat.nrun && v >= pat.wvbase[r]) dl = pat.wdelta[r];
-      lds_dma16(xv + (row0 + dl + (int)(4 * v)), win + 256 * q);
-    }
+  /* windows go straight from memory to LDS (16 bytes per lane), so every plane load can be
+   * issued before the first wait; run by run, 64 vectors u .. u + 63 of a run per load, so
+   * the run's offset is uniform and a lane's address is one add */
+  for (unsigned r = 0; r < pat.nrun; r++) {
+    const unsigned vb = pat.wvbase[r], ve = r + 1 < pat.nrun ? pat.wvbase[r + 1] : pat.wtotal;
+    const float *xr = xv + (row0 + pat.wdelta[r]);
+    for (unsigned u = vb; u < ve; u += 64)
+      if (u + lane < ve) lds_dma16(xr + 4 * (u + lane), win + 4 * u);
   }
   /* 32-bit byte offsets from the uniform base (the stored planes are below 4 GiB): one
    * address register per load */
@@ -770,7 +775,7 @@ symdia_r4_rows(const sym_args &a, const diag_pattern &pat, const float *__restri
   for (unsigned k = 0; k < NK; k++) {
     ev[k] = *reinterpret_cast<const f4a *>(eb + (ownb + k * PL * 4u));
     mv[k] = f4u{0.f, 0.f, 0.f, 0.f};
-    if (k >= 1) {
+    if (k >= 1 && !(D1 && k == 1)) {
       const unsigned j = (unsigned)max(row - a.d[k], 0);
       f4u m = *reinterpret_cast<const f4u *>(eb + ((j >> 8) * KS + k * PL + (j & 255u)) * 4u);
       if (row0 < a.d[k]) {   /* only the first slices: uniform over the wave */
@@ -787,9 +792,14 @@ symdia_r4_rows(const sym_args &a, const diag_pattern &pat, const float *__restri
       mv[k] = m;
     }
   }
+  const float up1 = D1 && slice > 0 ? el[(slice - 1) * KS + PL + 255] : 0.f;
   /* every load, the windows' included, is complete before LDS is read */
   __builtin_amdgcn_s_waitcnt(0x0f70);
   __builtin_amdgcn_wave_barrier();
+  if (D1) {
+    const float u = lane_up(ev[1].w);
+    mv[1] = f4u{lane == 0 ? up1 : u, ev[1].x, ev[1].y, ev[1].z};
+  }
   f4a acc = f4a{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (unsigned k = 0; k < NK; k++) {
@@ -813,14 +823,15 @@ symdia_r4_rows(const sym_args &a, const diag_pattern &pat, const float *__restri
   return acc;
 }
 
-/* NK: the number of planes (compile time, so that no plane needs a branch) */
-template<unsigned NWAVES, unsigned NK>
+/* NK: the number of planes (compile time, so that no plane needs a branch); D1: d[1] == 1 */
+template<unsigned NWAVES, unsigned NK, bool D1>
 __global__ void __launch_bounds__(64 * NWAVES) symdia_r4_kernel(sym_args a, diag_pattern pat)
 {
   extern __shared__ __align__(16) unsigned char sym_lds[];
   typedef float f4a __attribute__((ext_vector_type(4)));
   constexpr unsigned S = 256;
-  const unsigned lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  /* the wave index is uniform: read it from lane 0, so slice arithmetic stays scalar */
+  const unsigned lane = threadIdx.x & 63, wv = NWAVES == 1 ? 0u : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   float *win = reinterpret_cast<float *>(sym_lds) + wv * pat.lds_len;
   const unsigned xcd = blockIdx.x & 7u, i = (blockIdx.x >> 3) * kR4Waves + wv;
   unsigned slice = xcd * a.per_xcd + i;
@@ -835,7 +846,7 @@ __global__ void __launch_bounds__(64 * NWAVES) symdia_r4_kernel(sym_args a, diag
   float *__restrict__ yv = static_cast<float *>(a.y);
   const int row0 = (int)(slice * S), row = row0 + 4 * (int)lane;
   f4a acc = f4a{0.f, 0.f, 0.f, 0.f};
-  if (active) acc = symdia_r4_rows<NK>(a, pat, el, xv, win, slice, row0, lane);
+  if (active) acc = symdia_r4_rows<NK, D1>(a, pat, el, xv, win, slice, row0, lane);
   if (active) *reinterpret_cast<f4a *>(yv + row) = acc;
   if (a.dparts) {
     __shared__ double red[kR4Waves];
@@ -910,9 +921,15 @@ int launch_symdia(const sym_args &a0, const diag_pattern &pat0, const void *x, v
         else *dgrid = groups * 8;
       }
       const dim3 g4(groups * 8), b4(64 * kR4Waves);
-      if (a.K == 4) hipLaunchKernelGGL((symdia_r4_kernel<kR4Waves, 4>), g4, b4, lds4, stream(), a, c4);
-      else if (a.K == 10) hipLaunchKernelGGL((symdia_r4_kernel<kR4Waves, 10>), g4, b4, lds4, stream(), a, c4);
-      else hipLaunchKernelGGL((symdia_r4_kernel<kR4Waves, 14>), g4, b4, lds4, stream(), a, c4);
+      if (a.d[1] == 1) {
+        if (a.K == 4) hipLaunchKernelGGL((symdia_r4_kernel<kR4Waves, 4, true>), g4, b4, lds4, stream(), a, c4);
+        else if (a.K == 10) hipLaunchKernelGGL((symdia_r4_kernel<kR4Waves, 10, true>), g4, b4, lds4, stream(), a, c4);
+        else hipLaunchKernelGGL((symdia_r4_kernel<kR4Waves, 14, true>), g4, b4, lds4, stream(), a, c4);
+      } else {
+        if (a.K == 4) hipLaunchKernelGGL((symdia_r4_kernel<kR4Waves, 4, false>), g4, b4, lds4, stream(), a, c4);
+        else if (a.K == 10) hipLaunchKernelGGL((symdia_r4_kernel<kR4Waves, 10, false>), g4, b4, lds4, stream(), a, c4);
+        else hipLaunchKernelGGL((symdia_r4_kernel<kR4Waves, 14, false>), g4, b4, lds4, stream(), a, c4);
+      }
       ISPM_HIP(hipGetLastError());
       return ISPM_OK;
     }
