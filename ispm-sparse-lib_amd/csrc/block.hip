@@ -16,9 +16,6 @@
 
 namespace ispm {
 
-int spmm_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, int k,
-                  const void *X, size_t ldx, void *Y, size_t ldy, const layout_hints *hints);
-
 namespace {
 
 struct colsc {
@@ -147,7 +144,8 @@ namespace {
 
 int mspmm(ispm_plan *P, int k, const double *X, size_t ldx, double *Y, size_t ldy)
 {
-  return spmm_internal(&P->launch, &P->dev, ISPM_F64, k, X, ldx, Y, ldy, &P->hints);
+  return spmm_internal(&P->launch, &P->dev, ISPM_F64, k, X, ldx, Y, ldy, &P->hints,
+                       P->has_pat ? &P->pat : nullptr, P->dev_uni);
 }
 
 template<int OP> void mfinish(ispm_block_solver *S)

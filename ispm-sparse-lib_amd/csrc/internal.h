@@ -75,6 +75,12 @@ int spmv_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, 
                   void *y, const unsigned *skip, const diag_pattern *pat = nullptr,
                   const unsigned *uni = nullptr, const layout_hints *hints = nullptr);
 
+/* SpMM (k = 1..8 vectors, column j at X + j ldx) with the checks of ispm_spmm, a plan's
+ * layout facts and its shared diagonal pattern with its slice mask (all may be null). */
+int spmm_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, int k,
+                  const void *X, size_t ldx, void *Y, size_t ldy, const layout_hints *hints = nullptr,
+                  const diag_pattern *pat = nullptr, const unsigned *uni = nullptr);
+
 /* Split-diagonal form of a square matrix: the rows with off-diagonal entries form a SlELL
  * layout R of m rows (streamed mapping) whose row i is matrix row rmap[i]; y = diag .* x on
  * every other row, y = diag .* x + (R x) on those (one launch, see spmv_kernel.h). */

@@ -1168,7 +1168,8 @@ int ispm_plan_execute_multi(ispm_plan *p, int k, const void *X, size_t ldx, void
   if (!p) return fail_plain(ISPM_ERR_INVALID, "plan_execute_multi: null plan");
   if (!p->has_dev) return fail_plain(ISPM_ERR_INVALID, "plan_execute_multi: plan has no device matrix");
   if (plan_single_vector(p)) return fail_plain(ISPM_ERR_UNSUPPORTED, "plan_execute_multi: split-diagonal and symmetric plans multiply one vector");
-  return ispm_spmm(&p->launch, &p->dev, p->T, k, X, ldx, Y, ldy);
+  return spmm_internal(&p->launch, &p->dev, p->T, k, X, ldx, Y, ldy, &p->hints,
+                       p->has_pat ? &p->pat : nullptr, p->dev_uni);
 }
 
 int ispm_plan_update_coefs(ispm_plan *p, const ispm_csr *A)

@@ -87,9 +87,11 @@ int spmv_sym(const sym_desc &sd, int vec_type, const void *x, void *y, const uns
   return fail(ISPM_ERR_UNSUPPORTED, "spmv: unsupported (vector, element) types (%d, %d)", vec_type, et);
 }
 
-/* SpMM for the block solver: the checks of ispm_spmm, with the plan's layout facts */
+/* SpMM for plans and the block solver: the checks of ispm_spmm, with the plan's layout facts
+ * and its shared diagonal pattern (both may be null) */
 int spmm_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, int k,
-                  const void *X, size_t ldx, void *Y, size_t ldy, const layout_hints *hints)
+                  const void *X, size_t ldx, void *Y, size_t ldy, const layout_hints *hints,
+                  const diag_pattern *pat, const unsigned *uni)
 {
   int st = ensure_init();
   if (st) return st;
@@ -98,7 +100,18 @@ int spmm_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, 
   if (st) return st;
   if (k < 1 || k > 8) return fail(ISPM_ERR_UNSUPPORTED, "spmm: k=%d outside 1..8", k);
   if (A->n_slices == 0) return ISPM_OK;
-  if (k == 1) return spmv_internal(p, A, vec_type, X, Y, nullptr, nullptr, nullptr, hints);
+  if (k == 1) return spmv_internal(p, A, vec_type, X, Y, nullptr, pat, uni, hints);
+  if (spmvk::use_diagpat_spmm(p, A, vec_type, pat, uni)) {
+    switch (k) {
+      case 2: return spmvk::launch_diagpat_spmm_types<2>(A, vec_type, *pat, uni, X, ldx, Y, ldy);
+      case 3: return spmvk::launch_diagpat_spmm_types<3>(A, vec_type, *pat, uni, X, ldx, Y, ldy);
+      case 4: return spmvk::launch_diagpat_spmm_types<4>(A, vec_type, *pat, uni, X, ldx, Y, ldy);
+      case 5: return spmvk::launch_diagpat_spmm_types<5>(A, vec_type, *pat, uni, X, ldx, Y, ldy);
+      case 6: return spmvk::launch_diagpat_spmm_types<6>(A, vec_type, *pat, uni, X, ldx, Y, ldy);
+      case 7: return spmvk::launch_diagpat_spmm_types<7>(A, vec_type, *pat, uni, X, ldx, Y, ldy);
+      default: return spmvk::launch_diagpat_spmm_types<8>(A, vec_type, *pat, uni, X, ldx, Y, ldy);
+    }
+  }
   return spmm_dispatch(p, A, vec_type, k, X, ldx, Y, ldy, hints);
 }
 }  // namespace ispm

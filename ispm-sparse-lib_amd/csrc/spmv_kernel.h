@@ -541,6 +541,156 @@ int launch_diagpat(const ispm_slell_desc *A, const diag_pattern &pat, const unsi
   return diagpat_launch<T, E, kPatPlanes>(A, H, g, b, lds, a, c, uni);
 }
 
+/* Multi-vector form of the pattern kernel (the block solver's products, k = 2..8 vectors,
+ * column j at x + j ldx): one 256-row slice per workgroup, one row per thread.  A slice on
+ * the pattern loads its planes once; then, KG vectors at a time, the workgroup stages those
+ * vectors' x windows in LDS (16-byte loads, the windows of all KG vectors spread over the
+ * threads) and each thread adds its row's products for them, plane by plane in the pattern's
+ * order -- the order of the single-vector kernel, so column j equals that kernel's product of
+ * x_j bit for bit.  Gathering x per plane instead (as the generic kernels do) asks L2 for
+ * every vector's lines once per plane.  Other slices gather per row, ELL planes then
+ * diagonal sections, as the single-vector kernel does. */
+template<typename T, typename E, unsigned K, unsigned NPM>
+__global__ void __launch_bounds__(256)
+slell_diagpat_spmm_kernel(spmv_args a, diag_pattern pat, const unsigned *__restrict__ uni, unsigned kg)
+{
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *win = reinterpret_cast<T *>(lds_raw);
+  constexpr unsigned LOG2S = 8, S = 1u << LOG2S, VX = 16 / sizeof(T);
+  const unsigned t = threadIdx.x;
+  const unsigned slice = (blockIdx.x & 7u) * a.per_xcd + (blockIdx.x >> 3);
+  if (slice >= a.n_slices) return;   /* uniform over the workgroup */
+  const E *__restrict__ el = static_cast<const E *>(a.elms);
+  const unsigned *__restrict__ sp = a.sp;
+  const T *__restrict__ X = static_cast<const T *>(a.x);
+  T *__restrict__ Y = static_cast<T *>(a.y);
+  const size_t ldx = a.ldx, ldy = a.ldy;
+  const unsigned ncols = a.n_cols;
+  const unsigned row0 = slice << LOG2S;
+  const unsigned e0 = sp[3 * slice];
+  const bool on_pattern = (uni[slice >> 5] >> (slice & 31u)) & 1u;
+  if (on_pattern) {
+    const unsigned np = pat.n, nrun = pat.nrun, L = pat.lds_len, wt = pat.wtotal;
+    /* 16-byte window loads when every column is aligned and the covers lie inside x */
+    const bool fast = ((reinterpret_cast<uintptr_t>(X) | (ldx * sizeof(T))) % 16) == 0 &&
+                      (long long)row0 + pat.cover_lo >= 0 && (long long)row0 + pat.cover_hi <= (long long)ncols;
+    T ev[NPM];
+#pragma unroll
+    for (unsigned j = 0; j < NPM; j++) ev[j] = j < np ? elt<T, E>::get(el, e0 + (j << LOG2S) + t) : (T)0;
+    /* this thread's window vectors t + q S (the launch checks wt <= QM S) and their offsets
+     * in x, the same for every vector */
+    constexpr unsigned QM = 4;
+    int dq[QM];
+#pragma unroll
+    for (unsigned q = 0; q < QM; q++) {
+      const unsigned vv = t + q * S;
+      int dl = pat.wdelta[0];
+      for (unsigned r = 1; r < nrun; r++)
+        if (vv >= pat.wvbase[r]) dl = pat.wdelta[r];
+      dq[q] = (int)row0 + dl + (int)(vv * VX);
+    }
+    for (unsigned j0 = 0; j0 < K; j0 += kg) {
+      const unsigned nk = min(kg, K - j0);
+      if (j0) __syncthreads();   /* the previous vectors' windows are read */
+      if (fast) {
+        for (unsigned jj = 0; jj < nk; jj++) {
+          const T *xj = X + (size_t)(j0 + jj) * ldx;
+#pragma unroll
+          for (unsigned q = 0; q < QM; q++)
+            if (t + q * S < wt)
+              *reinterpret_cast<vecx<T> *>(win + jj * L + (t + q * S) * VX) =
+                  *reinterpret_cast<const vecx<T> *>(xj + (unsigned)dq[q]);
+        }
+      } else {
+        for (unsigned jj = 0; jj < nk; jj++) {
+          const T *xj = X + (size_t)(j0 + jj) * ldx;
+          for (unsigned r = 0; r < nrun; r++) {
+            const long long x0 = (long long)row0 + pat.rrel[r];
+            const unsigned Lr = pat.rlen[r], b = pat.rbase[r];
+            for (unsigned i = t; i < Lr; i += S) {
+              const long long c = x0 + i;
+              win[jj * L + b + i] = (c >= 0 && c < (long long)ncols) ? xj[c] : (T)0;
+            }
+          }
+        }
+      }
+      __syncthreads();
+      for (unsigned jj = 0; jj < nk; jj++) {
+        T acc = 0;
+#pragma unroll
+        for (unsigned j = 0; j < NPM; j++)
+          if (j < np) acc += ev[j] * win[jj * L + pat.poff[j] + t];
+        Y[(size_t)(j0 + jj) * ldy + row0 + t] = acc;
+      }
+    }
+  } else {
+    const unsigned *__restrict__ cl = a.cols;
+    const unsigned c0 = sp[3 * slice + 1], d0 = sp[3 * slice + 2];
+    const unsigned e1 = sp[3 * slice + 3], c1 = sp[3 * slice + 4];
+    const unsigned nell = (c1 - c0) >> LOG2S;
+    const unsigned nsec = ((e1 - e0) >> LOG2S) - nell;
+    T acc[K];
+#pragma unroll
+    for (unsigned j = 0; j < K; j++) acc[j] = 0;
+    for (unsigned q = 0; q < nell; q++) {
+      const unsigned off = (q << LOG2S) + t;
+      const T e = elt<T, E>::get(el, e0 + off);
+      const unsigned c = dev::col_load(cl + c0 + off);
+#pragma unroll
+      for (unsigned j = 0; j < K; j++) acc[j] += e * xload(X + (size_t)j * ldx, c, ncols);
+    }
+    for (unsigned d = 0; d < nsec; d++) {
+      const unsigned off = ((nell + d) << LOG2S) + t;
+      const T e = elt<T, E>::get(el, e0 + off);
+      const unsigned c = sp[d0 + d] + t - S;
+#pragma unroll
+      for (unsigned j = 0; j < K; j++) acc[j] += e * xload(X + (size_t)j * ldx, c, ncols);
+    }
+#pragma unroll
+    for (unsigned j = 0; j < K; j++) Y[(size_t)j * ldy + row0 + t] = acc[j];
+  }
+}
+
+/* LDS the multi-vector pattern kernel stages per workgroup: the windows of kg vectors, kg as
+ * many as fit 20 KiB (at least one; four 256-thread workgroups or more per CU) */
+template<typename T>
+inline unsigned diagpat_spmm_group(const diag_pattern &c, unsigned k)
+{
+  const size_t per = (size_t)c.lds_len * sizeof(T);
+  return (unsigned)std::max<size_t>(1, std::min<size_t>(k, 20480 / per));
+}
+
+template<typename T, typename E, unsigned K>
+int launch_diagpat_spmm(const ispm_slell_desc *A, const diag_pattern &pat, const unsigned *uni,
+                        const void *X, size_t ldx, void *Y, size_t ldy)
+{
+  spmv_args a;
+  a.skip = nullptr;
+  a.sp = A->slice_ptr;
+  a.cols = A->cols;
+  a.elms = A->elms;
+  a.x = X;
+  a.y = Y;
+  a.ldx = ldx;
+  a.ldy = ldy;
+  a.n_slices = A->n_slices;
+  a.log2S = 8;
+  a.n_cols = (unsigned)A->n_cols;
+  a.spb = 1;
+  a.per_xcd = (A->n_slices + 7) / 8;
+  diag_pattern c;
+  cover_pattern(pat, 16 / sizeof(T), 256, kPatPlanes, 1u << 20, c);
+  const unsigned kg = diagpat_spmm_group<T>(c, K);
+  const size_t lds = (size_t)kg * c.lds_len * sizeof(T);
+  if (lds > 65536 || c.wtotal > 4 * 256)
+    return fail(ISPM_ERR_UNSUPPORTED, "spmm: pattern windows of %u vectors need %zu bytes of LDS", c.wtotal, lds);
+  const dim3 g(a.per_xcd * 8), b(256);
+  if (pat.n <= 8) hipLaunchKernelGGL((slell_diagpat_spmm_kernel<T, E, K, 8>), g, b, lds, stream(), a, c, uni, kg);
+  else hipLaunchKernelGGL((slell_diagpat_spmm_kernel<T, E, K, 27>), g, b, lds, stream(), a, c, uni, kg);
+  ISPM_HIP(hipGetLastError());
+  return ISPM_OK;
+}
+
 /* ---- symmetric diagonal storage (ISPM_SPMV_PLAN_SYMMETRIC) ----
  * A symmetric matrix whose entries lie on a few diagonals (stencils) stores only its upper
  * diagonals d_0 = 0 < d_1 < ... < d_{K-1}: plane k of 256-row slice s holds A[row, row + d_k]
@@ -962,6 +1112,28 @@ inline bool use_diagpat(const ispm_launch *p, const ispm_slell_desc *A, int vt, 
          (H == 1 || H == 2) && p->N * H == A->slice_height && A->slice_height >= 64 &&
          A->slice_height <= 512 &&
          ((size_t)pat->lds_len + 2 * pat->nrun * (16 / type_size(vt))) * type_size(vt) <= 65536;
+}
+
+/* the multi-vector pattern kernel: 256-row slices, FP64 or FP32 (vectors and elements alike),
+ * at most 27 pattern planes */
+inline bool use_diagpat_spmm(const ispm_launch *p, const ispm_slell_desc *A, int vt,
+                             const diag_pattern *pat, const unsigned *uni)
+{
+  if (!use_diagpat(p, A, vt, 1, pat, uni) || A->slice_height != 256 || pat->n > 27 ||
+      !((vt == ISPM_F64 && A->elm_type == ISPM_F64) || (vt == ISPM_F32 && A->elm_type == ISPM_F32)))
+    return false;
+  /* the kernel's window list: at most four vectors per thread, one vector's windows in LDS */
+  diag_pattern c;
+  cover_pattern(*pat, 16 / type_size(vt), 256, kPatPlanes, 1u << 20, c);
+  return c.wtotal <= 4 * 256 && (size_t)c.lds_len * type_size(vt) <= 65536;
+}
+
+template<unsigned K>
+int launch_diagpat_spmm_types(const ispm_slell_desc *A, int vt, const diag_pattern &pat, const unsigned *uni,
+                              const void *X, size_t ldx, void *Y, size_t ldy)
+{
+  if (vt == ISPM_F64) return launch_diagpat_spmm<double, double, K>(A, pat, uni, X, ldx, Y, ldy);
+  return launch_diagpat_spmm<float, float, K>(A, pat, uni, X, ldx, Y, ldy);
 }
 
 inline int launch_diagpat_types(const ispm_slell_desc *A, int vt, const diag_pattern &pat,

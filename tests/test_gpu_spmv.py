@@ -228,6 +228,65 @@ def test_spmm_matches_spmv(mats):
                 np.testing.assert_array_equal(Y[j], yd.numpy())
 
 
+def off_pattern(A, rows, seed=17):
+    """A with one extra entry (far from the diagonal) in each of the given rows: the slices
+    holding them leave the shared diagonal pattern."""
+    rng = np.random.default_rng(seed)
+    rp, cols, vals = np.asarray(A.row_ptr), np.asarray(A.cols), np.asarray(A.vals, np.float64)
+    out_c, out_v, out_p = [], [], [0]
+    for i in range(A.n_rows):
+        c, v = cols[rp[i]:rp[i + 1]], vals[rp[i]:rp[i + 1]]
+        if i in rows:
+            extra = int(rng.integers(0, A.n_cols))
+            if extra not in c:
+                order = np.argsort(np.append(c, extra))
+                c, v = np.append(c, extra)[order], np.append(v, 0.25)[order]
+        out_c.append(c); out_v.append(v); out_p.append(out_p[-1] + len(c))
+    return I.CSR(A.n_rows, A.n_cols, np.array(out_p, np.int64), np.concatenate(out_c).astype(np.int32),
+                 np.concatenate(out_v))
+
+
+@pytest.mark.parametrize("vt,et", [(I.F64, I.F64), (I.F32, I.F32), (I.F64, I.F32)])
+@pytest.mark.parametrize("name", ["27pt16", "7pt32", "19pt24_off", "fsaiprod7pt32"])
+def test_pattern_spmm_matches_spmv(gpu, name, vt, et):
+    """Multi-vector products of 256-row diagonal plans with a shared pattern (a block
+    solver's stencil A or preconditioner product): every column bit-identical to the single-vector product of
+    the same plan, for aligned and unaligned column strides (the latter stage x without
+    16-byte loads), with slices off the pattern (19pt24_off) taking the per-row path; and
+    within the oracle's bound."""
+    if name == "27pt16":
+        A = G.stencil27_contrast(16, block=4)
+    elif name == "7pt32":
+        A = G.poisson7(32)
+    elif name == "19pt24_off":
+        A = off_pattern(G.stencil(24, 24, 24, points=19), {5, 700, 701, 9000, 13000})
+    else:   # the approximate inverse's product Ainv2 Ainv1 (13 diagonals on most slices)
+        a1, a2 = I.fastainv_sym(G.poisson7(32), I.F64, max_elts=64 * 32 ** 3, droptol=0.05)
+        A = I.csr_multiply(a2, a1, I.F64)
+    dt = np.float64 if vt == I.F64 else np.float32
+    p = I.Plan(A, vt, elm_type=et, explicit=dict(S=256, D=True))
+    inf = p.info()
+    assert inf.pattern_planes > 0 and inf.pattern_slices > 0
+    if name == "19pt24_off":
+        assert inf.pattern_slices < inf.device.n_slices
+    n = p.n_rows_padded
+    for k in (2, 3, 8):
+        for ld in (n, n + 1):
+            X = np.zeros((k, ld), dt)
+            X[:, :A.n_cols] = G.uniform(500 + k, k * A.n_cols).reshape(k, A.n_cols)
+            Xd = I.DeviceArray.from_numpy(X.ravel())
+            Yd = I.DeviceArray(k * ld, dt)
+            p.execute_multi(k, Xd, ld, Yd, ld)
+            Y = Yd.numpy().reshape(k, ld)
+            for j in range(k):
+                xd = I.DeviceArray.from_numpy(np.ascontiguousarray(X[j, :n]))
+                yd = I.DeviceArray(n, dt)
+                p.execute(xd, yd)
+                np.testing.assert_array_equal(Y[j, :n], yd.numpy(), err_msg=f"k={k} ld={ld} column {j}")
+            hs = I.csr_to_slell(A, 256, 1, False, True, out_type=et)
+            check(A, hs, X[k - 1, :A.n_cols], Y[k - 1, :n], vt, et)
+
+
 @pytest.mark.parametrize("name", ["irregular", "skewed", "short_long"])
 @pytest.mark.parametrize("S", [1, 4, 32])
 @pytest.mark.parametrize("vt,et", TYPES)
