@@ -231,7 +231,9 @@ enum ispm_plan_kind { ISPM_PLAN_DEFAULT = 0, ISPM_PLAN_ESTIMATE = 1, ISPM_PLAN_E
  * diagonals as dense planes of 256-row slices; each lower entry is read from its mirror, which
  * an earlier slice streamed moments before, so HBM carries about half the matrix.  Plan
  * creation fails with ISPM_ERR_INVALID when the matrix is not symmetric or not on few
- * diagonals.  Execution and the solvers only (no SpMM, no coefficient refresh). */
+ * diagonals.  Execution, SpMM (ispm_plan_execute_multi and the block solver; FP64 and FP32
+ * vectors, each column bit-identical to the single-vector product), the solvers and
+ * coefficient refresh. */
 enum ispm_plan_flags { ISPM_SPMV_NO_TEXTURE_MEMORY = 1, ISPM_SPMV_PLAN_NO_DEVICE = 2,
                        ISPM_SPMV_PLAN_SPLIT_DIAG = 4, ISPM_SPMV_PLAN_SYMMETRIC = 8 };
 

@@ -144,6 +144,7 @@ namespace {
 
 int mspmm(ispm_plan *P, int k, const double *X, size_t ldx, double *Y, size_t ldy)
 {
+  if (P->sym) return spmm_sym(P->symd, ISPM_F64, k, X, ldx, Y, ldy);
   return spmm_internal(&P->launch, &P->dev, ISPM_F64, k, X, ldx, Y, ldy, &P->hints,
                        P->has_pat ? &P->pat : nullptr, P->dev_uni);
 }
@@ -223,7 +224,7 @@ int ispm_block_solver_create(ispm_block_solver **out, ispm_plan *A, ispm_plan *M
   if (!M1 && M2) return fail(ISPM_ERR_INVALID, "block_solver_create: second factor without the first");
   if (!A->has_dev || A->T != ISPM_F64) return fail(ISPM_ERR_INVALID, "block_solver_create: A needs an FP64-vector device plan");
   for (const ispm_plan *M : {A, M1, M2})
-    if (M && plan_single_vector(M)) return fail(ISPM_ERR_UNSUPPORTED, "block_solver_create: split-diagonal and symmetric plans multiply one vector");
+    if (M && M->split) return fail(ISPM_ERR_UNSUPPORTED, "block_solver_create: split-diagonal plans multiply one vector");
   for (const ispm_plan *M : {M1, M2})
     if (M && (!M->has_dev || M->T != ISPM_F64 || M->L.n_rows != A->L.n_rows))
       return fail(ISPM_ERR_INVALID, "block_solver_create: preconditioner plans must be FP64 device plans of A's size");

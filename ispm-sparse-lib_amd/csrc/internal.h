@@ -132,6 +132,10 @@ struct sym_desc {
 int spmv_sym(const sym_desc &sd, int vec_type, const void *x, void *y, const unsigned *skip,
              const void *dotv = nullptr, double *dparts = nullptr, unsigned *dgrid = nullptr,
              unsigned x_guard = 0);
+/* Y = A X for k = 1..8 vectors (column j at X + j ldx, Y + j ldy) in symmetric diagonal
+ * storage, every column as spmv_sym computes it (FP64 and FP32 vectors; BF16 elements
+ * multiply one vector). */
+int spmm_sym(const sym_desc &sd, int vec_type, int k, const void *X, size_t ldx, void *Y, size_t ldy);
 /* the guard (elements on each side) the four-rows-per-lane symmetric kernel needs */
 unsigned sym_x_guard(const sym_desc &sd);
 

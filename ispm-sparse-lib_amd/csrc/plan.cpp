@@ -1167,7 +1167,11 @@ int ispm_plan_execute_multi(ispm_plan *p, int k, const void *X, size_t ldx, void
 {
   if (!p) return fail_plain(ISPM_ERR_INVALID, "plan_execute_multi: null plan");
   if (!p->has_dev) return fail_plain(ISPM_ERR_INVALID, "plan_execute_multi: plan has no device matrix");
-  if (plan_single_vector(p)) return fail_plain(ISPM_ERR_UNSUPPORTED, "plan_execute_multi: split-diagonal and symmetric plans multiply one vector");
+  if (p->split) return fail_plain(ISPM_ERR_UNSUPPORTED, "plan_execute_multi: split-diagonal plans multiply one vector");
+  if (p->sym) {
+    const int st = ensure_init();
+    return st ? st : spmm_sym(p->symd, p->T, k, X, ldx, Y, ldy);
+  }
   return spmm_internal(&p->launch, &p->dev, p->T, k, X, ldx, Y, ldy, &p->hints,
                        p->has_pat ? &p->pat : nullptr, p->dev_uni);
 }

@@ -37,8 +37,6 @@ struct ispm_plan {
 
 /* rows (padded) of the matrix a plan multiplies */
 inline unsigned plan_rows(const ispm_plan *p) { return p->split ? p->split_n : (unsigned)p->L.n_rows; }
-/* a plan whose product is one vector at a time through its own kernels (no SpMM) */
-inline bool plan_single_vector(const ispm_plan *p) { return p->split || p->sym; }
 
 namespace ispm {
 size_t tsize(int t);

@@ -87,6 +87,28 @@ int spmv_sym(const sym_desc &sd, int vec_type, const void *x, void *y, const uns
   return fail(ISPM_ERR_UNSUPPORTED, "spmv: unsupported (vector, element) types (%d, %d)", vec_type, et);
 }
 
+int spmm_sym(const sym_desc &sd, int vec_type, int k, const void *X, size_t ldx, void *Y, size_t ldy)
+{
+  if (!X || !Y || !sd.elms) return fail(ISPM_ERR_INVALID, "spmm: null argument");
+  if (k < 1 || k > 8) return fail(ISPM_ERR_UNSUPPORTED, "spmm: k=%d outside 1..8", k);
+  if (k > 1 && (ldx < sd.n_cols || ldy < (size_t)sd.n_slices * 256))
+    return fail(ISPM_ERR_INVALID, "spmm: leading dimensions %zu, %zu below the plan's columns and rows", ldx, ldy);
+  if (k == 1) return spmv_sym(sd, vec_type, X, Y, nullptr);
+  if (sd.K < 1 || sd.K > spmvk::kSymPlanes) return fail(ISPM_ERR_INVALID, "spmm: %u symmetric planes", sd.K);
+  if (sd.n_slices == 0) return ISPM_OK;
+  spmvk::sym_args a{};
+  a.elms = sd.elms;
+  a.n_slices = sd.n_slices;
+  a.n_cols = sd.n_cols;
+  a.K = sd.K;
+  for (unsigned q = 0; q < spmvk::kSymPlanes; q++) a.d[q] = sd.d[q];
+  const int et = sd.elm_type;
+  if (vec_type == ISPM_F64 && et == ISPM_F64) return spmvk::launch_symdia_spmm_k<double, double>(a, sd.pat, k, X, ldx, Y, ldy);
+  if (vec_type == ISPM_F64 && et == ISPM_F32) return spmvk::launch_symdia_spmm_k<double, float>(a, sd.pat, k, X, ldx, Y, ldy);
+  if (vec_type == ISPM_F32 && et == ISPM_F32) return spmvk::launch_symdia_spmm_k<float, float>(a, sd.pat, k, X, ldx, Y, ldy);
+  return fail(ISPM_ERR_UNSUPPORTED, "spmm: symmetric plans of (vector, element) types (%d, %d)", vec_type, et);
+}
+
 /* SpMM for plans and the block solver: the checks of ispm_spmm, with the plan's layout facts
  * and its shared diagonal pattern (both may be null) */
 int spmm_internal(const ispm_launch *p, const ispm_slell_desc *A, int vec_type, int k,

@@ -1104,6 +1104,142 @@ int launch_symdia(const sym_args &a0, const diag_pattern &pat0, const void *x, v
   return ISPM_OK;
 }
 
+/* Multi-vector product of symmetric diagonal storage (k = 2..8 vectors, column j at x + j ldx):
+ * one 256-row slice per workgroup, one row per thread, slices in the single-vector kernel's
+ * (level) order.  The row's K own entries and K - 1 mirrors are loaded once; then, KG vectors
+ * at a time, the vectors' x windows are staged in LDS and each row adds, plane by plane, its
+ * upper product and then its mirrored one -- the order of symdia_kernel, so every column is
+ * bit-identical to the single-vector symmetric product. */
+template<typename T, typename E, unsigned KV, unsigned NK>
+__global__ void __launch_bounds__(256)
+symdia_spmm_kernel(sym_args a, diag_pattern pat, size_t ldx, size_t ldy, unsigned kg)
+{
+  extern __shared__ __align__(16) unsigned char sym_lds[];
+  T *win = reinterpret_cast<T *>(sym_lds);
+  constexpr unsigned S = 256, VX = 16 / sizeof(T), QM = 4, PL = kSymPlane;
+  const unsigned t = threadIdx.x;
+  const unsigned xcd = blockIdx.x & 7u, i = blockIdx.x >> 3;
+  unsigned slice = xcd * a.per_xcd + i;
+  if (a.band) {
+    const unsigned q0 = xcd * a.P / 8, w = (xcd + 1) * a.P / 8 - q0;
+    slice = i < w * a.Z ? (i / w) * a.P + q0 + i % w : a.Z * a.P + (i - w * a.Z) * 8 + xcd;
+  }
+  if (slice >= a.n_slices) return;   /* uniform over the workgroup */
+  const E *__restrict__ el = static_cast<const E *>(a.elms);
+  const T *__restrict__ X = static_cast<const T *>(a.x);
+  T *__restrict__ Y = static_cast<T *>(a.y);
+  const unsigned ncols = a.n_cols, K = a.K, L = pat.lds_len, wt = pat.wtotal, nrun = pat.nrun;
+  const unsigned row0 = slice * S, row = row0 + t, KS = K * PL;
+  typedef typename sym_raw<E>::type R;
+  R ev[NK], mv[NK];
+#pragma unroll
+  for (unsigned k = 0; k < NK; k++) {
+    ev[k] = mv[k] = R(0);
+    if (k < K) ev[k] = (R)el[slice * KS + k * PL + t];
+    if (k >= 1 && k < K) {
+      const int j = (int)row - a.d[k];
+      if (j >= 0) mv[k] = (R)el[((unsigned)j >> 8) * KS + k * PL + ((unsigned)j & 255u)];
+    }
+  }
+  const bool fast = ((reinterpret_cast<uintptr_t>(X) | (ldx * sizeof(T))) % 16) == 0 &&
+                    (long long)row0 + pat.cover_lo >= 0 && (long long)row0 + pat.cover_hi <= (long long)ncols;
+  int dq[QM];
+#pragma unroll
+  for (unsigned q = 0; q < QM; q++) {
+    const unsigned vv = t + q * S;
+    int dl = pat.wdelta[0];
+    for (unsigned r = 1; r < nrun; r++)
+      if (vv >= pat.wvbase[r]) dl = pat.wdelta[r];
+    dq[q] = (int)row0 + dl + (int)(vv * VX);
+  }
+  for (unsigned j0 = 0; j0 < KV; j0 += kg) {
+    const unsigned nk = min(kg, KV - j0);
+    if (j0) __syncthreads();   /* the previous vectors' windows are read */
+    for (unsigned jj = 0; jj < nk; jj++) {
+      const T *xj = X + (size_t)(j0 + jj) * ldx;
+      if (fast) {
+#pragma unroll
+        for (unsigned q = 0; q < QM; q++)
+          if (t + q * S < wt)
+            *reinterpret_cast<vecx<T> *>(win + jj * L + (t + q * S) * VX) = *reinterpret_cast<const vecx<T> *>(xj + dq[q]);
+      } else {
+        for (unsigned r = 0; r < nrun; r++) {
+          const long long x0 = (long long)row0 + pat.rrel[r];
+          const unsigned Lr = pat.rlen[r], b = pat.rbase[r];
+          for (unsigned u = t; u < Lr; u += S) {
+            const long long c = x0 + u;
+            win[jj * L + b + u] = (c >= 0 && c < (long long)ncols) ? xj[c] : (T)0;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    for (unsigned jj = 0; jj < nk; jj++) {
+      T acc = 0;
+#pragma unroll
+      for (unsigned k = 0; k < NK; k++)
+        if (k < K) {
+          acc += sym_cvt<T, E>(ev[k]) * win[jj * L + a.up_off[k] + t];
+          if (k >= 1) acc += sym_cvt<T, E>(mv[k]) * win[jj * L + a.lo_off[k] + t];
+        }
+      Y[(size_t)(j0 + jj) * ldy + row] = acc;
+    }
+  }
+}
+
+template<typename T, typename E, unsigned KV>
+int launch_symdia_spmm(const sym_args &a0, const diag_pattern &pat0, const void *X, size_t ldx,
+                       void *Y, size_t ldy)
+{
+  sym_args a = a0;
+  a.x = X;
+  a.y = Y;
+  a.skip = nullptr;
+  a.dotv = nullptr;
+  a.dparts = nullptr;
+  a.per_xcd = (a.n_slices + 7) / 8;
+  /* the single-vector kernel's level order (mirrors stay in the XCD's L2) */
+  const unsigned P = (unsigned)(a.d[a.K - 1] / 256);
+  a.band = 0;
+  if (P >= 8 && (P % 8 == 0 || P >= 128) && a.n_slices / P >= 2) {
+    a.band = 1;
+    a.P = P;
+    a.Z = a.n_slices / P;
+    a.per_xcd = (P + 7) / 8 * a.Z + (a.n_slices - a.Z * P + 7) / 8;
+  }
+  constexpr unsigned VX = 16 / sizeof(T);
+  diag_pattern c;
+  cover_pattern(pat0, VX, 256, kPatPlanes, 1u << 20, c);
+  for (unsigned k = 0; k < a.K; k++) {
+    a.up_off[k] = c.poff[(a.K - 1) + k];
+    a.lo_off[k] = k ? c.poff[(a.K - 1) - k] : 0u;
+  }
+  const unsigned kg = (unsigned)std::max<size_t>(1, std::min<size_t>(KV, 20480 / ((size_t)c.lds_len * sizeof(T))));
+  const size_t lds = (size_t)kg * c.lds_len * sizeof(T);
+  if (lds > 65536 || c.wtotal > 4 * 256)
+    return fail(ISPM_ERR_UNSUPPORTED, "spmm: symmetric windows of %u vectors need %zu bytes of LDS", c.wtotal, lds);
+  const dim3 g(a.per_xcd * 8), b(256);
+  if (a.K <= 10) hipLaunchKernelGGL((symdia_spmm_kernel<T, E, KV, 10>), g, b, lds, stream(), a, c, ldx, ldy, kg);
+  else hipLaunchKernelGGL((symdia_spmm_kernel<T, E, KV, kSymPlanes>), g, b, lds, stream(), a, c, ldx, ldy, kg);
+  ISPM_HIP(hipGetLastError());
+  return ISPM_OK;
+}
+
+template<typename T, typename E>
+int launch_symdia_spmm_k(const sym_args &a, const diag_pattern &pat, int k, const void *X, size_t ldx,
+                         void *Y, size_t ldy)
+{
+  switch (k) {
+    case 2: return launch_symdia_spmm<T, E, 2>(a, pat, X, ldx, Y, ldy);
+    case 3: return launch_symdia_spmm<T, E, 3>(a, pat, X, ldx, Y, ldy);
+    case 4: return launch_symdia_spmm<T, E, 4>(a, pat, X, ldx, Y, ldy);
+    case 5: return launch_symdia_spmm<T, E, 5>(a, pat, X, ldx, Y, ldy);
+    case 6: return launch_symdia_spmm<T, E, 6>(a, pat, X, ldx, Y, ldy);
+    case 7: return launch_symdia_spmm<T, E, 7>(a, pat, X, ldx, Y, ldy);
+    default: return launch_symdia_spmm<T, E, 8>(a, pat, X, ldx, Y, ldy);
+  }
+}
+
 /* pattern path: a plan's diagonal fixed-height slices, one row per thread, single vector */
 inline bool use_diagpat(const ispm_launch *p, const ispm_slell_desc *A, int vt, unsigned H,
                         const diag_pattern *pat, const unsigned *uni)

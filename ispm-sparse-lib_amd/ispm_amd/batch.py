@@ -5,16 +5,18 @@ import ctypes as C
 
 import numpy as np
 
-from . import CSR, F64, PLAN_MIN_BYTES, DeviceArray, Plan, check, lib
+from . import CSR, F64, PLAN_MIN_BYTES, SPMV_PLAN_SYMMETRIC, DeviceArray, Plan, check, lib
 from .solver import PCG, PREC_FP64, SolveStats, SolverOpts
 
 
-def block_solve(A: CSR, Bm, M=None, tol=1e-10, maxit=100000, check_every=16):
+def block_solve(A: CSR, Bm, M=None, tol=1e-10, maxit=100000, check_every=16, symmetric=False):
     """Solve A X[j] = Bm[j] for the k rows of Bm (k <= 8) as one block of PCG recurrences.
+    symmetric: A as a symmetric diagonal plan (ISPM_SPMV_PLAN_SYMMETRIC; the factors' plans
+    must then pad to the same row count, as for n a multiple of 512).
     Returns (X as a k x n array, list of per-column stats dicts)."""
     Bm = np.atleast_2d(np.asarray(Bm, np.float64))
     k = Bm.shape[0]
-    Ap = Plan(A, F64, elm_type=F64)
+    Ap = Plan(A, F64, elm_type=F64, flags=SPMV_PLAN_SYMMETRIC if symmetric else 0)
     Mp = None
     if M is not None:
         Mp = (Plan(M[0], F64, elm_type=F64, kind=PLAN_MIN_BYTES), Plan(M[1], F64, elm_type=F64, kind=PLAN_MIN_BYTES))

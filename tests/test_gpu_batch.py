@@ -30,6 +30,21 @@ def test_block_pcg_matches_single(gpu, k):
     assert abs(stats[0]["iterations"] - r0.iterations) <= 1
 
 
+@pytest.mark.parametrize("k", [2, 5])
+def test_block_pcg_symmetric_storage(gpu, k):
+    """Block PCG with A as a symmetric diagonal plan (its SpMM reads the upper planes and
+    the mirrors once for all columns): every column converges to the oracle's residual, in
+    the iterations of the same block solve with A stored in full (within one)."""
+    A = G.stencil27_contrast(16, block=4)
+    M = I.fastainv_sym(A, I.F64, droptol=0.05)
+    Bm = np.stack([O.csr_spmv(A, G.uniform(700 + j, A.n_rows)) for j in range(k)])
+    Xs, ss = B.block_solve(A, Bm, M=M, tol=1e-10, symmetric=True)
+    Xf, sf = B.block_solve(A, Bm, M=M, tol=1e-10)
+    for j in range(k):
+        assert ss[j]["converged"] and relres(A, Bm[j], Xs[j]) <= 1e-10
+        assert abs(ss[j]["iterations"] - sf[j]["iterations"]) <= 1
+
+
 def test_block_columns_converge_independently(gpu):
     A = G.poisson7(12)
     Bm = np.stack([O.csr_spmv(A, G.uniform(5, A.n_rows)), np.zeros(A.n_rows) + 1e-3,
