@@ -71,10 +71,11 @@ def time_spmv(I, run, steps, warmup, flush=None):
     return float(np.mean(ts)), float(np.median(ts))
 
 
-def spmv_leg(I, G, A, vt, et, steps, warmup, flags=0, flush=None, guarded=False):
+def spmv_leg(I, G, A, vt, et, steps, warmup, flags=0, flush=None, guarded=False, kind=None):
     """One plan's SpMV timed; `guarded`: x is held with the plan's zero guard zones and the
-    product runs through ispm_plan_execute_guarded, as the solver's own vectors are."""
-    plan = I.Plan(A, vt, elm_type=et, flags=flags)
+    product runs through ispm_plan_execute_guarded, as the solver's own vectors are; `kind`:
+    the planning mode (default: the reference's default plan)."""
+    plan = I.Plan(A, vt, elm_type=et, flags=flags, kind=I.PLAN_DEFAULT if kind is None else kind)
     inf = plan.info()
     n = plan.n_rows_padded
     dt = np.float64 if vt == I.F64 else np.float32
@@ -297,7 +298,12 @@ def batching_legs(I, G, O, S):
     t = time.perf_counter()
     A = G.fv7_lognormal(128)
     log(f"generated 128^3 finite-volume: n={A.n_rows} nnz={A.nnz} in {time.perf_counter() - t:.1f}s")
-    out["cfg2_spmv_fp64"] = spmv_leg(I, G, A, I.F64, I.F64, 20, 3, flush=CacheFlush(I))
+    flush = CacheFlush(I)
+    out["cfg2_spmv_fp64"] = spmv_leg(I, G, A, I.F64, I.F64, 20, 3, flush=flush)
+    # the same product with the plan the exhaustive autotune picks (plan.h:97-144)
+    out["cfg2_spmv_fp64_tuned"] = spmv_leg(I, G, A, I.F64, I.F64, 20, 3, flush=flush, kind=I.PLAN_EXHAUSTIVE)
+    log(f"cfg2 spmv: default plan {out['cfg2_spmv_fp64']['avg_s'] * 1e6:.1f} us, tuned "
+        f"{out['cfg2_spmv_fp64_tuned']['avg_s'] * 1e6:.1f} us (cache flushed)")
     M = I.fastainv_sym(A, I.F64, max_elts=64 * A.n_rows)
     b = host_spmv(A, G.uniform(1, A.n_rows))
     sol, _ = solve_leg(I, S, O, A, b, M, "fp64", S.PCG, 3, 1, 1e-8)
