@@ -145,10 +145,10 @@ def cpu_spmv_baseline(O, A, seconds=10.0):
             "seconds_per_spmv": dt}
 
 
-def solve_leg(I, S, O, A, b, factors, precision, method, steps, warmup, tol):
+def solve_leg(I, S, O, A, b, factors, precision, method, steps, warmup, tol, **solver_kw):
     """Time `steps` complete solves (after `warmup` untimed ones) from x = 0; every solve is
     checked with the CPU oracle's FP64 CSR residual."""
-    solver = S.Solver(A, M=factors, precision=precision, method=method, tol=tol)
+    solver = S.Solver(A, M=factors, precision=precision, method=method, tol=tol, **solver_kw)
     n = solver.n
     bd = I.DeviceArray.from_numpy(np.concatenate([b, np.zeros(n - A.n_rows)]))
     xd = I.DeviceArray(n)
@@ -308,7 +308,10 @@ def batching_legs(I, G, O, S):
     b = host_spmv(A, G.uniform(1, A.n_rows))
     sol, _ = solve_leg(I, S, O, A, b, M, "fp64", S.PCG, 3, 1, 1e-8)
     out["cfg2_pcg_fp64"] = sol
-    log(f"cfg2 pcg: {sol['seconds'] * 1e3:.1f} ms, {sol['iterations']} it")
+    sol, _ = solve_leg(I, S, O, A, b, M, "fp64", S.PCG, 3, 1, 1e-8, A_kind=I.PLAN_EXHAUSTIVE)
+    out["cfg2_pcg_fp64_tuned"] = sol
+    log(f"cfg2 pcg: {out['cfg2_pcg_fp64']['seconds'] * 1e3:.1f} ms, {sol['iterations']} it; with the "
+        f"autotuned A plan {sol['seconds'] * 1e3:.1f} ms")
     # cfg4a: block PCG (SpMM over k columns, the two factors as SlELL plans) vs k single solves
     Ap = I.Plan(A, I.F64, elm_type=I.F64)
     Mp = (I.Plan(M[0], I.F64, elm_type=I.F64, kind=I.PLAN_MIN_BYTES),

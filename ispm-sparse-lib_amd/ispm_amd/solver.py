@@ -6,7 +6,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import (BF16, CSR, F32, F64, PLAN_MIN_BYTES, SPMV_PLAN_SPLIT_DIAG, SPMV_PLAN_SYMMETRIC,
+from . import (BF16, CSR, F32, F64, PLAN_DEFAULT, PLAN_MIN_BYTES, SPMV_PLAN_SPLIT_DIAG, SPMV_PLAN_SYMMETRIC,
                DeviceArray, IspmError, Plan, check, csr_multiply, lib)
 
 PPCG, PCG, PCG1 = 0, 1, 2
@@ -101,11 +101,14 @@ class Solver:
     as their product (see preconditioner_matrices; fsai_product chooses), or one CSR matrix;
     each matrix as a SlELL plan or a split-diagonal plan (split_diag chooses; None: the
     faster, timed at set-up).  A symmetric matrix on few diagonals may use symmetric diagonal
-    storage for A (symmetric chooses; None: the faster, timed at set-up)."""
+    storage for A (symmetric chooses; None: the faster, timed at set-up).  A_kind: how A's
+    SlELL plan is made (PLAN_DEFAULT, the reference's guess, unless given; PLAN_EXHAUSTIVE
+    times the candidate formats, plan.h:97-144)."""
 
     def __init__(self, A: CSR, M=None, precision="fp64", method=None, tol=1e-10, maxit=100000,
                  check_every=16, use_graph=True, reliable_delta=0.01, factor_type=None,
-                 A_plan=None, M_plans=None, fsai_product=None, split_diag=None, symmetric=None):
+                 A_plan=None, M_plans=None, fsai_product=None, split_diag=None, symmetric=None,
+                 A_kind=None):
         L = _bind()
         mixed = precision == "mixed"
         vt = F32 if mixed else F64
@@ -117,7 +120,7 @@ class Solver:
             # storage that streams half the matrix: the faster of the two with the solve's
             # vectors (the mixed solve's FP64 residuals use the same plan with FP64 vectors)
             ea = F32 if mixed else F64
-            cands = [] if symmetric else [Plan(A, vt, elm_type=ea)]
+            cands = [] if symmetric else [Plan(A, vt, elm_type=ea, kind=PLAN_DEFAULT if A_kind is None else A_kind)]
             if symmetric is not False:
                 try:
                     cands.append(Plan(A, vt, elm_type=ea, flags=SPMV_PLAN_SYMMETRIC))
