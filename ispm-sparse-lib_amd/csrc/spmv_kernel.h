@@ -909,12 +909,14 @@ symdia_r4_rows(const sym_args &a, const diag_pattern &pat, const float *__restri
   /* windows go straight from memory to LDS (16 bytes per lane), so every plane load can be
    * issued before the first wait; run by run, 64 vectors u .. u + 63 of a run per load, so
    * the run's offset is uniform and a lane's address is one add */
-  for (unsigned r = 0; r < pat.nrun; r++) {
-    const unsigned vb = pat.wvbase[r], ve = r + 1 < pat.nrun ? pat.wvbase[r + 1] : pat.wtotal;
-    const float *xr = xv + (row0 + pat.wdelta[r]);
-    for (unsigned u = vb; u < ve; u += 64)
-      if (u + lane < ve) lds_dma16(xr + 4 * (u + lane), win + 4 * u);
-  }
+#pragma unroll
+  for (unsigned r = 0; r < 9; r++)   /* fixed bound: the run table is read once, as scalars */
+    if (r < pat.nrun) {
+      const unsigned vb = pat.wvbase[r], ve = r + 1 < pat.nrun ? pat.wvbase[r + 1] : pat.wtotal;
+      const float *xr = xv + (row0 + pat.wdelta[r]);
+      for (unsigned u = vb; u < ve; u += 64)
+        if (u + lane < ve) lds_dma16(xr + 4 * (u + lane), win + 4 * u);
+    }
   /* 32-bit byte offsets from the uniform base (the stored planes are below 4 GiB): one
    * address register per load */
   const char *eb = reinterpret_cast<const char *>(el);
