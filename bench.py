@@ -286,6 +286,50 @@ def hblock_legs(I, G, O, steps, warmup):
     return out
 
 
+def spmm_legs(I, G, O, steps, warmup):
+    """Multi-vector products (k = 3 and 8 columns, column-major with leading dimension n) of a
+    7-point and a 27-point 128^3 stencil, FP64 and FP32, with the default (diagonal-pattern)
+    plan and with symmetric diagonal storage: times against the single-vector product, the
+    bytes of the matrix plus k columns read and written, and column k-1 compared bit for bit
+    with the plan's single-vector product and against the oracle."""
+    out = {}
+    for mname, A in (("7pt128", G.poisson7(128)), ("27pt128", G.stencil27_contrast(128))):
+        x = G.uniform(7, 8 * A.n_rows)
+        for tname, vt in (("fp64", I.F64), ("fp32", I.F32)):
+            dt = np.float64 if vt == I.F64 else np.float32
+            for sname, fl in (("pattern", 0), ("symmetric", I.SPMV_PLAN_SYMMETRIC)):
+                plan = I.Plan(A, vt, elm_type=vt, flags=fl)
+                inf = plan.info()
+                n = plan.n_rows_padded
+                X = np.zeros((8, n), dt)
+                X[:, :A.n_rows] = x.reshape(8, A.n_rows)
+                Xd = I.DeviceArray.from_numpy(X.ravel())
+                Yd = I.DeviceArray(8 * n, dt)
+                y1 = I.DeviceArray(n, dt)
+                leg = {"format": dict(S=inf.S, D=inf.D, symmetric=bool(fl)), "stored_bytes": inf.stored_bytes,
+                       "pattern_planes": inf.pattern_planes}
+                t1, _ = time_spmv(I, lambda: plan.execute(Xd, y1), steps, warmup)
+                leg["k1_s"] = t1
+                vec = (n + A.n_rows) * np.dtype(dt).itemsize
+                for k in (3, 8):
+                    tk, _ = time_spmv(I, lambda: plan.execute_multi(k, Xd, n, Yd, n), steps, warmup)
+                    xk = I.DeviceArray.from_numpy(np.ascontiguousarray(X[k - 1]))
+                    plan.execute(xk, y1)
+                    I.synchronize()
+                    yk = Yd.numpy()[(k - 1) * n:k * n]
+                    same = bool(np.array_equal(yk, y1.numpy()))
+                    xr = X[k - 1, :A.n_rows].astype(np.float64)
+                    err = float(np.max(np.abs(yk[:A.n_rows] - O.csr_spmv(A, xr)) / np.maximum(O.csr_spmv_abs(A, xr), 1e-300)))
+                    byts = inf.stored_bytes - vec + k * vec
+                    leg[f"k{k}"] = {"seconds": tk, "GBps": byts / tk * 1e-9, "per_column_vs_k1": tk / k / t1,
+                                    "column_bit_identical": same, "max_rel_err_vs_oracle": err}
+                    log(f"spmm {mname} {tname} {sname} k={k}: {tk * 1e6:.1f} us ({byts / tk * 1e-9:.0f} GB/s, "
+                        f"{tk / k / t1:.2f} of a single product per column), same {same}, rel err {err:.1e}")
+                out[f"{mname}_{tname}_{sname}"] = leg
+                del plan, Xd, Yd, y1
+    return out
+
+
 def batching_legs(I, G, O, S):
     """The other workload configurations, timed on the device with inputs resident: the 128^3
     finite-volume operator (FP64 SpMV and FSAI-PCG to 1e-8), its 3 and 8 right-hand sides as
@@ -451,6 +495,8 @@ def main():
     ap.add_argument("--droptol-sweep", action="store_true",
                     help="also time the FP64 and mixed PCG solves with the factors of drop tolerances "
                          "2^-4 and 2^-5 (set-up on the host + solve, one line of its own)")
+    ap.add_argument("--spmm", action="store_true",
+                    help="multi-vector products of 7- and 27-point 128^3 stencils (pattern and symmetric plans)")
     ap.add_argument("--hblock", action="store_true",
                     help="time the SpMV of a banded two-unknowns-per-node matrix stored as hblock pairs, "
                          "LDS x windows against global gathers (one JSON line of its own)")
@@ -477,6 +523,10 @@ def main():
         log("calibration: 3 device-to-device copies of 1 GiB")
     if args.foam:
         print(json.dumps({"metric": "foam-extend call pattern", "legs": foam_legs(I, G, O)}))
+        return
+    if args.spmm:
+        print(json.dumps({"metric": "SpMM of stencil plans, k = 3 and 8",
+                          "legs": spmm_legs(I, G, O, max(args.steps, 10), args.warmup)}))
         return
     if args.hblock:
         print(json.dumps({"metric": "hblock SpMV, LDS x windows vs global gathers",
