@@ -866,12 +866,12 @@ __global__ void __launch_bounds__((256 / sym_rows_per_thread<T, E>())) symdia_ke
 /* FP32 vectors and elements, four rows per lane.  The kernel above is bound by the rate at
  * which the vector memory pipeline accepts load instructions (address unit busy ~85% of the
  * time), not by bytes: here every load moves 16 bytes per lane, so a 256-row slice is one
- * wave that issues 14 own-plane loads, 13 mirror loads (4 consecutive rows at 4-byte
- * alignment, inside the plane thanks to its repeated rows) and about 10 window loads, against
+ * wave that issues 14 own-plane loads, 12 or 13 mirror loads (4 consecutive rows at 4-byte
+ * alignment, inside the plane thanks to its repeated rows) and about 9 window loads, against
  * about 80 load instructions over two waves above.  A workgroup is one wave with its own LDS
  * windows: no barrier, and LDS and registers are handed on as each wave ends (four waves per
  * group ran 6% slower, 103.6 against 97 us at 27 points 192^3).  The windows go from memory
- * to LDS directly, so all 37 loads are issued before the one wait, and the products are
+ * to LDS directly, so every load is issued before the one wait, and the products are
  * kept in plane order (a compiler barrier per plane): left free, the compiler read every
  * plane's x values from LDS before the first product and needed 221 registers (two waves
  * per SIMD); this way 120 (four).  x must be readable, and zero outside [0, n_cols), on every
