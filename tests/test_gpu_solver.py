@@ -74,6 +74,20 @@ def test_symmetric_storage_of_A(gpu, prec, method):
     assert abs(res.iterations - ref.iterations) <= max(3, ref.iterations // 20)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_autotuned_A_plan(gpu, prec):
+    """A's SlELL plan from the exhaustive autotune (Solver(A_kind=PLAN_EXHAUSTIVE)) on an
+    RCM-ordered finite-volume matrix: the solve reaches the oracle's residual in the
+    iterations of the default plan's solve (up to rounding: within 5%)."""
+    A = G.fv7_lognormal(24)
+    b = O.csr_spmv(A, G.uniform(9, A.n_rows))
+    M = I.fastainv_sym(A, I.F64, droptol=0.05)
+    ref = S.solve(A, b, M=M, tol=1e-9, precision=prec, method=S.PCG)
+    res = S.solve(A, b, M=M, tol=1e-9, precision=prec, method=S.PCG, A_kind=I.PLAN_EXHAUSTIVE)
+    assert res.converged and oracle_relres(A, b, res.x) <= 1e-9
+    assert abs(res.iterations - ref.iterations) <= max(3, ref.iterations // 20)
+
+
 @pytest.mark.parametrize("prec,method", VARIANTS)
 def test_no_preconditioner_and_nonzero_guess(gpu, prec, method):
     A = G.random_sparse(3000, 41, max_len=12, spd=True)
