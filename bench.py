@@ -366,7 +366,7 @@ def batching_legs(I, G, O, S):
         Bm = np.zeros((k, n))
         for j in range(k):
             Bm[j, :A.n_rows] = host_spmv(A, G.uniform(300 + j, A.n_rows))
-        opts = S.SolverOpts(S.PCG, S.PREC_FP64, 1e-8, 100000, 16, 0, 0.1)
+        opts = S.SolverOpts(S.PCG, S.PREC_FP64, 1e-8, 100000, 16, 1, 0.1)   # graph-replayed batches
         bs = C.c_void_p()
         I.check(I.lib().ispm_block_solver_create(C.byref(bs), Ap.p, Mp[0].p, Mp[1].p, k, C.byref(opts)))
         bd = I.DeviceArray.from_numpy(Bm.ravel())

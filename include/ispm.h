@@ -436,7 +436,8 @@ int ispm_plan_cache_store(const char *path, unsigned long long fp, int vec_type,
 /* k = 1..8 right-hand sides sharing one matrix: independent FP64 PCG recurrences per column
  * whose matrix products are SpMMs (one pass over A, Ainv1, Ainv2 per iteration for all
  * columns).  B, X: device, column j at B + j*ldb / X + j*ldx (padded length); X holds the
- * initial guesses.  stats: array of k.  No reference counterpart. */
+ * initial guesses.  stats: array of k.  opts->use_graph: batches of check_every iterations
+ * are captured once (per X) and replayed as a hipGraph.  No reference counterpart. */
 typedef struct ispm_block_solver ispm_block_solver;
 int ispm_block_solver_create(ispm_block_solver **out, ispm_plan *A, ispm_plan *Ainv1,
                              ispm_plan *Ainv2, int k, const ispm_solver_opts *opts);

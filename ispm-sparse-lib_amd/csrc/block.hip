@@ -7,6 +7,7 @@
  * No counterpart in the reference (its caller solves one right-hand side at a time). */
 #include <hip/hip_runtime.h>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 #include "device.h"
@@ -138,6 +139,12 @@ struct ispm_block_solver {
   double *R = nullptr, *Z = nullptr, *P = nullptr, *Q = nullptr, *T = nullptr;
   colsc *cs = nullptr, *hcs = nullptr;
   double *parts = nullptr;
+  /* a batch of check_every iterations captured as a hipGraph, for the X it was captured with */
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double *graph_X = nullptr;
+  size_t graph_ldx = 0;
+  int graph_iters = 0;
 };
 
 namespace {
@@ -193,6 +200,40 @@ int iterate(ispm_block_solver *S, double *X, size_t ldx)
   hipLaunchKernelGGL(k_mxr, g, b, 0, sm, X, ldx, S->R, (const double *)S->P, (const double *)S->Q, ld, n, S->cs, S->parts);
   mfinish<OP_XR>(S);
   ISPM_HIP(hipGetLastError());
+  return ISPM_OK;
+}
+
+/* it iterations, replayed from a captured hipGraph when opts.use_graph is set (the
+ * kernels read the per-column flags and scalars on the device, so a batch needs no host
+ * step); eager launches otherwise or under ISPM_NO_GRAPH=1 */
+int run_batch(ispm_block_solver *S, double *X, size_t ldx, int it)
+{
+  const char *ng = std::getenv("ISPM_NO_GRAPH");
+  if (!S->o.use_graph || (ng && ng[0] == '1')) {
+    for (int i = 0; i < it; i++) {
+      const int st = iterate(S, X, ldx);
+      if (st) return st;
+    }
+    return ISPM_OK;
+  }
+  if (!S->exec || S->graph_iters != it || S->graph_X != X || S->graph_ldx != ldx) {
+    if (S->exec) { (void)hipGraphExecDestroy(S->exec); S->exec = nullptr; }
+    if (S->graph) { (void)hipGraphDestroy(S->graph); S->graph = nullptr; }
+    hipStream_t sm = stream();
+    ISPM_HIP(hipStreamBeginCapture(sm, hipStreamCaptureModeThreadLocal));
+    int st = ISPM_OK;
+    for (int i = 0; i < it && !st; i++) st = iterate(S, X, ldx);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(sm, &g);
+    if (st) { if (g) (void)hipGraphDestroy(g); return st; }
+    if (e != hipSuccess) return hip_status(e, "hipStreamEndCapture");
+    S->graph = g;
+    ISPM_HIP(hipGraphInstantiate(&S->exec, g, nullptr, nullptr, 0));
+    S->graph_iters = it;
+    S->graph_X = X;
+    S->graph_ldx = ldx;
+  }
+  ISPM_HIP(hipGraphLaunch(S->exec, stream()));
   return ISPM_OK;
 }
 
@@ -285,8 +326,7 @@ int ispm_block_solver_solve(ispm_block_solver *S, const double *B, size_t ldb, d
   std::vector<int> converged(k, 0), repl(k, 0);
   int total_iters = 0;
   for (int guard = 0; guard < 100000; guard++) {
-    for (int i = 0; i < S->o.check_every; i++)
-      if ((st = iterate(S, X, ldx))) return st;
+    if ((st = run_batch(S, X, ldx, S->o.check_every))) return st;
     if ((st = pull(S))) return st;
     int active = 0, maxit = 0;
     for (int j = 0; j < k; j++) { active += S->hcs[j].active ? 1 : 0; maxit = std::max(maxit, (int)S->hcs[j].iters); }
@@ -342,6 +382,8 @@ void ispm_block_solver_destroy(ispm_block_solver *S)
   ispm_free(S->cs);
   ispm_free(S->parts);
   if (S->hcs) (void)hipHostFree(S->hcs);
+  if (S->exec) (void)hipGraphExecDestroy(S->exec);
+  if (S->graph) (void)hipGraphDestroy(S->graph);
   delete S;
 }
 

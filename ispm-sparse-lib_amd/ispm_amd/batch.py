@@ -9,10 +9,12 @@ from . import CSR, F64, PLAN_MIN_BYTES, SPMV_PLAN_SYMMETRIC, DeviceArray, Plan, 
 from .solver import PCG, PREC_FP64, SolveStats, SolverOpts
 
 
-def block_solve(A: CSR, Bm, M=None, tol=1e-10, maxit=100000, check_every=16, symmetric=False):
+def block_solve(A: CSR, Bm, M=None, tol=1e-10, maxit=100000, check_every=16, symmetric=False,
+                use_graph=True):
     """Solve A X[j] = Bm[j] for the k rows of Bm (k <= 8) as one block of PCG recurrences.
     symmetric: A as a symmetric diagonal plan (ISPM_SPMV_PLAN_SYMMETRIC; the factors' plans
-    must then pad to the same row count, as for n a multiple of 512).
+    must then pad to the same row count, as for n a multiple of 512).  use_graph: replay
+    batches of check_every iterations from a captured hipGraph (else eager launches).
     Returns (X as a k x n array, list of per-column stats dicts)."""
     Bm = np.atleast_2d(np.asarray(Bm, np.float64))
     k = Bm.shape[0]
@@ -21,7 +23,7 @@ def block_solve(A: CSR, Bm, M=None, tol=1e-10, maxit=100000, check_every=16, sym
     if M is not None:
         Mp = (Plan(M[0], F64, elm_type=F64, kind=PLAN_MIN_BYTES), Plan(M[1], F64, elm_type=F64, kind=PLAN_MIN_BYTES))
     n = Ap.n_rows_padded
-    opts = SolverOpts(PCG, PREC_FP64, tol, maxit, check_every, 0, 0.1)
+    opts = SolverOpts(PCG, PREC_FP64, tol, maxit, check_every, int(use_graph), 0.1)
     s = C.c_void_p()
     check(lib().ispm_block_solver_create(C.byref(s), Ap.p, Mp[0].p if Mp else None, Mp[1].p if Mp else None,
                                          k, C.byref(opts)))

@@ -45,6 +45,20 @@ def test_block_pcg_symmetric_storage(gpu, k):
         assert abs(ss[j]["iterations"] - sf[j]["iterations"]) <= 1
 
 
+def test_block_graph_and_eager_agree(gpu):
+    """Batches replayed from the captured hipGraph (several replays, with the host's checks
+    and a restart in between) give the eager launches' X bit for bit and the same counts."""
+    A = G.fv7_lognormal(16, rcm=True)
+    M = I.fastainv_sym(A, I.F64, droptol=0.05)
+    Bm = np.stack([O.csr_spmv(A, G.uniform(800 + j, A.n_rows)) for j in range(4)])
+    Xg, sg = B.block_solve(A, Bm, M=M, tol=1e-10, check_every=8, use_graph=True)
+    Xe, se = B.block_solve(A, Bm, M=M, tol=1e-10, check_every=8, use_graph=False)
+    np.testing.assert_array_equal(Xg, Xe)
+    for j in range(4):
+        assert sg[j]["iterations"] == se[j]["iterations"] and sg[j]["replacements"] == se[j]["replacements"]
+        assert sg[j]["converged"] and relres(A, Bm[j], Xg[j]) <= 1e-10
+
+
 def test_block_columns_converge_independently(gpu):
     A = G.poisson7(12)
     Bm = np.stack([O.csr_spmv(A, G.uniform(5, A.n_rows)), np.zeros(A.n_rows) + 1e-3,
