@@ -572,14 +572,15 @@ slell_diagpat_spmm_kernel(spmv_args a, diag_pattern pat, const unsigned *__restr
   if (on_pattern) {
     const unsigned np = pat.n, nrun = pat.nrun, L = pat.lds_len, wt = pat.wtotal;
     /* 16-byte window loads when every column is aligned and the covers lie inside x */
-    const bool fast = ((reinterpret_cast<uintptr_t>(X) | (ldx * sizeof(T))) % 16) == 0 &&
+    /* this thread's window vectors are t + q S, q < QM, with their offsets in x (the same for
+     * every vector) computed once; longer window lists, unaligned columns and covers
+     * reaching past x stage through the bounds-checked loop */
+    constexpr unsigned QM = 4;
+    const bool fast = ((reinterpret_cast<uintptr_t>(X) | (ldx * sizeof(T))) % 16) == 0 && wt <= QM * S &&
                       (long long)row0 + pat.cover_lo >= 0 && (long long)row0 + pat.cover_hi <= (long long)ncols;
     T ev[NPM];
 #pragma unroll
     for (unsigned j = 0; j < NPM; j++) ev[j] = j < np ? elt<T, E>::get(el, e0 + (j << LOG2S) + t) : (T)0;
-    /* this thread's window vectors t + q S (the launch checks wt <= QM S) and their offsets
-     * in x, the same for every vector */
-    constexpr unsigned QM = 4;
     int dq[QM];
 #pragma unroll
     for (unsigned q = 0; q < QM; q++) {
@@ -682,8 +683,7 @@ int launch_diagpat_spmm(const ispm_slell_desc *A, const diag_pattern &pat, const
   cover_pattern(pat, 16 / sizeof(T), 256, kPatPlanes, 1u << 20, c);
   const unsigned kg = diagpat_spmm_group<T>(c, K);
   const size_t lds = (size_t)kg * c.lds_len * sizeof(T);
-  if (lds > 65536 || c.wtotal > 4 * 256)
-    return fail(ISPM_ERR_UNSUPPORTED, "spmm: pattern windows of %u vectors need %zu bytes of LDS", c.wtotal, lds);
+  if (lds > 65536) return fail(ISPM_ERR_UNSUPPORTED, "spmm: pattern windows need %zu bytes of LDS", lds);
   const dim3 g(a.per_xcd * 8), b(256);
   if (pat.n <= 8) hipLaunchKernelGGL((slell_diagpat_spmm_kernel<T, E, K, 8>), g, b, lds, stream(), a, c, uni, kg);
   else hipLaunchKernelGGL((slell_diagpat_spmm_kernel<T, E, K, 27>), g, b, lds, stream(), a, c, uni, kg);
@@ -1143,7 +1143,9 @@ symdia_spmm_kernel(sym_args a, diag_pattern pat, size_t ldx, size_t ldy, unsigne
       if (j >= 0) mv[k] = (R)el[((unsigned)j >> 8) * KS + k * PL + ((unsigned)j & 255u)];
     }
   }
-  const bool fast = ((reinterpret_cast<uintptr_t>(X) | (ldx * sizeof(T))) % 16) == 0 &&
+  /* window vectors t + q S, q < QM, offsets computed once (longer lists, unaligned columns
+   * and covers reaching past x stage through the bounds-checked loop) */
+  const bool fast = ((reinterpret_cast<uintptr_t>(X) | (ldx * sizeof(T))) % 16) == 0 && wt <= QM * S &&
                     (long long)row0 + pat.cover_lo >= 0 && (long long)row0 + pat.cover_hi <= (long long)ncols;
   int dq[QM];
 #pragma unroll
@@ -1218,8 +1220,7 @@ int launch_symdia_spmm(const sym_args &a0, const diag_pattern &pat0, const void 
   }
   const unsigned kg = (unsigned)std::max<size_t>(1, std::min<size_t>(KV, 20480 / ((size_t)c.lds_len * sizeof(T))));
   const size_t lds = (size_t)kg * c.lds_len * sizeof(T);
-  if (lds > 65536 || c.wtotal > 4 * 256)
-    return fail(ISPM_ERR_UNSUPPORTED, "spmm: symmetric windows of %u vectors need %zu bytes of LDS", c.wtotal, lds);
+  if (lds > 65536) return fail(ISPM_ERR_UNSUPPORTED, "spmm: symmetric windows need %zu bytes of LDS", lds);
   const dim3 g(a.per_xcd * 8), b(256);
   if (a.K <= 10) hipLaunchKernelGGL((symdia_spmm_kernel<T, E, KV, 10>), g, b, lds, stream(), a, c, ldx, ldy, kg);
   else hipLaunchKernelGGL((symdia_spmm_kernel<T, E, KV, kSymPlanes>), g, b, lds, stream(), a, c, ldx, ldy, kg);
@@ -1260,10 +1261,10 @@ inline bool use_diagpat_spmm(const ispm_launch *p, const ispm_slell_desc *A, int
   if (!use_diagpat(p, A, vt, 1, pat, uni) || A->slice_height != 256 || pat->n > 27 ||
       !((vt == ISPM_F64 && A->elm_type == ISPM_F64) || (vt == ISPM_F32 && A->elm_type == ISPM_F32)))
     return false;
-  /* the kernel's window list: at most four vectors per thread, one vector's windows in LDS */
+  /* at least one vector's windows fit in LDS */
   diag_pattern c;
   cover_pattern(*pat, 16 / type_size(vt), 256, kPatPlanes, 1u << 20, c);
-  return c.wtotal <= 4 * 256 && (size_t)c.lds_len * type_size(vt) <= 65536;
+  return (size_t)c.lds_len * type_size(vt) <= 65536;
 }
 
 template<unsigned K>

@@ -247,7 +247,7 @@ def off_pattern(A, rows, seed=17):
 
 
 @pytest.mark.parametrize("vt,et", [(I.F64, I.F64), (I.F32, I.F32), (I.F64, I.F32)])
-@pytest.mark.parametrize("name", ["27pt16", "7pt32", "19pt24_off", "fsaiprod7pt32"])
+@pytest.mark.parametrize("name", ["27pt16", "7pt32", "19pt24_off", "fsaiprod7pt32", "wide27"])
 def test_pattern_spmm_matches_spmv(gpu, name, vt, et):
     """Multi-vector products of 256-row diagonal plans with a shared pattern (a block
     solver's stencil A or preconditioner product): every column bit-identical to the single-vector product of
@@ -258,6 +258,8 @@ def test_pattern_spmm_matches_spmv(gpu, name, vt, et):
         A = G.stencil27_contrast(16, block=4)
     elif name == "7pt32":
         A = G.poisson7(32)
+    elif name == "wide27":   # FP64 windows beyond four vectors per thread
+        A = G.stencil(300, 8, 6, 27, seed=8)
     elif name == "19pt24_off":
         A = off_pattern(G.stencil(24, 24, 24, points=19), {5, 700, 701, 9000, 13000})
     else:   # the approximate inverse's product Ainv2 Ainv1 (13 diagonals on most slices)
@@ -395,6 +397,9 @@ def _sym_matrix(name):
         # widest offset 4225 spans 16 slices: level order, two positions per XCD, ten
         # slices past the last full level
         return G.stencil(64, 65, 40, 27, seed=6)
+    if name == "wide27":
+        # 27-point on 300-wide lines: the FP64 window list exceeds four vectors per thread
+        return G.stencil(300, 8, 6, 27, seed=8)
     if name == "levels131":
         # widest offset 33668 spans 131 slices: level order with bands of 16 and 17
         # positions, one slice past the last full level
@@ -417,7 +422,7 @@ def test_symmetric_plan(gpu, name, vt, et):
 
 
 @pytest.mark.parametrize("vt,et", [(I.F64, I.F64), (I.F64, I.F32), (I.F32, I.F32)])
-@pytest.mark.parametrize("name", ["poisson7", "stencil27_contrast", "small27", "levels16", "levels131"])
+@pytest.mark.parametrize("name", ["poisson7", "stencil27_contrast", "small27", "levels16", "levels131", "wide27"])
 def test_symmetric_spmm_matches_spmv(gpu, name, vt, et):
     """Multi-vector products of symmetric diagonal plans: every column bit-identical to the
     plan's single-vector product, for aligned and unaligned column strides, and within the
