@@ -22,6 +22,9 @@ namespace {
 struct colsc {
   double rho, pq, alpha, beta, rr, tol2, r64;
   unsigned active, iters;
+  /* X_j += alpha P_j of the last iteration not yet applied: the next k_mp applies it (it reads
+   * P_j anyway), or k_mflush once the column stops */
+  unsigned pend;
 };
 
 constexpr int kMaxK = 8;
@@ -67,39 +70,58 @@ __global__ void __launch_bounds__(kFinThreads) k_mfin(const double *parts, int G
   if (OP == OP_R64) c->r64 = r;
   if (OP == OP_XR) {
     c->rr = r;
+    c->pend = 1;
     c->iters++;
     if (r <= c->tol2) c->active = 0;
   }
 }
 
-/* P_j = Z_j + beta_j P_j */
-__global__ void __launch_bounds__(256) k_mp(double *__restrict__ P, const double *__restrict__ Z, size_t ld, int n, const colsc *cs)
+/* X_j += alpha_j P_j (the previous iteration's update, if pending: alpha_j is still that
+ * iteration's and P_j not yet overwritten); P_j = Z_j + beta_j P_j */
+__global__ void __launch_bounds__(256) k_mp(double *__restrict__ X, size_t ldx, double *__restrict__ P,
+                                            const double *__restrict__ Z, size_t ld, int n, const colsc *cs)
 {
   const int j = blockIdx.y;
   if (!cs[j].active) return;
-  const double b = cs[j].beta;
-  double *p = P + j * ld;
+  const double b = cs[j].beta, a = cs[j].pend ? cs[j].alpha : 0.0;
+  double *p = P + j * ld, *x = X + j * ldx;
   const double *z = Z + j * ld;
   const int gt = blockIdx.x * 256 + threadIdx.x, gs = gridDim.x * 256;
-  for (int i = gt; i < n; i += gs) p[i] = z[i] + b * p[i];
+  for (int i = gt; i < n; i += gs) {
+    const double pi = p[i];
+    x[i] += a * pi;
+    p[i] = z[i] + b * pi;
+  }
 }
 
-/* X_j += alpha_j P_j; R_j -= alpha_j Q_j; (R_j, R_j); column done below tol2 */
+/* the pending X_j += alpha_j P_j of columns that stopped */
+__global__ void __launch_bounds__(256) k_mflush(double *__restrict__ X, size_t ldx, const double *__restrict__ P,
+                                                size_t ld, int n, const colsc *cs)
+{
+  const int j = blockIdx.y;
+  if (!cs[j].pend) return;
+  const double a = cs[j].alpha;
+  double *x = X + j * ldx;
+  const double *p = P + j * ld;
+  const int gt = blockIdx.x * 256 + threadIdx.x, gs = gridDim.x * 256;
+  for (int i = gt; i < n; i += gs) x[i] += a * p[i];
+}
+
+/* R_j -= alpha_j Q_j; (R_j, R_j); column done below tol2 (its X update follows in k_mp or
+ * k_mflush) */
 __global__ void __launch_bounds__(256)
-k_mxr(double *__restrict__ X, size_t ldx, double *__restrict__ R, const double *__restrict__ P,
-      const double *__restrict__ Q, size_t ld, int n, const colsc *cs, double *parts)
+k_mxr(double *__restrict__ R, const double *__restrict__ Q, size_t ld, int n, const colsc *cs, double *parts)
 {
   __shared__ double lds[4];
   const int j = blockIdx.y;
   const colsc *c = cs + j;
   if (!c->active) return;
   const double a = c->alpha;
-  double *x = X + j * ldx, *r = R + j * ld;
-  const double *p = P + j * ld, *q = Q + j * ld;
+  double *r = R + j * ld;
+  const double *q = Q + j * ld;
   double v[1] = {0.0};
   const int gt = blockIdx.x * 256 + threadIdx.x, gs = gridDim.x * 256;
   for (int i = gt; i < n; i += gs) {
-    x[i] += a * p[i];
     const double ri = r[i] - a * q[i];
     r[i] = ri;
     v[0] += ri * ri;
@@ -193,11 +215,11 @@ int iterate(ispm_block_solver *S, double *X, size_t ldx)
   }
   hipLaunchKernelGGL((k_mdot<OP_RZ>), g, b, 0, sm, (const double *)S->R, (const double *)S->Z, ld, n, S->cs, S->parts);
   mfinish<OP_RZ>(S);
-  hipLaunchKernelGGL(k_mp, g, b, 0, sm, S->P, (const double *)S->Z, ld, n, (const colsc *)S->cs);
+  hipLaunchKernelGGL(k_mp, g, b, 0, sm, X, ldx, S->P, (const double *)S->Z, ld, n, (const colsc *)S->cs);
   if ((st = mspmm(S->A, k, S->P, ld, S->Q, ld))) return st;
   hipLaunchKernelGGL((k_mdot<OP_PQ>), g, b, 0, sm, (const double *)S->P, (const double *)S->Q, ld, n, S->cs, S->parts);
   mfinish<OP_PQ>(S);
-  hipLaunchKernelGGL(k_mxr, g, b, 0, sm, X, ldx, S->R, (const double *)S->P, (const double *)S->Q, ld, n, S->cs, S->parts);
+  hipLaunchKernelGGL(k_mxr, g, b, 0, sm, S->R, (const double *)S->Q, ld, n, S->cs, S->parts);
   mfinish<OP_XR>(S);
   ISPM_HIP(hipGetLastError());
   return ISPM_OK;
@@ -332,7 +354,13 @@ int ispm_block_solver_solve(ispm_block_solver *S, const double *B, size_t ldb, d
     for (int j = 0; j < k; j++) { active += S->hcs[j].active ? 1 : 0; maxit = std::max(maxit, (int)S->hcs[j].iters); }
     total_iters = maxit;
     if (active && maxit < S->o.maxit) continue;
-    /* every column stopped (or out of iterations): check the true residuals */
+    /* every column stopped (or out of iterations): apply the pending X updates, then check
+     * the true residuals */
+    {
+      dim3 g(S->grid, k), b(256);
+      hipLaunchKernelGGL(k_mflush, g, b, 0, stream(), X, ldx, (const double *)S->P, S->ld, S->n, (const colsc *)S->cs);
+      ISPM_HIP(hipGetLastError());
+    }
     std::vector<colsc> before(S->hcs, S->hcs + k);
     if ((st = residuals(S, B, ldb, X, ldx)) || (st = pull(S))) return st;
     int again = 0;
@@ -341,6 +369,7 @@ int ispm_block_solver_solve(ispm_block_solver *S, const double *B, size_t ldb, d
       const double rel = std::sqrt(c.r64) / bn[j];
       converged[j] = rel <= tol;
       c.iters = before[j].iters;
+      c.pend = 0;   /* flushed above */
       if (!converged[j] && maxit < S->o.maxit) {
         /* drifted: restart this column from the true residual with a tighter target */
         rec[j] *= 0.5;

@@ -59,6 +59,31 @@ def test_block_graph_and_eager_agree(gpu):
         assert sg[j]["converged"] and relres(A, Bm[j], Xg[j]) <= 1e-10
 
 
+def test_block_maxit_stop_applies_pending_updates(gpu):
+    """Stopped by maxit, every column's X holds all its updates (the last one is applied by
+    the flush, not by a next iteration): it matches a single-RHS PCG stopped after as many
+    updates, within rounding."""
+    from ispm_amd import solver as S
+    A = G.fv7_lognormal(16, rcm=True)
+    M = I.fastainv_sym(A, I.F64, droptol=0.05)
+    Bm = np.stack([O.csr_spmv(A, G.uniform(950 + j, A.n_rows)) for j in range(3)])
+    X, st = B.block_solve(A, Bm, M=M, tol=1e-14, maxit=8, check_every=4)
+    assert all(s_["iterations"] == 8 and not s_["converged"] for s_ in st)
+    for j in range(3):
+        ref = None
+        for m in (7, 8, 9):
+            r = S.solve(A, Bm[j], M=M, tol=1e-14, precision="fp64", method=S.PCG, maxit=m, check_every=4,
+                        fsai_product=False, split_diag=False, symmetric=False)
+            if r.iterations == 8:
+                ref = r
+                break
+        assert ref is not None, [S.solve(A, Bm[j], M=M, tol=1e-14, precision="fp64", method=S.PCG, maxit=m,
+                                         check_every=4, fsai_product=False, split_diag=False,
+                                         symmetric=False).iterations for m in (7, 8, 9)]
+        scale = np.max(np.abs(ref.x))
+        assert np.max(np.abs(X[j] - ref.x)) <= 1e-9 * scale
+
+
 def test_block_columns_converge_independently(gpu):
     A = G.poisson7(12)
     Bm = np.stack([O.csr_spmv(A, G.uniform(5, A.n_rows)), np.zeros(A.n_rows) + 1e-3,
