@@ -356,8 +356,9 @@ def batching_legs(I, G, O, S):
     out["cfg2_pcg_fp64_tuned"] = sol
     log(f"cfg2 pcg: {out['cfg2_pcg_fp64']['seconds'] * 1e3:.1f} ms, {sol['iterations']} it; with the "
         f"autotuned A plan {sol['seconds'] * 1e3:.1f} ms")
-    # cfg4a: block PCG (SpMM over k columns, the two factors as SlELL plans) vs k single solves
-    Ap = I.Plan(A, I.F64, elm_type=I.F64)
+    # cfg4a: block PCG (SpMM over k columns, the two factors as SlELL plans) vs k single solves,
+    # both with A's plan from the exhaustive autotune
+    Ap = I.Plan(A, I.F64, elm_type=I.F64, kind=I.PLAN_EXHAUSTIVE)
     Mp = (I.Plan(M[0], I.F64, elm_type=I.F64, kind=I.PLAN_MIN_BYTES),
           I.Plan(M[1], I.F64, elm_type=I.F64, kind=I.PLAN_MIN_BYTES))
     n = Ap.n_rows_padded
